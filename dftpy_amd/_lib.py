@@ -1,0 +1,115 @@
+"""ctypes binding of ``libofdft_amd.so`` (C ABI declared in include/ofdft_amd.h).
+
+There is no CPU fallback: importing the library fails with ImportError when
+the shared object has not been built, and every call raises OfdftError when
+the C side reports an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libofdft_amd.so")
+HEADER_PATH = os.path.join(_HERE, "..", "include", "ofdft_amd.h")
+
+F64 = 0
+F32 = 1
+
+E_TF, E_VW, E_NL, E_XC, E_H, E_PP, S_VRHO, S_RHO, S_DTHETA = range(9)
+NSUM = 9
+
+CONV_NONE, CONV_HARTREE, CONV_GG, CONV_WT = range(4)
+
+
+class OfdftError(RuntimeError):
+    """Error reported by the HIP library."""
+
+
+class Functional(ctypes.Structure):
+    _fields_ = [
+        ("tf", ctypes.c_int), ("vw", ctypes.c_int), ("wt", ctypes.c_int),
+        ("lda", ctypes.c_int), ("hartree", ctypes.c_int), ("pseudo", ctypes.c_int),
+        ("x", ctypes.c_double), ("y", ctypes.c_double), ("alpha", ctypes.c_double),
+        ("beta", ctypes.c_double), ("rho0", ctypes.c_double),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_ll = ctypes.c_longlong
+_dp = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = {
+    "ofdft_plan_create": (_i, [ctypes.POINTER(_vp), _i, _i, _i, _dp, _i, _i, _vp]),
+    "ofdft_plan_destroy": (_i, [_vp]),
+    "ofdft_plan_set_stream": (_i, [_vp, _vp]),
+    "ofdft_plan_workspace_bytes": (_ll, [_vp]),
+    "ofdft_set_functional": (_i, [_vp, ctypes.POINTER(Functional)]),
+    "ofdft_set_vloc": (_i, [_vp, _vp]),
+    "ofdft_eval": (_i, [_vp, _vp, _vp, _dp]),
+    "ofdft_eval_phi": (_i, [_vp, _vp, _vp, _d, _d, _vp, _vp, _dp]),
+    "ofdft_hvp": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _dp]),
+    "ofdft_fft_r2c": (_i, [_vp, _vp, _vp]),
+    "ofdft_fft_c2r": (_i, [_vp, _vp, _vp]),
+    "ofdft_convolve": (_i, [_vp, _i, _vp, _vp]),
+    "ofdft_dot3": (_i, [_vp, _vp, _vp, _dp]),
+    "ofdft_axpby": (_i, [_vp, _d, _vp, _d, _vp]),
+    "ofdft_lincomb": (_i, [_vp, _d, _vp, _d, _vp, _vp]),
+    "ofdft_residual": (_i, [_vp, _vp, _vp, _d, _vp, _dp]),
+    "ofdft_cg_update": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _dp]),
+    "ofdft_dot_vphip": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "ofdft_convert": (_i, [_vp, _vp, _i, _vp, _ll]),
+    "ofdft_last_error": (ctypes.c_char_p, []),
+    "ofdft_version": (_i, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load the HIP library once; ImportError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libofdft_amd.so is not built (expected at %s); run `python -m dftpy_amd.build_lib`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().ofdft_last_error().decode(errors="replace")
+        raise OfdftError("ofdft error %d: %s" % (rc, msg))
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def darr(n):
+    return (ctypes.c_double * n)()
+
+
+def exported_symbols():
+    """Function names declared in include/ofdft_amd.h."""
+    import re
+    with open(HEADER_PATH) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(ofdft_[a-z0-9_]+)\s*\(", text)))
+
+
+def as_numpy(carr):
+    return np.array(carr[:], dtype=np.float64)

@@ -1,0 +1,178 @@
+// LDS-resident mixed-radix Stockham FFT building blocks for gfx950.
+//
+// A workgroup holds `ntr` complex sequences of length n in LDS (sequence b
+// at lds[b*ld .. b*ld+n)).  Each pass of radix R maps one butterfly to one
+// thread slot: every slot loads its R inputs into registers, the group
+// synchronises, then every slot writes its R outputs (in-place Stockham
+// autosort).  Twiddles come from a per-length table w[k] = exp(-2 pi i k/n)
+// computed on the host in FP64 and stored in the transform precision.
+//
+// Forward = numpy convention exp(-2 pi i kn/N), unnormalised; inverse uses
+// the conjugate table and is also unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ofdft {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T>
+__device__ __forceinline__ cplx<T> rotq(cplx<T> a, bool inv) {
+  return inv ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
+}
+
+// Per-length pass schedule, passed by value to kernels.
+constexpr int kMaxPass = 12;
+struct Pass1D {
+  int n;
+  int npass;
+  int radix[kMaxPass];
+};
+
+// ---------------------------------------------------------------------------
+// small DFT kernels on registers (y_q = sum_r v_r w^{rq}, w = exp(-+2 pi i/R))
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void dft2(cplx<T>* v) {
+  cplx<T> a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft3(cplx<T>* v, bool inv) {
+  const T h = T(0.86602540378443864676);  // sqrt(3)/2
+  cplx<T> s = cadd(v[1], v[2]);
+  cplx<T> d = csub(v[1], v[2]);
+  cplx<T> m = {v[0].x - T(0.5) * s.x, v[0].y - T(0.5) * s.y};
+  cplx<T> r = rotq(cscale(d, h), inv);  // -+ i*h*d
+  v[0] = cadd(v[0], s);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft4(cplx<T>* v, bool inv) {
+  cplx<T> t0 = cadd(v[0], v[2]);
+  cplx<T> t1 = csub(v[0], v[2]);
+  cplx<T> t2 = cadd(v[1], v[3]);
+  cplx<T> t3 = rotq(csub(v[1], v[3]), inv);
+  v[0] = cadd(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[2] = csub(t0, t2);
+  v[3] = csub(t1, t3);
+}
+
+template <typename T>
+__device__ __forceinline__ void dft5(cplx<T>* v, bool inv) {
+  const T c1 = T(0.30901699437494742410);   // cos(2 pi/5)
+  const T c2 = T(-0.80901699437494742410);  // cos(4 pi/5)
+  const T s1 = T(0.95105651629515357212);   // sin(2 pi/5)
+  const T s2 = T(0.58778525229247312917);   // sin(4 pi/5)
+  cplx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  cplx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  cplx<T> a0 = v[0];
+  cplx<T> b1 = {a0.x + c1 * t1.x + c2 * t2.x, a0.y + c1 * t1.y + c2 * t2.y};
+  cplx<T> b2 = {a0.x + c2 * t1.x + c1 * t2.x, a0.y + c2 * t1.y + c1 * t2.y};
+  cplx<T> e1 = {s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y};
+  cplx<T> e2 = {s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y};
+  cplx<T> r1 = rotq(e1, inv);  // -+ i e1
+  cplx<T> r2 = rotq(e2, inv);
+  v[0] = {a0.x + t1.x + t2.x, a0.y + t1.y + t2.y};
+  v[1] = cadd(b1, r1);
+  v[4] = csub(b1, r1);
+  v[2] = cadd(b2, r2);
+  v[3] = csub(b2, r2);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void bfly_load(const cplx<T>* __restrict__ x, int j, int nb, int k, int twstep,
+                                          const cplx<T>* __restrict__ tw, bool inv, cplx<T>* v) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = x[j + r * nb];
+  if (k != 0) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      cplx<T> w = tw[k * r * twstep];
+      if (inv) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+  if constexpr (R == 2) dft2(v);
+  if constexpr (R == 3) dft3(v, inv);
+  if constexpr (R == 4) dft4(v, inv);
+  if constexpr (R == 5) dft5(v, inv);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void bfly_store(cplx<T>* __restrict__ x, int obase, int Ns, const cplx<T>* v) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[obase + r * Ns] = v[r];
+}
+
+// In-place Stockham FFT of `ntr` LDS sequences.  Each thread handles up to KB
+// butterfly slots per pass; the caller sizes blockDim.x so that
+// KB * blockDim.x >= ntr * max_pass(n / R).  Must be called by the whole
+// workgroup (contains barriers).
+template <typename T, int KB>
+__device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const cplx<T>* __restrict__ tw,
+                        bool inv) {
+  const int n = pl.n;
+  int Ns = 1;
+  for (int ip = 0; ip < pl.npass; ++ip) {
+    const int R = pl.radix[ip];
+    const int nb = n / R;
+    const int total = ntr * nb;
+    const int twstep = n / (Ns * R);
+    cplx<T> v[KB][5];
+    int ob[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      const int t = threadIdx.x + kk * blockDim.x;
+      ob[kk] = -1;
+      if (t < total) {
+        const int b = t / nb;
+        const int j = t - b * nb;
+        const int k = j % Ns;
+        const cplx<T>* xs = lds + b * ld;
+        ob[kk] = b * ld + (j - k) * R + k;
+        switch (R) {
+          case 2: bfly_load<T, 2>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          case 3: bfly_load<T, 3>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          case 4: bfly_load<T, 4>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          default: bfly_load<T, 5>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      if (ob[kk] >= 0) {
+        switch (R) {
+          case 2: bfly_store<T, 2>(lds, ob[kk], Ns, v[kk]); break;
+          case 3: bfly_store<T, 3>(lds, ob[kk], Ns, v[kk]); break;
+          case 4: bfly_store<T, 4>(lds, ob[kk], Ns, v[kk]); break;
+          default: bfly_store<T, 5>(lds, ob[kk], Ns, v[kk]); break;
+        }
+      }
+    }
+    __syncthreads();
+    Ns *= R;
+  }
+}
+
+}  // namespace ofdft

@@ -1,0 +1,198 @@
+"""Device plan: one grid, one precision, one stream (wraps ofdft_plan).
+
+All arrays handed to a Plan are torch tensors resident on the GPU, in the
+plan precision, C-contiguous.  The plan checks dtype, device, shape and
+contiguity before every call so that a kernel never sees a buffer of the
+wrong size.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+_PREC = {"fp64": L.F64, "fp32": L.F32}
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("dftpy_amd runs on a ROCm GPU; torch.cuda.is_available() is False")
+
+
+class Plan:
+    """ofdft_plan for an n0 x n1 x n2 grid with lattice rows in bohr."""
+
+    def __init__(self, nr, lattice, precision="fp64", full=False, device=None):
+        require_gpu()
+        self.lib = L.load()
+        if precision not in _PREC:
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self.prec_code = _PREC[precision]
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        self.cdtype = torch.complex128 if precision == "fp64" else torch.complex64
+        self.nr = tuple(int(x) for x in nr)
+        self.nh = self.nr[2] // 2 + 1
+        self.N = int(np.prod(self.nr))
+        self.shape_c = (self.nr[0], self.nr[1], self.nh)
+        self.full = bool(full)
+        self.lattice = np.ascontiguousarray(np.asarray(lattice, dtype=np.float64).reshape(3, 3))
+        self.volume = abs(np.linalg.det(self.lattice))
+        self.dV = self.volume / self.N
+        self._h = ctypes.c_void_p()
+        lat = (ctypes.c_double * 9)(*self.lattice.ravel().tolist())
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            L.check(self.lib.ofdft_plan_create(ctypes.byref(self._h), self.nr[0], self.nr[1], self.nr[2], lat,
+                                               self.prec_code, 1 if self.full else 0, ctypes.c_void_p(stream)))
+        self.functional = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.ofdft_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ------------------------------------------------------------------
+    def _stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(self.lib.ofdft_plan_set_stream(self._h, ctypes.c_void_p(s)))
+
+    def _real(self, t, name):
+        if t is None:
+            return None
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("%s must be a torch tensor" % name)
+        if t.device != self.device or t.dtype != self.dtype:
+            raise TypeError("%s must be %s on %s (got %s on %s)" % (name, self.dtype, self.device, t.dtype, t.device))
+        if t.numel() != self.N or not t.is_contiguous():
+            raise ValueError("%s must be a contiguous field of %d values" % (name, self.N))
+        return L.ptr(t)
+
+    def _realn(self, t, name, n):
+        if t.device != self.device or t.dtype != self.dtype or t.numel() != n or not t.is_contiguous():
+            raise ValueError("%s must be a contiguous %s tensor of %d values on %s" % (name, self.dtype, n, self.device))
+        return L.ptr(t)
+
+    def _cplx(self, t, name):
+        if t.device != self.device or t.dtype != self.cdtype:
+            raise TypeError("%s must be %s on %s" % (name, self.cdtype, self.device))
+        if t.numel() != int(np.prod(self.shape_c)) or not t.is_contiguous():
+            raise ValueError("%s must be a contiguous half spectrum %s" % (name, self.shape_c))
+        return L.ptr(t)
+
+    def empty(self):
+        return torch.empty(self.nr, dtype=self.dtype, device=self.device)
+
+    def zeros(self):
+        return torch.zeros(self.nr, dtype=self.dtype, device=self.device)
+
+    # ------------------------------------------------------------------
+    def set_functional(self, tf=False, vw=False, wt=False, lda=False, hartree=False, pseudo=False, x=1.0, y=1.0,
+                       alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=0.0):
+        f = L.Functional(int(tf), int(vw), int(wt), int(lda), int(hartree), int(pseudo), float(x), float(y),
+                         float(alpha), float(beta), float(rho0))
+        self._stream()
+        L.check(self.lib.ofdft_set_functional(self._h, ctypes.byref(f)))
+        self.functional = dict(tf=tf, vw=vw, wt=wt, lda=lda, hartree=hartree, pseudo=pseudo, x=x, y=y, alpha=alpha,
+                               beta=beta, rho0=rho0)
+
+    def set_vloc(self, vloc):
+        self._stream()
+        L.check(self.lib.ofdft_set_vloc(self._h, self._real(vloc, "vloc")))
+
+    def eval(self, rho, V=None):
+        sums = L.darr(L.NSUM)
+        self._stream()
+        L.check(self.lib.ofdft_eval(self._h, self._real(rho, "rho"), self._real(V, "V"), sums))
+        return L.as_numpy(sums)
+
+    def eval_phi(self, phi, p=None, c=1.0, s=0.0, V=None, aux=None):
+        sums = L.darr(L.NSUM)
+        self._stream()
+        auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
+        L.check(self.lib.ofdft_eval_phi(self._h, self._real(phi, "phi"), self._real(p, "p"), float(c), float(s),
+                                        self._real(V, "V"), auxp, sums))
+        return L.as_numpy(sums)
+
+    def hvp(self, phi, p, V, aux, mu, Ap):
+        out = ctypes.c_double()
+        self._stream()
+        auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
+        L.check(self.lib.ofdft_hvp(self._h, self._real(phi, "phi"), self._real(p, "p"), self._real(V, "V"), auxp,
+                                   float(mu), self._real(Ap, "Ap"), ctypes.byref(out)))
+        return out.value
+
+    def r2c(self, f, out=None):
+        if out is None:
+            out = torch.empty(self.shape_c, dtype=self.cdtype, device=self.device)
+        self._stream()
+        L.check(self.lib.ofdft_fft_r2c(self._h, self._real(f, "in"), self._cplx(out, "out")))
+        return out
+
+    def c2r(self, F, out=None):
+        if out is None:
+            out = self.empty()
+        self._stream()
+        L.check(self.lib.ofdft_fft_c2r(self._h, self._cplx(F, "in"), self._real(out, "out")))
+        return out
+
+    def convolve(self, kind, f, out=None):
+        if out is None:
+            out = self.empty()
+        self._stream()
+        L.check(self.lib.ofdft_convolve(self._h, int(kind), self._real(f, "in"), self._real(out, "out")))
+        return out
+
+    # vector kernels --------------------------------------------------
+    def dot3(self, a, b):
+        out = L.darr(3)
+        self._stream()
+        L.check(self.lib.ofdft_dot3(self._h, self._real(a, "a"), self._real(b, "b"), out))
+        return out[0], out[1], out[2]
+
+    def lincomb(self, alpha, x, beta, y, z):
+        self._stream()
+        L.check(self.lib.ofdft_lincomb(self._h, float(alpha), self._real(x, "x"), float(beta), self._real(y, "y"),
+                                       self._real(z, "z")))
+        return z
+
+    def residual(self, V, phi, mu, res):
+        out = ctypes.c_double()
+        self._stream()
+        L.check(self.lib.ofdft_residual(self._h, self._real(V, "V"), self._real(phi, "phi"), float(mu),
+                                        self._real(res, "res"), ctypes.byref(out)))
+        return out.value
+
+    def cg_update(self, a, p, Ap, d, r):
+        out = ctypes.c_double()
+        self._stream()
+        L.check(self.lib.ofdft_cg_update(self._h, float(a), self._real(p, "p"), self._real(Ap, "Ap"),
+                                         self._real(d, "d"), self._real(r, "r"), ctypes.byref(out)))
+        return out.value
+
+    def dot_vphip(self, V, phi, p):
+        out = ctypes.c_double()
+        self._stream()
+        L.check(self.lib.ofdft_dot_vphip(self._h, self._real(V, "V"), self._real(phi, "phi"), self._real(p, "p"),
+                                         ctypes.byref(out)))
+        return out.value
+
+    def convert_from(self, src, dst=None):
+        """dst (this plan's precision) = src (any float precision), same size."""
+        if dst is None:
+            dst = self.empty()
+        sp = L.F64 if src.dtype == torch.float64 else L.F32
+        if not src.is_contiguous() or src.device != self.device or src.numel() != dst.numel():
+            raise ValueError("convert: source must be a contiguous field on the plan device")
+        self._stream()
+        L.check(self.lib.ofdft_convert(self._h, L.ptr(dst), sp, L.ptr(src), dst.numel()))
+        return dst
+
+    def workspace_bytes(self):
+        return int(self.lib.ofdft_plan_workspace_bytes(self._h))

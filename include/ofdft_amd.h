@@ -1,0 +1,156 @@
+/*
+ * ofdft_amd.h -- C ABI of the MI355X (gfx950) orbital-free DFT hot path.
+ *
+ * The library replaces the numpy arithmetic under DFTpy's functional and
+ * optimizer API (the reference is pure Python; there is no FFI in the
+ * reference, so every entry point below names the Python function whose
+ * arithmetic it performs, as reference file:line under src/dftpy).
+ *
+ * Conventions
+ *  - A plan is bound to one real grid n0 x n1 x n2 (C order, index
+ *    (i0*n1+i1)*n2+i2), one lattice (row-major 3x3, rows = lattice vectors in
+ *    bohr), one precision and one HIP stream.  The half-complex layout is
+ *    n0 x n1 x (n2/2+1) interleaved complex (numpy rfftn layout).
+ *  - Precision OFDFT_F64: every device array is double / double2.
+ *    OFDFT_F32: fields and spectra are float / float2; every energy, norm
+ *    and dot product is accumulated in FP64.
+ *  - All device pointers are owned by the caller (torch tensors); the plan
+ *    owns only its workspace and tables.  Calls are asynchronous on the plan
+ *    stream except those that return host scalars (they synchronise the
+ *    plan stream before returning).
+ *  - Every call returns 0 on success or a negative OFDFT_E* code;
+ *    ofdft_last_error() gives the text of the calling thread's last error.
+ *    No C++ exception crosses this boundary.
+ *  - One plan must not be used from several host threads at once.
+ */
+#ifndef OFDFT_AMD_H
+#define OFDFT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFDFT_F64 0
+#define OFDFT_F32 1
+
+#define OFDFT_OK 0
+#define OFDFT_EINVAL (-1)
+#define OFDFT_EHIP (-2)
+#define OFDFT_ENOMEM (-3)
+#define OFDFT_ESTATE (-4)
+
+/* Slots of the FP64 result array filled by ofdft_eval / ofdft_eval_phi.
+ * Energies are final (dV and functional fractions applied).            */
+#define OFDFT_E_TF 0      /* KEDF-TF            kedf/tf.py:66-80           */
+#define OFDFT_E_VW 1      /* KEDF-VW            kedf/vw.py:84-94           */
+#define OFDFT_E_NL 2      /* KEDF-NL (WT)       kedf/wt.py:123-130         */
+#define OFDFT_E_XC 3      /* XC (LDA)           xc/semilocal_xc.py:302-307 */
+#define OFDFT_E_H 4       /* HARTREE            hartree.py:37-38           */
+#define OFDFT_E_PP 5      /* PSEUDO             pseudo/__init__.py:223-228 */
+#define OFDFT_S_VRHO 6    /* integral V*rho dV (optimization.py:542-544)   */
+#define OFDFT_S_RHO 7     /* integral rho dV                               */
+#define OFDFT_S_DTHETA 8  /* 2*integral V*phi(t)*dphi/dt dV (optimization.py:270-277) */
+#define OFDFT_NSUM 9
+
+typedef struct ofdft_plan_s* ofdft_plan;
+
+/* Functional selection and options.  Replaces the construction of
+ * TotalFunctional(KEDF, XC, HARTREE, PSEUDO) (total_functional.py:49-78)
+ * with KEDF name expansion kedf/__init__.py:27-69 and options :124-164.   */
+typedef struct {
+  int tf;          /* include Thomas-Fermi (kedf/tf.py:66)                */
+  int vw;          /* include von Weizsacker (kedf/vw.py:115)             */
+  int wt;          /* include Wang-Teter nonlocal term (kedf/wt.py:109)   */
+  int lda;         /* include Slater+PZ LDA (xc/semilocal_xc.py:283)      */
+  int hartree;     /* include Hartree (hartree.py:24)                     */
+  int pseudo;      /* include the local pseudopotential (needs vloc)      */
+  double x;        /* TF fraction; also lambda of the WT kernel (kernel.py:333) */
+  double y;        /* vW fraction                                          */
+  double alpha;    /* WT exponents; alpha must equal beta                  */
+  double beta;
+  double rho0;     /* WT kernel density (wt.py:58-73); <=0: kernel unset   */
+} ofdft_functional;
+
+/* Plan lifetime.  Replaces DirectGrid/ReciprocalGrid set-up
+ * (grid.py:26-57, :384-415) and the FFT plan cache (math_utils.py:94-155).
+ * full=0: rfftn/irfftn semantics (grid full=False, field.py:166-170).
+ * full=1: the reference's full complex grid (grid.py:302 default), i.e.
+ * Re(ifftn(M * fftn(f))); realised on the half spectrum with the
+ * Hermitian-symmetrised multiplier (M(G_k) + M(G_-k))/2.                  */
+int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lattice9, int precision, int full,
+                      void* hip_stream);
+int ofdft_plan_destroy(ofdft_plan plan);
+int ofdft_plan_set_stream(ofdft_plan plan, void* hip_stream);
+/* Device bytes of workspace the plan owns. */
+long long ofdft_plan_workspace_bytes(ofdft_plan plan);
+
+/* Functional options; builds the WT kernel table on the device when
+ * f->wt (get_ke_kernel_wt, kedf/wt.py:58-73; WTKernel kernel.py:325-334;
+ * LindhardFunction kernel.py:14-136).                                      */
+int ofdft_set_functional(ofdft_plan plan, const ofdft_functional* f);
+
+/* Local pseudopotential V_loc(r): copies n0*n1*n2 values (plan precision)
+ * into plan storage (LocalPseudo._vreal, pseudo/__init__.py:219-246).      */
+int ofdft_set_vloc(ofdft_plan plan, const void* vloc_dev);
+
+/* One energy+potential evaluation on rho (TotalFunctional.compute,
+ * total_functional.py:107-146).  V_out (device, may be NULL) receives the
+ * total potential; sums_out (host, OFDFT_NSUM doubles) receives the
+ * energies.  Synchronises the stream.                                     */
+int ofdft_eval(ofdft_plan plan, const void* rho, void* V_out, double* sums_out);
+
+/* Evaluation at phi(t) = c*phi + s*p, rho = phi(t)^2 (the trial point of
+ * Optimization.ValueAndDerivative, optimization.py:228-280; p may be NULL).
+ * Also fills OFDFT_S_DTHETA with the derivative along
+ * dphi/dt = c*p - s*phi.  aux_out (device, 2*N, may be NULL) receives the
+ * vW potential and the WT convolution needed by ofdft_hvp.               */
+int ofdft_eval_phi(ofdft_plan plan, const void* phi, const void* p, double c, double s, void* V_out,
+                   void* aux_out, double* sums_out);
+
+/* Exact Hessian-vector product of the constrained TN system at phi:
+ * Ap = (V - mu) p + phi * dV/drho[2 phi p]  -- the quantity
+ * get_direction_TN forms by forward differences (optimization.py:105-115).
+ * V and aux come from ofdft_eval_phi at phi.  *pAp_out = sum(p*Ap).
+ * Synchronises the stream.                                               */
+int ofdft_hvp(ofdft_plan plan, const void* phi, const void* p, const void* V, const void* aux, double mu,
+              void* Ap_out, double* pAp_out);
+
+/* Raw transforms (DirectField.fft field.py:413-430 without the dV factor;
+ * ReciprocalField.ifft field.py:779-804 without the 1/dV factor):
+ * r2c = numpy.fft.rfftn, c2r = numpy.fft.irfftn (1/N normalised).
+ * c2r does not modify its input.                                         */
+int ofdft_fft_r2c(ofdft_plan plan, const void* in_real, void* out_cplx);
+int ofdft_fft_c2r(ofdft_plan plan, const void* in_cplx, void* out_real);
+
+/* Reciprocal-space convolution out = irfftn(M(G) * rfftn(in)) with
+ * M = 4 pi/G^2 (kind 1, Hartree, hartree.py:34-36), G^2 (kind 2, vW,
+ * vw.py:46-49) or the WT kernel (kind 3, wt.py:22).                      */
+#define OFDFT_CONV_NONE 0
+#define OFDFT_CONV_HARTREE 1
+#define OFDFT_CONV_GG 2
+#define OFDFT_CONV_WT 3
+int ofdft_convolve(ofdft_plan plan, int kind, const void* in_real, void* out_real);
+
+/* ---- vector kernels used by the optimizer (optimization.py:92-280) ---- */
+/* out[0] = sum(a*b), out[1] = sum(a*a), out[2] = sum(b*b) in FP64 (host).  */
+int ofdft_dot3(ofdft_plan plan, const void* a, const void* b, double* out3);
+/* y = alpha*x + beta*y */
+int ofdft_axpby(ofdft_plan plan, double alpha, const void* x, double beta, void* y);
+/* z = alpha*x + beta*y (z may alias x or y) */
+int ofdft_lincomb(ofdft_plan plan, double alpha, const void* x, double beta, const void* y, void* z);
+/* res = (V - mu) * phi ; *res2 = sum(res^2)  (optimization.py:323, :469) */
+int ofdft_residual(ofdft_plan plan, const void* V, const void* phi, double mu, void* res, double* res2);
+/* TN CG update: d += a*p; r -= a*Ap; *r2 = sum(r^2) (optimization.py:124-127) */
+int ofdft_cg_update(ofdft_plan plan, double a, const void* p, const void* Ap, void* d, void* r, double* r2);
+/* sum(V*phi*p) (optimization.py:371) */
+int ofdft_dot_vphip(ofdft_plan plan, const void* V, const void* phi, const void* p, double* out);
+/* precision conversion between plans: dst (dst precision) = src (src precision), N values */
+int ofdft_convert(ofdft_plan dst_plan, void* dst, int src_precision, const void* src, long long n);
+
+const char* ofdft_last_error(void);
+int ofdft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1125 @@
+"""FP64 numpy/scipy restatement of DFTpy's density-optimization hot path.
+
+TEST INFRASTRUCTURE ONLY -- see ``oracle/__init__.py``.  Nothing in
+``dftpy_amd`` imports this file; the HIP product has no CPU fallback.
+
+Every function cites the reference file:line (paths relative to the
+reference's ``src/dftpy``) whose arithmetic it restates.  The code is written
+from the reference text; it is not a copy.
+
+Pinning (what checks each function against something other than itself):
+
+==========================  ==================================================
+function                    pinned by (tests/test_oracle_*.py)
+==========================  ==================================================
+Grid / fft / ifft           test_field.py:124-131 and :254-292 analogues
+                            (integral == F[0,0,0], round trip, Gaussian pair)
+tf_functional               closed form: uniform gas, finite difference of E
+lda_functional              PZ81 constants, FD of E, uniform-gas value
+vw_functional               zero for uniform rho; single-cosine closed form
+wt_functional, lindhard     test_functional.py:27 known answer 2.916818700014412
+                            on Al_fde_rho.pp; F(eta->0)=1-lambda limit; FD of E
+hartree_functional          single-cosine closed form E = pi*Omega*d^2/G^2
+LocalPseudo (PME, exact)    test_read_pp.py:12 (1.3497046, atol 1e-4);
+                            test_pme.py PME-vs-exact energies and forces;
+                            brute-force structure factor on a tiny grid
+local_forces (exact, PME)   test_pme.py:32-34 mutual agreement; FD of energy
+Ewald                       test_ewald.py:33-37 (e_corr, e_real, e_rec)
+optimize_rho (TN)           notebook density_optimization.ipynb:174-183 log
+                            (E0 -8.090977710718, final -8.2811389962163 Ha,
+                            mu 0.30115196532053357); test_density_opt.py:46
+                            (GaAs -43.3046 Ha)
+optimize_rho (LBFGS)        converges to the TN energy (parity unpinned
+                            beyond that: no reference test runs LBFGS)
+==========================  ==================================================
+
+Unit constants: ase's CODATA-2014 values Bohr = 0.5291772105638411 Angstrom,
+Ha = 27.21138602436724 eV (ase is not installed; digits are "parity
+unpinned" beyond the fixture tolerances).
+"""
+import numpy as np
+from scipy.interpolate import splrep, splev
+from scipy import special as sp
+from scipy.optimize._dcsrch import DCSRCH
+
+BOHR = 0.5291772105638411  # Angstrom per bohr (ase.units.Bohr)
+HARTREE = 27.21138602436724  # eV per hartree (ase.units.Ha)
+ZERO = 1e-30  # constants.py:85
+
+__all__ = [
+    "BOHR", "HARTREE", "ZERO", "Grid", "Ions", "fft", "ifft",
+    "tf_functional", "vw_functional", "wt_functional", "wt_kernel", "lindhard",
+    "lda_functional", "lda_v2rho2", "hartree_functional", "PseudoTable", "read_recpot",
+    "empty_core_table", "LocalPseudo", "Ewald", "TotalFunctional",
+    "Optimization", "read_qepp", "read_poscar", "ecut2nr", "best_fft_size",
+    "cbspline_Mn",
+]
+
+
+# ---------------------------------------------------------------------------
+# grids and transforms
+# ---------------------------------------------------------------------------
+class Grid:
+    """Direct grid plus the reciprocal quantities the path needs.
+
+    grid.py:26-57 (BaseGrid: dV, nrG), grid.py:384-415 (bg = 2*pi*inv(L).T),
+    grid.py:584-620 (G from fftfreq / rfftfreq), grid.py:523-553 (gg, q,
+    invgg), grid.py:658-695 (half-space mask, serial case), grid.py:462-465
+    (guess_ecut).  ``lattice`` rows are the lattice vectors in bohr.
+    """
+
+    def __init__(self, lattice, nr, full=False):
+        self.lattice = np.asarray(lattice, dtype=np.float64).reshape(3, 3)
+        self.nr = np.asarray(nr, dtype=np.int64).reshape(3)
+        self.full = bool(full)
+        self.volume = abs(np.linalg.det(self.lattice))
+        self.nnr = int(np.prod(self.nr))
+        self.dV = self.volume / self.nnr
+        self.nrG = self.nr.copy()
+        if not self.full:
+            self.nrG[2] = self.nr[2] // 2 + 1
+        self.bg = (2.0 * np.pi * np.linalg.inv(self.lattice)).T
+        self._g = None
+        self._gg = None
+        self._mask = None
+
+    @property
+    def g(self):
+        if self._g is None:
+            ax = []
+            for i in range(3):
+                n = int(self.nr[i])
+                if i == 2 and not self.full:
+                    ax.append(np.fft.rfftfreq(n, d=1.0 / n))
+                else:
+                    ax.append(np.fft.fftfreq(n, d=1.0 / n))
+            S = np.asarray(np.meshgrid(*ax, indexing="ij"))
+            self._g = np.einsum("j...,jk->k...", S, self.bg)
+        return self._g
+
+    @property
+    def gg(self):
+        if self._gg is None:
+            g = self.g
+            self._gg = np.einsum("lijk,lijk->ijk", g, g)
+        return self._gg
+
+    @property
+    def q(self):
+        return np.sqrt(self.gg)
+
+    @property
+    def invgg(self):
+        gg = self.gg.copy()
+        gg[0, 0, 0] = 1.0
+        inv = 1.0 / gg
+        inv[0, 0, 0] = 0.0
+        return inv
+
+    @property
+    def mask(self):
+        """Half-space mask (grid.py:658-695 with zero offsets)."""
+        if self._mask is None:
+            nr = self.nr
+            D = nr // 2
+            Dmod = nr % 2
+            m = np.ones(tuple(self.nrG), dtype=bool)
+            if self.full:
+                m[:, :, D[2] + 1:] = False
+            m[0, D[1] + 1:, 0] = False
+            m[D[0] + 1:, :, 0] = False
+            if Dmod[2] == 0:
+                m[0, 0, D[2]] = False
+                m[0, D[1] + 1:, D[2]] = False
+                m[D[0] + 1:, :, D[2]] = False
+                if Dmod[1] == 0:
+                    m[0, D[1], D[2]] = False
+                if Dmod[0] == 0:
+                    m[D[0], 0, D[2]] = False
+                    m[D[0], D[1] + 1:, D[2]] = False
+            if Dmod[0] == 0:
+                m[D[0], D[1] + 1:, 0] = False
+                if Dmod[1] == 0:
+                    m[D[0], D[1], 0] = False
+            if Dmod[1] == 0:
+                m[0, D[1], 0] = False
+            if np.all(Dmod == 0):
+                m[D[0], D[1], D[2]] = False
+            self._mask = m
+        return self._mask
+
+    def lengths(self):
+        return np.sqrt((self.lattice ** 2).sum(axis=1))
+
+    def guess_ecut(self):
+        sp1 = self.lengths() / self.nr
+        sp2 = self.lengths() / (self.nr - 1)
+        s = 0.5 * (sp1 + sp2)
+        return np.pi ** 2 / (2 * s.max() ** 2)
+
+    def g2max(self):
+        return 2.0 * self.guess_ecut()
+
+
+def fft(f, grid):
+    """DirectField.fft: field.py:413-430 (x dV), backend field.py:159-172."""
+    if grid.full:
+        return np.fft.fftn(f) * grid.dV
+    return np.fft.rfftn(f) * grid.dV
+
+
+def ifft(F, grid):
+    """ReciprocalField.ifft(force_real=True): field.py:779-804 (/ dV)."""
+    if grid.full:
+        return np.fft.ifftn(F).real / grid.dV
+    return np.fft.irfftn(F, s=tuple(grid.nr), axes=(0, 1, 2)) / grid.dV
+
+
+def power_int(x, num, den):
+    """math_utils.py:158-173 (repeated multiply then root)."""
+    y = x.copy()
+    for _ in range(num - 1):
+        y = y * x
+    if den == 1:
+        return y
+    if den == 2:
+        return np.sqrt(y)
+    if den == 3:
+        return np.cbrt(y)
+    if den == 4:
+        return np.sqrt(np.sqrt(y))
+    return y ** (1.0 / den)
+
+
+# ---------------------------------------------------------------------------
+# functionals (each returns (energy, potential); energy/potential None when
+# not requested)
+# ---------------------------------------------------------------------------
+def tf_functional(rho, grid, x=1.0, calc=("E", "V")):
+    """Thomas-Fermi: kedf/tf.py:13-35, :66-80."""
+    E = V = None
+    if "E" in calc:
+        ed = power_int(rho, 5, 3) * ((3.0 / 10.0) * (3.0 * np.pi ** 2) ** (2.0 / 3.0))
+        E = ed.sum() * grid.dV * x
+    if "V" in calc:
+        fac = (3.0 / 10.0) * (5.0 / 3.0) * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+        V = fac * np.cbrt(rho * rho) * x
+    return E, V
+
+
+def vw_potential(rho, grid, phi=None, lphi=False):
+    """kedf/vw.py:29-53 (sigma=None)."""
+    rhom = np.where(rho > 0, rho, ZERO)
+    s = phi.copy() if (lphi and phi is not None) else np.sqrt(rhom)
+    a = ifft(fft(s, grid) * grid.gg, grid) * 0.5
+    s = np.where(np.abs(s) < ZERO, ZERO, s)
+    return a / s
+
+
+def vw_functional(rho, grid, y=1.0, calc=("E", "V"), phi=None, lphi=False):
+    """kedf/vw.py:84-94, :115-131."""
+    pot = vw_potential(rho, grid, phi=phi, lphi=lphi)
+    E = V = None
+    if "E" in calc:
+        m = rho > 0
+        E = np.einsum("i,i->", rho[m], pot[m]) * grid.dV * y
+    if "V" in calc:
+        V = pot * y
+    return E, V
+
+
+def lindhard(eta, lbda, mu, tol=1e-10):
+    """Inverse Lindhard function with its four branches: kedf/kernel.py:14-136.
+
+    Branch precedence follows the reference's successive masked assignments:
+    normal, then eta<tol, then |eta-1|<tol, then eta>3.65.
+    """
+    eta = np.asarray(eta, dtype=np.float64)
+    tol2 = tol / 10
+    c0 = (eta > tol2) & (np.abs(eta - 1.0) > tol2)
+    c1 = eta < tol
+    c2 = np.abs(eta - 1.0) < tol
+    c3 = eta > 3.65
+    out = eta.copy()
+    e = eta[c0]
+    out[c0] = 1.0 / (0.5 + 0.25 * (1.0 - e ** 2) * np.log((1.0 + e) / np.abs(1.0 - e)) / e) \
+        - 3.0 * mu * e ** 2 - lbda
+    e = eta[c1]
+    out[c1] = 1.0 - lbda + e ** 2 * (1.0 / 3.0 - 3.0 * mu)
+    e = eta[c2]
+    out[c2] = 2.0 - lbda - 3.0 * mu + 20.0 * (e - 1.0)
+    e = eta[c3]
+    out[c3] = _lindhard_large(e, lbda, mu)
+    return out
+
+
+# Coefficients of the large-eta asymptotic series of the inverse Lindhard
+# function in powers of 1/eta^2 (the values the reference tabulates in
+# kedf/kernel.py:80-120).
+_LARGE_COEF = (
+    -0.13714285714285712, -6.39999999999999875e-2, -3.77825602968460128e-2,
+    -2.51824061652633074e-2, -1.80879839616166146e-2, -1.36715733124818332e-2,
+    -1.07236045520990083e-2, -8.65192783339199453e-3, -7.1372762502456763e-3,
+    -5.9945117538835746e-3, -5.10997527675418131e-3, -4.41060829979912465e-3,
+    -3.84763737842981233e-3, -3.38745061493813488e-3, -3.00624946457977689e-3,
+)
+
+
+def _lindhard_large(eta, lbda, mu):
+    t = 1.0 / eta ** 2
+    acc = _LARGE_COEF[-1]
+    for c in _LARGE_COEF[-2::-1]:
+        acc = c + t * acc
+    return 3.0 * (1.0 - mu) * eta ** 2 - lbda - 0.6 + t * acc
+
+
+def wt_kernel(q, rho0, x=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0):
+    """WTKernel: kedf/kernel.py:325-334 (vW fraction fixed at 1)."""
+    tkf = 2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0)
+    ctf = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+    factor = 5.0 / (9.0 * alpha * beta * rho0 ** (alpha + beta - 5.0 / 3.0))
+    factor *= ctf
+    return lindhard(q / tkf, x, 1.0) * factor
+
+
+class KernelCache(dict):
+    """ke_kernel_saved semantics: kedf/__init__.py:101-111, wt.py:58-73."""
+
+    def __init__(self):
+        super().__init__(Kernel=None, rho0=0.0, shape=None)
+
+
+def wt_functional(rho, grid, x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0,
+                  rho0=None, calc=("E", "V"), cache=None):
+    """Wang-Teter nonlocal term: kedf/wt.py:12-56, :58-73, :109-132."""
+    if rho0 is None:
+        rho0 = rho.mean()
+    if cache is None:
+        cache = KernelCache()
+    if abs(cache["rho0"] - rho0) > 1e-6 or rho.shape != cache["shape"]:
+        cache["Kernel"] = wt_kernel(grid.q, rho0, x=x, alpha=alpha, beta=beta)
+        cache["rho0"] = rho0
+        cache["shape"] = rho.shape
+    K = cache["Kernel"]
+    rhoz = np.where(rho < ZERO, ZERO, rho)
+    E = V = None
+    same = abs(beta - alpha) < 1e-9
+    if "V" in calc:
+        if same:
+            rb = rhoz ** beta
+            V = 2.0 * alpha * (rb / rhoz) * ifft(fft(rb, grid) * K, grid)
+        else:
+            V = alpha * rhoz ** (alpha - 1.0) * ifft(fft(rhoz ** beta, grid) * K, grid)
+            V = V + beta * rhoz ** (beta - 1.0) * ifft(fft(rhoz ** alpha, grid) * K, grid)
+    if "E" in calc:
+        if same and V is not None:
+            ed = V * rho / (2 * alpha)
+        else:
+            rb = rhoz ** beta
+            ra = rb if same else rhoz ** alpha
+            ed = ifft(fft(rb, grid) * K, grid) * ra
+        E = ed.sum() * grid.dV
+    return E, V
+
+
+_PZ = dict(a=0.0311, b=-0.048, c=0.0020, d=-0.0116, gamma=-0.1423, beta1=1.0529, beta2=0.3334)
+
+
+def lda_functional(rho, grid, calc=("E", "V")):
+    """Slater exchange + Perdew-Zunger correlation: xc/semilocal_xc.py:283-320."""
+    a, b, c, d = _PZ["a"], _PZ["b"], _PZ["c"], _PZ["d"]
+    gm, b1, b2 = _PZ["gamma"], _PZ["beta1"], _PZ["beta2"]
+    rc = np.cbrt(rho)
+    rc = np.where(rc < 1e-30, 1e-30, rc)
+    rs = np.cbrt(3.0 / (4.0 * np.pi)) / rc
+    lo = rs < 1
+    hi = ~lo
+    E = V = None
+    if "E" in calc:
+        ex = -3.0 / 4.0 * np.cbrt(3.0 / np.pi) * rc
+        r = rs[lo]
+        ex[lo] += a * np.log(r) + b + c * r * np.log(r) + d * r
+        r = rs[hi]
+        ex[hi] += gm / (1.0 + b1 * np.sqrt(r) + b2 * r)
+        E = np.einsum("ijk,ijk->", ex, rho) * grid.dV
+    if "V" in calc:
+        V = np.cbrt(-3.0 / np.pi) * rc
+        r = rs[lo]
+        V[lo] += np.log(r) * (a + 2.0 / 3 * c * r) + b - 1.0 / 3 * a + 1.0 / 3 * (2 * d - c) * r
+        r = rs[hi]
+        sr = np.sqrt(r)
+        V[hi] += (gm + (7.0 / 6.0 * gm * b1) * sr + (4.0 / 3.0 * gm * b2 * r)) / (1.0 + b1 * sr + b2 * r) ** 2
+    return E, V
+
+
+def lda_v2rho2(rho):
+    """dV_xc/drho of the same LDA ('V2' branch): xc/semilocal_xc.py:321-333."""
+    a, c, d = _PZ["a"], _PZ["c"], _PZ["d"]
+    gm, b1, b2 = _PZ["gamma"], _PZ["beta1"], _PZ["beta2"]
+    rc = np.cbrt(rho)
+    rcs = np.where(rc < 1e-30, 1e-30, rc)
+    rs = np.cbrt(3.0 / (4.0 * np.pi)) / rcs
+    lo = rs < 1
+    hi = ~lo
+    fx = -np.cbrt(3.0 / np.pi) / 3.0 * rc / rho
+    fc = np.empty_like(rho)
+    r = rs[lo]
+    fc[lo] = -a / 3.0 - (c / 9.0 * (np.log(r) * 2.0 + 1.0) + d * 2.0 / 9.0) * r
+    r = rs[hi]
+    ta = b1 * np.sqrt(r)
+    tb = b2 * r
+    den = 1.0 + ta + tb
+    fc[hi] = gm / 36.0 * (5.0 * ta + 7.0 * ta * ta + 8.0 * tb + 16.0 * tb * tb + 21.0 * ta * tb) / den / den / den
+    return fx + fc / rho
+
+
+def hartree_functional(rho, grid, calc=("E", "V")):
+    """hartree.py:24-45."""
+    V = ifft(fft(rho, grid) * grid.invgg * 4 * np.pi, grid)
+    E = np.einsum("ijk,ijk->", V, rho) * grid.dV / 2.0 if "E" in calc else None
+    return E, V
+
+
+# ---------------------------------------------------------------------------
+# ions, pseudopotential tables, structure factors
+# ---------------------------------------------------------------------------
+class Ions:
+    """Minimal ase.Atoms stand-in: positions (bohr, Cartesian), symbols,
+    charges, cell rows (bohr).  get_scaled_positions wraps into [0, 1) as
+    ase's default does (ions.py:5-129 builds on ase.Atoms)."""
+
+    def __init__(self, symbols, positions, cell, charges=None):
+        self.symbols = list(symbols)
+        self.positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+        self.charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+
+    @property
+    def nat(self):
+        return len(self.symbols)
+
+    def set_charges(self, zval):
+        self.charges = np.array([zval[s] for s in self.symbols], dtype=np.float64)
+
+    def get_ncharges(self):
+        return self.charges.sum()
+
+    def get_scaled_positions(self):
+        f = np.linalg.solve(self.cell.T, self.positions.T).T
+        f %= 1.0
+        f %= 1.0
+        return f
+
+    @property
+    def symbols_uniq(self):
+        return np.sort(np.unique(self.symbols))
+
+
+class PseudoTable:
+    """A reciprocal-space radial local PP table (gp, vp, zval): the data a
+    BasePseudo hands ReadPseudo (abstract_pseudo.py:49-111,
+    pseudo/__init__.py:773-814)."""
+
+    def __init__(self, gp, vp, zval):
+        self.gp = np.asarray(gp, dtype=np.float64)
+        self.vp = np.asarray(vp, dtype=np.float64)
+        self.zval = zval
+        # cubic interpolating spline of v(q) without the q=0 point:
+        # pseudo/__init__.py:689-697
+        self.tck = splrep(self.gp[1:], self.vp[1:], k=3)
+
+
+def read_recpot(fname):
+    """CASTEP recpot reader: pseudo/recpot.py:11-41."""
+    with open(fname) as fh:
+        lines = fh.readlines()
+    ibegin = 0
+    iend = None
+    for i, line in enumerate(lines):
+        if "END COMMENT" in line:
+            ibegin = i + 3
+        elif ibegin > 1 and (line.strip() == "1000" or len(line.strip()) == 1):
+            iend = i
+            break
+    if iend is None:
+        raise AttributeError("Error : Check the PP file : {}".format(fname))
+    body = " ".join(l.strip() for l in lines[ibegin:iend])
+    gmax = float(lines[ibegin - 1].strip()) * BOHR
+    v = np.array(body.split()).astype(np.float64) / HARTREE / BOHR ** 3
+    r = np.linspace(0, gmax, num=len(v))
+    zval = round((v[0] - v[1]) * (r[1] ** 2) / (4.0 * np.pi))
+    return PseudoTable(r, v, zval)
+
+
+def empty_core_table(Z, rc, qmax=30.0, npts=2001):
+    """Ashcroft empty-core v(q) = -4 pi Z cos(q rc)/q^2 on linspace(0, qmax,
+    npts); v(0) is the finite recpot-convention value 2 pi Z rc^2.  The zval
+    is recovered the way recpot.py:41 recovers it."""
+    q = np.linspace(0.0, qmax, npts)
+    v = np.empty_like(q)
+    v[1:] = -4.0 * np.pi * Z * np.cos(q[1:] * rc) / q[1:] ** 2
+    v[0] = 2.0 * np.pi * Z * rc ** 2
+    zval = round((v[0] - v[1]) * (q[1] ** 2) / (4.0 * np.pi))
+    return PseudoTable(q, v, zval)
+
+
+def quartic_interpolation(f, dx):
+    """math_utils.py:390-400."""
+    t0 = f[2]
+    t1 = 2.0 * f[0] - 16.0 * f[1] + 16.0 * f[3] - 2.0 * f[4]
+    t2 = -f[0] + 16.0 * f[1] - 30.0 * f[2] + 16.0 * f[3] - f[4]
+    t3 = -2.0 * f[0] + 4.0 * f[1] - 4.0 * f[3] + 2.0 * f[4]
+    t4 = f[0] - 4.0 * f[1] + 6.0 * f[2] - 4.0 * f[3] + f[4]
+    return t0 + dx * (t1 + dx * (t2 + dx * (t3 + dx * t4))) / 24.0
+
+
+def vline(q, table):
+    """calc_vlines for one species: pseudo/__init__.py:398-423."""
+    out = np.zeros_like(q)
+    m = q < table.gp[-1]
+    if m.any():
+        out[m] = splev(q[m], table.tck, der=0)
+    m = q < table.gp[1]
+    vp = table.vp
+    dp = table.gp[1] - table.gp[0]
+    out[m] = quartic_interpolation([vp[2], vp[1], vp[0], vp[1], vp[2]], q[m] / dp)
+    return out
+
+
+def cbspline_Mn(x, order):
+    """Cardinal B-spline values M_n(x + k), k = 0..order-1 stored at
+    index 1..order: ewald.py:63-82."""
+    Mn = np.zeros(order + 1)
+    Mn[1] = x
+    Mn[2] = 1.0 - x
+    for i in range(3, order + 1):
+        for j in range(0, i):
+            n = i - j
+            Mn[n] = (x + n - 1) * Mn[n] + (j + 1 - x) * Mn[n - 1]
+            Mn[n] /= i - 1
+    return Mn
+
+
+def _cbspline_Mn_partial(x, order_total, order):
+    """calc_Mn with a lower recursion order but the array sized for
+    ``order_total`` (ewald.py:63-82 called with order=BsplineOrder-1)."""
+    Mn = np.zeros(order_total + 1)
+    Mn[1] = x
+    Mn[2] = 1.0 - x
+    for i in range(3, order + 1):
+        for j in range(0, i):
+            n = i - j
+            Mn[n] = (x + n - 1) * Mn[n] + (j + 1 - x) * Mn[n - 1]
+            Mn[n] /= i - 1
+    return Mn
+
+
+class CBspline:
+    """PME B-spline machinery: ewald.py:11-194."""
+
+    def __init__(self, grid, order=10):
+        self.grid = grid
+        self.order = order
+        self._Barray = None
+
+    def bm(self):
+        """ewald.py:84-101."""
+        M1 = cbspline_Mn(1.0, self.order)
+        out = []
+        for i in range(3):
+            q = 2.0 * np.pi * np.arange(self.grid.nrG[i]) / self.grid.nr[i]
+            t = np.exp(-1j * (self.order - 1.0) * q)
+            f = np.zeros_like(t)
+            for k in range(1, self.order):
+                f += M1[k] * np.exp(-1j * k * q)
+            out.append(t / f)
+        return out
+
+    @property
+    def Barray(self):
+        """ewald.py:47-55."""
+        if self._Barray is None:
+            b = self.bm()
+            self._Barray = np.einsum("i,j,k->ijk", b[0], b[1], b[2])
+        return self._Barray
+
+    def spread(self, scaled_positions, weights, Q=None):
+        """Q-array spread: ewald.py:137-194 (get_PME_Qarray / _calc_PME_Qarray)."""
+        nr = self.grid.nr
+        if Q is None:
+            Q = np.zeros(tuple(nr))
+        o = self.order
+        ix = np.mgrid[:o, :o, :o].reshape((3, -1))
+        for s, w in zip(scaled_positions, weights):
+            Up = s * nr
+            Mn = [cbspline_Mn(Up[j] - np.floor(Up[j]), o) for j in range(3)]
+            val = np.einsum("i,j,k->ijk", w * Mn[0][1:], Mn[1][1:], Mn[2][1:]).ravel()
+            l = np.mod(1 + np.floor(Up).astype(np.int32).reshape((3, 1)) - ix, nr.reshape((3, 1)))
+            np.add.at(Q, (l[0], l[1], l[2]), val)
+        return Q
+
+
+class LocalPseudo:
+    """Local pseudopotential term: pseudo/__init__.py:80-103, :219-246,
+    :398-452 (V_loc), :505-571 (forces)."""
+
+    def __init__(self, grid, ions, tables, PME=True, order=10):
+        self.grid = grid
+        self.ions = ions
+        self.tables = dict(tables)
+        self.PME = PME
+        self.order = order
+        for s in ions.symbols_uniq:
+            if s not in self.tables:
+                raise ValueError("There is no pseudopotential for {:4s} atom".format(s))
+        ions.set_charges({k: t.zval for k, t in self.tables.items()})
+        self.bspline = CBspline(grid, order)
+        self._vreal = None
+        self._vlines = None
+
+    @property
+    def vlines(self):
+        if self._vlines is None:
+            q = self.grid.q
+            self._vlines = {k: vline(q, self.tables[k]) for k in sorted(self.tables)}
+        return self._vlines
+
+    def strf(self, i, sign=-1.0):
+        """ions.py:121-129."""
+        return np.exp(sign * 1j * np.einsum("lijk,l->ijk", self.grid.g, self.ions.positions[i]))
+
+    def v_of_g(self):
+        g = self.grid
+        v = np.zeros(tuple(g.nrG), dtype=np.complex128)
+        if self.PME:
+            sc = self.ions.get_scaled_positions()
+            for key in sorted(self.tables):
+                idx = [i for i, s in enumerate(self.ions.symbols) if s == key]
+                if not idx:
+                    continue
+                Q = self.bspline.spread(sc[idx], np.ones(len(idx)))
+                v = v + self.vlines[key] * fft(Q, g)
+            v = v * self.bspline.Barray * g.nnr / g.volume
+        else:
+            for key in sorted(self.tables):
+                for i, s in enumerate(self.ions.symbols):
+                    if s == key:
+                        v += self.vlines[key] * self.strf(i)
+        return v
+
+    @property
+    def vreal(self):
+        if self._vreal is None:
+            self._vreal = ifft(self.v_of_g(), self.grid)
+        return self._vreal
+
+    def compute(self, rho, calc=("E", "V")):
+        V = self.vreal
+        E = np.einsum("ijk,ijk->", V, rho) * self.grid.dV if "E" in calc else None
+        return E, V
+
+    def forces(self, rho):
+        return self._forces_pme(rho) if self.PME else self._forces_exact(rho)
+
+    def _forces_exact(self, rho):
+        """pseudo/__init__.py:505-521."""
+        g = self.grid
+        rhoG = fft(rho, g)
+        m = g.mask
+        gv = g.g
+        F = np.zeros((self.ions.nat, 3))
+        for i in range(self.ions.nat):
+            den = self.vlines[self.ions.symbols[i]][m] * (rhoG[m] * self.strf(i, +1.0)[m]).imag
+            for j in range(3):
+                F[i, j] = np.einsum("i,i->", gv[j][m], den)
+        return F * 2.0 / g.volume
+
+    def _forces_pme(self, rho):
+        """pseudo/__init__.py:523-571."""
+        g = self.grid
+        o = self.order
+        rhoG = fft(rho, g)
+        denG = rhoG * np.conjugate(self.bspline.Barray)
+        nr = g.nr
+        cell_inv = np.linalg.inv(g.lattice)  # == reciprocal lattice.T / (2 pi)
+        F = np.zeros((self.ions.nat, 3))
+        ix = np.mgrid[:o, :o, :o].reshape((3, -1))
+        sc = self.ions.get_scaled_positions()
+        for key in self.ions.symbols_uniq:
+            dg = denG * self.vlines[key]
+            dg[0, 0, 0] = 0.0
+            rpb = ifft(dg, g)
+            for i in range(self.ions.nat):
+                if self.ions.symbols[i] != key:
+                    continue
+                Up = sc[i] * nr
+                fr = Up - np.floor(Up)
+                Mn = [cbspline_Mn(fr[j], o) for j in range(3)]
+                M2 = [_cbspline_Mn_partial(fr[j], o, o - 1) for j in range(3)]
+                dQ = np.zeros((3, o ** 3))
+                dQ[0] = nr[0] * np.einsum("i,j,k->ijk", M2[0][1:] - M2[0][:-1], Mn[1][1:], Mn[2][1:]).ravel()
+                dQ[1] = nr[1] * np.einsum("i,j,k->ijk", Mn[0][1:], M2[1][1:] - M2[1][:-1], Mn[2][1:]).ravel()
+                dQ[2] = nr[2] * np.einsum("i,j,k->ijk", Mn[0][1:], Mn[1][1:], M2[2][1:] - M2[2][:-1]).ravel()
+                l = np.mod(1 + np.floor(Up).astype(np.int32).reshape((3, 1)) - ix, nr.reshape((3, 1)))
+                F[i] = -np.sum(np.matmul(dQ.T, cell_inv) * rpb[l[0], l[1], l[2]][:, None], axis=0)
+        return F
+
+
+class Ewald:
+    """Ion-ion energy: ewald.py:197-278 (eta), :280-322 (real-space sum),
+    :448-482 (reciprocal and correction terms), :785-807 (PME reciprocal)."""
+
+    def __init__(self, grid, ions, PME=True, precision=1.0e-8, order=10):
+        self.grid = grid
+        self.ions = ions
+        self.PME = PME
+        self.precision = precision
+        self.order = order
+        gmax = np.sqrt(grid.g2max())
+        self.eta = self.best_eta(precision, gmax)
+        self._energy = None
+
+    def best_eta(self, precision, gmax):
+        """ewald.py:256-278."""
+        q2 = np.sum(self.ions.charges * self.ions.charges)
+        eta = 1.6
+        while True:
+            ub = 4.0 * np.pi * self.ions.nat * q2 * np.sqrt(eta / np.pi) * sp.erfc(gmax / 2.0 * np.sqrt(1.0 / eta))
+            if ub < precision:
+                return eta
+            eta = eta - 0.01
+
+    def energy_corr(self):
+        """ewald.py:468-482."""
+        dc = -np.sqrt(self.eta / np.pi) * np.sum(self.ions.charges ** 2)
+        gz = -4.0 * np.pi * (1.0 / (4.0 * self.eta * self.grid.volume) / 2.0) * self.ions.get_ncharges() ** 2
+        return dc + gz
+
+    def energy_real(self):
+        """Pair sum with cutoff rmax = erfcinv(precision/3)/sqrt(eta) over
+        all periodic images (ewald.py:280-322)."""
+        lat = self.grid.lattice
+        L = np.sqrt((lat ** 2).sum(axis=1))
+        rmax = sp.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+        N = np.ceil(rmax / L).astype(int)
+        T = np.mgrid[-N[0]:N[0] + 1, -N[1]:N[1] + 1, -N[2]:N[2] + 1].reshape(3, -1).T.astype(np.float64)
+        shifts = T @ lat
+        pos = self.ions.positions
+        q = self.ions.charges
+        es = np.sqrt(self.eta)
+        tot = 0.0
+        for i in range(self.ions.nat):
+            img = (pos[None, :, :] - shifts[:, None, :]).reshape(-1, 3)
+            qq = np.tile(q, len(shifts))
+            d = np.sqrt(((img - pos[i]) ** 2).sum(axis=1))
+            m = (d < rmax) & (d > 0.001)
+            tot += q[i] * np.sum(qq[m] * sp.erfc(es * d[m]) / d[m])
+        return tot / 2.0
+
+    def energy_rec(self):
+        """ewald.py:448-465 (exact structure factor)."""
+        g = self.grid
+        S = np.zeros(tuple(g.nrG), dtype=np.complex128)
+        for i in range(self.ions.nat):
+            S += np.exp(-1j * np.einsum("lijk,l->ijk", g.g, self.ions.positions[i])) * self.ions.charges[i]
+        m = g.mask
+        e = np.sum((np.conjugate(S) * S)[m] * np.exp(-g.gg[m] / (4.0 * self.eta)) * g.invgg[m])
+        return 4.0 * np.pi * e.real / g.volume
+
+    def energy_rec_pme(self):
+        """ewald.py:785-807."""
+        g = self.grid
+        bs = CBspline(g, self.order)
+        Q = bs.spread(self.ions.get_scaled_positions(), self.ions.charges)
+        S = fft(Q, g) * bs.Barray
+        m = g.mask
+        e = np.sum((np.conjugate(S) * S)[m] * np.exp(-g.gg[m] / (4.0 * self.eta)) * g.invgg[m])
+        return 4.0 * np.pi * e.real / g.volume / g.dV ** 2
+
+    @property
+    def energy(self):
+        """ewald.py:486-506."""
+        if self._energy is None:
+            rec = self.energy_rec_pme() if self.PME else self.energy_rec()
+            self._energy = self.energy_corr() + self.energy_real() + rec
+        return self._energy
+
+
+# ---------------------------------------------------------------------------
+# total functional
+# ---------------------------------------------------------------------------
+class TotalFunctional:
+    """TotalFunctional.get_energy_potential: total_functional.py:107-146,
+    with the KEDF name expansion of kedf/__init__.py:27-69, :124-191 and the
+    II constant of :134-137.
+
+    kedf: 'TF', 'VW', 'TFVW' (x, y) or 'WT' (TF + vW + WT-NL).
+    """
+
+    def __init__(self, grid, kedf="WT", kedf_options=None, xc=True, hartree=True,
+                 pseudo=None, ewald=None):
+        self.grid = grid
+        self.kedf = kedf.upper()
+        self.opts = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=None)
+        if kedf_options:
+            self.opts.update(kedf_options)
+        self.xc = xc
+        self.hartree = hartree
+        self.pseudo = pseudo
+        self.ewald = ewald
+        self.cache = KernelCache()
+        names = {"TF": ("TF",), "VW": ("VW",), "TFVW": ("TF", "VW"), "X_TF_Y_VW": ("TF", "VW"),
+                 "XTFYVW": ("TF", "VW"), "WT": ("TF", "VW", "WT-NL")}
+        self.knames = names[self.kedf]
+
+    def compute(self, rho, calc=("E", "V"), split=False, phi=None, lphi=False):
+        g = self.grid
+        o = self.opts
+        parts = {}
+        if "TF" in self.knames:
+            parts["KEDF-TF"] = tf_functional(rho, g, x=o["x"], calc=calc)
+        if "VW" in self.knames:
+            parts["KEDF-VW"] = vw_functional(rho, g, y=o["y"], calc=calc, phi=phi, lphi=lphi)
+        if "WT-NL" in self.knames:
+            parts["KEDF-NL"] = wt_functional(rho, g, x=o["x"], y=o["y"], alpha=o["alpha"], beta=o["beta"],
+                                             rho0=o["rho0"], calc=calc, cache=self.cache)
+        if self.xc:
+            parts["XC"] = lda_functional(rho, g, calc=calc)
+        if self.hartree:
+            parts["HARTREE"] = hartree_functional(rho, g, calc=calc)
+        if self.pseudo is not None:
+            parts["PSEUDO"] = self.pseudo.compute(rho, calc=calc)
+        E = None
+        V = None
+        if "E" in calc:
+            E = 0.0
+            for k in parts:
+                E += parts[k][0]
+            if self.ewald is not None:
+                E += self.ewald.energy
+        if "V" in calc:
+            V = np.zeros_like(rho)
+            for k in parts:
+                V = V + parts[k][1]
+        if split:
+            out = {k: v[0] for k, v in parts.items()}
+            if self.ewald is not None:
+                out["II"] = self.ewald.energy
+            return E, V, out
+        return E, V
+
+
+# ---------------------------------------------------------------------------
+# optimizer
+# ---------------------------------------------------------------------------
+class _ObjGrad:
+    """math_utils.py:427-460 (ObjectiveGradient with cached results)."""
+
+    def __init__(self, f):
+        self._f = f
+        self.x = np.inf
+        self.results = []
+
+    def f(self, x):
+        self.x = x
+        self.results = self._f(self.x)
+        return self.results[0]
+
+    def fprime(self, x):
+        if not np.allclose(x, self.x):
+            self.f(x)
+        return self.results[1]
+
+
+class Optimization:
+    """Direct minimization on phi = sqrt(rho): optimization/optimization.py
+    (defaults :64-78, TN :92-148, LBFGS :150-173 and math_utils.py:307-325,
+    orthogonalization :205-226, value/derivative :228-280, main loop
+    :283-525, convergence :527-540, chemical potential :542-544).
+
+    ``evaluator(rho, calc, phi)`` returns (E, V).
+    """
+
+    def __init__(self, evaluator, grid, method="TN", options=None, log=None):
+        self.ev = evaluator
+        self.grid = grid
+        self.method = method
+        self.opts = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200,
+                     "maxls": 30, "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "ncheck": 2, "h0": 1.0}
+        if options:
+            self.opts.update(options)
+        self.log = log if log is not None else []
+        self.nfeval = 0
+
+    def integral(self, f):
+        return f.sum() * self.grid.dV
+
+    def _eval(self, rho, calc, phi):
+        self.nfeval += 1
+        return self.ev(rho, calc, phi)
+
+    def direction_tn(self, res0, phi, mu):
+        direction = np.zeros_like(res0)
+        eps = 1.0e-9
+        res = -res0.copy()
+        p = res.copy()
+        r0 = np.einsum("ijk,ijk->", res, res)
+        rconv = r0 * 0.1
+        best = direction
+        rl = [1e10]
+        it = 0
+        for it in range(self.opts["maxfun"]):
+            phi1 = phi + eps * p
+            rho1 = phi1 * phi1
+            _, V1 = self._eval(rho1, ("V",), phi1)
+            Ap = ((V1 - mu) * phi1 - res0) / eps
+            pAp = np.einsum("ijk,ijk->", p, Ap)
+            if pAp < 0.0:
+                if it == 0:
+                    direction = r0 / pAp * p
+                break
+            alpha = r0 / pAp
+            direction += alpha * p
+            res -= alpha * Ap
+            r1 = np.einsum("ijk,ijk->", res, res)
+            if r1 < min(rl):
+                best = direction.copy()
+            rl.append(r1)
+            if r1 < rconv:
+                break
+            elif r1 > 1000 * min(rl[:-1]):
+                direction = best
+                break
+            elif it > 10 and abs(r0 - r1) < 0.1 * r0:
+                direction = best
+                break
+            beta = r1 / r0
+            r0 = r1
+            p = res + beta * p
+        return direction, it + 1
+
+    def direction_lbfgs(self, res):
+        s, y, rh = self.lb_s, self.lb_y, self.lb_rho
+        q = -res
+        al = np.zeros(len(s))
+        for i in range(len(s) - 1, 0, -1):  # the reference skips i = 0 (optimization.py:155)
+            a = rh[i] * np.einsum("ijk,ijk->", s[i], q)
+            al[i] = a
+            q = q - a * y[i]
+        if not self.opts["h0"]:
+            gamma = 1.0 if len(s) < 1 else np.einsum("ijk,ijk->", s[-1], y[-1]) / np.einsum("ijk,ijk->", y[-1], y[-1])
+            d = gamma * q
+        else:
+            d = self.opts["h0"] * q
+        for i in range(len(s)):
+            b = rh[i] * np.sum(y[i] * d)
+            d = d + s[i] * (al[i] - b)
+        return d, 1
+
+    def lbfgs_update(self, dx, dg):
+        if len(self.lb_s) > self.opts["maxcor"]:
+            self.lb_s.pop(0)
+            self.lb_y.pop(0)
+            self.lb_rho.pop(0)
+        self.lb_s.append(dx)
+        self.lb_y.append(dg)
+        self.lb_rho.append(1.0 / np.sum(dg * dx))
+
+    def orth_norm(self, p, phi):
+        Ne = self.integral(phi * phi)
+        p = p - (self.integral(p * phi) / Ne) * phi
+        pn = self.integral(p * p)
+        theta = np.sqrt(pn / Ne)
+        return p * np.sqrt(Ne / pn), theta
+
+    def value_deriv(self, phi, p, theta, func=None):
+        newphi = phi * np.cos(theta) + p * np.sin(theta)
+        newrho = newphi * newphi
+        if func is None:
+            func = self._eval(newrho, ("E", "V"), newphi)
+        p2 = p * np.cos(theta) - phi * np.sin(theta)
+        grad = 2.0 * np.einsum("ijk,ijk,ijk->", func[1], newphi, p2) * self.grid.dV
+        return [func[0], grad, newphi, func]
+
+    def line_search(self, obj, f0, theta):
+        """math_utils.py:37-52 -> scipy line_search_wolfe1 /
+        scalar_search_wolfe1 (scipy optimize/_linesearch.py:37-174): Moré-Thuente
+        DCSRCH on theta with c1, c2, amin=0, amax=pi, xtol, maxiter=100.  The
+        reference's current code drops ``alpha0`` into **kwargs, so the first
+        trial is theta=1 ('unit'); option ls_first_step='theta' starts at the
+        running theta instead, which reproduces the saved notebook log."""
+        fc = [0]
+
+        def phi_(s):
+            fc[0] += 1
+            return obj.f(s)
+
+        a1 = theta if self.opts.get("ls_first_step", "unit") == "theta" else 1.0
+        d = DCSRCH(phi_, obj.fprime, self.opts["c1"], self.opts["c2"], self.opts["xtol"], 0.0, np.pi)
+        stp, _, _, _ = d(a1, phi0=f0[0], derphi0=f0[1], maxiter=100)
+        return stp, fc[0]
+
+    def check_converge(self, hist):
+        econv = self.opts["econv"]
+        nc = self.opts["ncheck"]
+        E = hist[-1]
+        if econv is not None:
+            if len(hist) - 1 < nc:
+                return False
+            for i in range(nc):
+                if abs(hist[-2 - i] - E) > econv:
+                    return False
+        return True
+
+    def optimize_rho(self, guess_rho=None, guess_phi=None):
+        if guess_phi is None:
+            rho = guess_rho.copy()
+            phi = np.where(rho > 0, np.sqrt(np.where(rho > 0, rho, 1.0)), 1e-300)
+            phi *= np.sqrt(rho.sum() / (phi * phi).sum())
+        else:
+            phi = guess_phi.copy()
+        rho = phi * phi
+        func = self._eval(rho, ("E", "V"), phi)
+        mu = self.integral(func[1] * rho) / self.integral(rho)
+        residual = (func[1] - mu) * phi
+        resA = [residual]
+        hist = [func[0]]
+        self.log.append((0, func[0], func[0], float(np.sum(residual * residual) * self.grid.dV), 1, 1))
+        theta = 0.1
+        converged = 2
+        if self.method == "LBFGS":
+            self.lb_s, self.lb_y, self.lb_rho = [], [], []
+        for it in range(1, self.opts["maxiter"]):
+            if self.method == "TN":
+                p, nd = self.direction_tn(resA[-1], phi, mu)
+            elif self.method == "LBFGS":
+                p, nd = self.direction_lbfgs(resA[-1])
+            else:
+                raise AttributeError("The %s direction method not implemented." % self.method)
+            p, theta0 = self.orth_norm(p, phi)
+            d0 = np.einsum("ijk,ijk,ijk->", func[1], phi, p) * 2.0
+            if d0 > 0:
+                p, theta0 = self.orth_norm(-resA[-1], phi)
+            theta = min(theta0, theta)
+            f0 = self.value_deriv(phi, p, 0.0, func=func)
+            obj = _ObjGrad(lambda t: self.value_deriv(phi, p, t))
+            stp, nls = self.line_search(obj, f0, theta)
+            if stp is None:
+                converged = 1
+                break
+            theta = stp
+            newphi, func = obj.results[2], obj.results[3]
+            old_phi, phi = phi, newphi
+            rho = phi * phi
+            mu = self.integral(func[1] * rho) / self.integral(rho)
+            residual = (func[1] - mu) * phi
+            resA.append(residual)
+            if self.method == "LBFGS":
+                self.lbfgs_update(phi - old_phi, resA[-1] - resA[-2])
+            hist.append(func[0])
+            self.log.append((it, func[0], hist[-1] - hist[-2], float(np.sum(residual * residual) * self.grid.dV),
+                             nd, nls))
+            if self.check_converge(hist):
+                converged = 0
+                break
+            if len(resA) > 2:
+                resA.pop(0)
+        self.mu = mu
+        self.rho = rho
+        self.phi = phi
+        self.energy = func[0]
+        self.converged = converged
+        return rho
+
+
+# ---------------------------------------------------------------------------
+# file readers and grid-size helpers
+# ---------------------------------------------------------------------------
+def read_qepp(fname, full=True):
+    """QE pp density reader for ibrav=0 files: formats/qepp.py:17-117.
+    Returns (ions, rho, grid); rho is the Fortran-ordered data."""
+    with open(fname) as fh:
+        fh.readline()
+        nrx = [int(x) for x in fh.readline().split()[:3]]
+        fh.seek(0)
+        lines = fh.read().split("\n")
+    head = lines[1].split()
+    nat, ntyp = int(head[6]), int(head[7])
+    sp_line = lines[2].split()
+    ibrav = int(sp_line[0])
+    celldm = np.asarray(sp_line[1:], dtype=float)
+    if ibrav != 0:
+        raise NotImplementedError("only ibrav=0 files are read")
+    at = np.array([[float(x) for x in lines[3 + i].split()] for i in range(3)]) * celldm[0]
+    k = 7
+    atm = []
+    zval = {}
+    for _ in range(ntyp):
+        s = lines[k].split()
+        atm.append(s[1])
+        zval[s[1]] = float(s[2])
+        k += 1
+    syms = []
+    pos = []
+    for _ in range(nat):
+        s = lines[k].split()
+        pos.append([float(x) for x in s[1:4]])
+        syms.append(atm[int(s[4]) - 1])
+        k += 1
+    data = np.array(" ".join(lines[k:]).split(), dtype=np.float64)
+    grid = Grid(at, nrx, full=full)
+    rho = data.reshape(tuple(nrx), order="F")
+    ions = Ions(syms, np.asarray(pos) * celldm[0], at)
+    ions.set_charges(zval)
+    return ions, np.ascontiguousarray(rho), grid
+
+
+def read_poscar(fname):
+    """VASP POSCAR (Direct coordinates) in bohr."""
+    with open(fname) as fh:
+        L = [l for l in fh.read().split("\n")]
+    scale = float(L[1].split()[0])
+    cell = np.array([[float(x) for x in L[2 + i].split()[:3]] for i in range(3)]) * scale / BOHR
+    names = L[5].split()
+    counts = [int(x) for x in L[6].split()]
+    mode = L[7].strip().lower()
+    n = sum(counts)
+    fr = np.array([[float(x) for x in L[8 + i].split()[:3]] for i in range(n)])
+    syms = [s for s, c in zip(names, counts) for _ in range(c)]
+    if mode.startswith("d"):
+        pos = fr @ cell
+    else:
+        pos = fr * scale / BOHR
+    return Ions(syms, pos, cell)
+
+
+def best_fft_size(N, max_prime=13, scale=0.99, even=True):
+    """mpi/mp_serial.py:17-83."""
+    a = int(np.ceil(np.log2(N))) + 1
+    b = int(np.ceil(np.log(N) / np.log(3))) + 1
+    c = int(np.ceil(np.log(N) / np.log(5))) + 1
+    d = int(np.ceil(np.log(N) / np.log(7))) + 1
+    istart = 1 if even else 0
+    mg = np.mgrid[istart:a, :b, :c, :d].reshape(4, -1)
+    arr0 = 2 ** mg[0] * 3 ** mg[1] * 5 ** mg[2] * 7 ** mg[3]
+    if N < 100:
+        arr1 = arr0[np.logical_and(arr0 > N / (max_prime + 1), arr0 < 2 * N)]
+    else:
+        arr1 = arr0[np.logical_and(arr0 > N / (max_prime + 1), arr0 < 1.2 * N)]
+    allv = list(arr1)
+    for item in [11, 13]:
+        allv.extend(arr1 * item)
+    allv = np.asarray(allv)
+    return int(np.min(allv[allv > np.ceil(scale * N) - 1]))
+
+
+def ecut2nr(lattice, ecut=None, spacing=None):
+    """math_utils.py:282-298."""
+    sp_ = np.ones(3) * (np.sqrt(np.pi ** 2 / ecut * 0.5) if spacing is None else spacing)
+    metric = np.dot(lattice, lattice.T)
+    nr = np.zeros(3, dtype=np.int64)
+    for i in range(3):
+        nr[i] = int(np.sqrt(metric[i, i]) / sp_[i])
+        nr[i] += nr[i] % 2
+        nr[i] = best_fft_size(nr[i])
+    return nr

@@ -1,0 +1,156 @@
+"""Parity of the HIP transform and evaluation kernels with numpy / the oracle.
+
+Tolerances: FP64 transforms 1e-12 relative to max|F|; FP32 transforms 2e-6;
+FP64 per-term energies 1e-10 relative, potentials 1e-9 Ha max-abs (the
+FP64 bar of the hot path); FP32 energies 1e-5 relative.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from dftpy_amd import workloads as W
+from tests.conftest import DATA
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(32, 32, 32), (20, 30, 20), (36, 36, 72), (48, 40, 30), (96, 96, 96), (24, 50, 18), (15, 9, 20), (20, 18, 15), (160, 24, 192)]
+
+
+def _plan(nr, lattice, prec="fp64", full=False):
+    from dftpy_amd.plan import Plan
+    return Plan(nr, lattice, precision=prec, full=full)
+
+
+@pytest.mark.parametrize("nr", SHAPES)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_r2c_c2r_match_numpy(nr, prec):
+    print("case r2c/c2r", nr, prec, flush=True)
+    P = _plan(nr, np.eye(3) * 10.0, prec)
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(nr)
+    ft = torch.tensor(f, dtype=P.dtype, device=P.device)
+    F = P.r2c(ft).cpu().numpy()
+    ref = np.fft.rfftn(f)
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    assert np.abs(F - ref).max() <= tol * np.abs(ref).max()
+    g = P.c2r(torch.tensor(ref, dtype=P.cdtype, device=P.device)).cpu().numpy()
+    assert np.abs(g - f).max() <= tol * np.abs(f).max() * 10
+    # c2r must not modify its input
+    Ft = torch.tensor(ref, dtype=P.cdtype, device=P.device)
+    F0 = Ft.clone()
+    P.c2r(Ft)
+    assert torch.equal(Ft, F0)
+
+
+def _al4(full=False, kedf="TFVW", y=1.0 / 9.0):
+    wl = W.build("al4_32")
+    grid = O.Grid(wl["cell"], wl["nr"], full=full)
+    ions = O.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    tabs = {k: O.empty_core_table(*W.EMPTY_CORE[k]) for k in wl["tables"]}
+    pp = O.LocalPseudo(grid, ions, tabs, PME=True)
+    return wl, grid, ions, pp
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_convolve_matches_oracle(kind):
+    wl, grid, ions, pp = _al4()
+    P = _plan(wl["nr"], wl["cell"])
+    rho = wl["rho_ini"]
+    out = P.convolve(kind, torch.tensor(rho, device=P.device)).cpu().numpy()
+    mult = grid.invgg * 4 * np.pi if kind == 1 else grid.gg
+    ref = np.fft.irfftn(np.fft.rfftn(rho) * mult, s=wl["nr"], axes=(0, 1, 2))
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _check_eval(P, grid, tot, rho, tol_e=1e-10, tol_v=1e-9):
+    E, V, parts = tot.compute(rho, split=True)
+    Vg = torch.empty(P.nr, dtype=P.dtype, device=P.device)
+    sums = P.eval(torch.tensor(rho, dtype=P.dtype, device=P.device), Vg)
+    names = {"KEDF-TF": 0, "KEDF-VW": 1, "KEDF-NL": 2, "XC": 3, "HARTREE": 4, "PSEUDO": 5}
+    for k, e in parts.items():
+        if k == "II":
+            continue
+        got = sums[names[k]]
+        print(k, got, e, flush=True)
+        assert abs(got - e) <= tol_e * max(abs(e), 1e-3), (k, got, e)
+    dv = np.abs(Vg.cpu().numpy() - V).max()
+    print("max|dV|", dv)
+    assert dv <= tol_v
+
+
+def test_eval_tfvw_al4_fp64():
+    wl, grid, ions, pp = _al4()
+    tot = O.TotalFunctional(grid, kedf="TFVW", kedf_options=dict(x=1.0, y=1.0 / 9.0), pseudo=pp)
+    P = _plan(wl["nr"], wl["cell"])
+    P.set_functional(tf=True, vw=True, lda=True, hartree=True, pseudo=True, x=1.0, y=1.0 / 9.0)
+    P.set_vloc(torch.tensor(pp.vreal, device=P.device))
+    _check_eval(P, grid, tot, wl["rho_ini"])
+
+
+def test_eval_wt_al4_fp64():
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    tot = O.TotalFunctional(grid, kedf="WT", pseudo=pp)
+    P = _plan(wl["nr"], wl["cell"])
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_vloc(torch.tensor(pp.vreal, device=P.device))
+    _check_eval(P, grid, tot, rho)
+
+
+def test_eval_wt_fixture_full_grid():
+    """Al_fde_rho.pp: non-orthogonal cell, full=True; KEDF WT known answer
+    2.916818700014412 (test_functional.py:27)."""
+    ions, rho, grid = O.read_qepp(os.path.join(DATA, "Al_fde_rho.pp"), full=True)
+    tot = O.TotalFunctional(grid, kedf="WT", xc=True, hartree=True)
+    P = _plan(tuple(grid.nr), grid.lattice, full=True)
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, rho0=rho.mean())
+    _check_eval(P, grid, tot, rho)
+    sums = P.eval(torch.tensor(rho, device=P.device))
+    assert abs(sums[0] + sums[1] + sums[2] - 2.916818700014412) < 1e-10
+
+
+def test_eval_fp32_al4():
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    tot = O.TotalFunctional(grid, kedf="WT", pseudo=pp)
+    E, V, parts = tot.compute(rho, split=True)
+    P = _plan(wl["nr"], wl["cell"], "fp32")
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_vloc(torch.tensor(pp.vreal, dtype=torch.float32, device=P.device))
+    Vg = torch.empty(P.nr, dtype=P.dtype, device=P.device)
+    sums = P.eval(torch.tensor(rho, dtype=torch.float32, device=P.device), Vg)
+    for i, k in enumerate(["KEDF-TF", "KEDF-VW", "KEDF-NL", "XC", "HARTREE", "PSEUDO"]):
+        assert abs(sums[i] - parts[k]) <= 1e-5 * max(abs(parts[k]), 1e-2), (k, sums[i], parts[k])
+    assert np.abs(Vg.cpu().numpy() - V).max() <= 1e-4
+
+
+def test_hvp_matches_central_difference():
+    """Exact HVP vs central difference of (V(phi+hp) - mu)(phi+hp) (FP64)."""
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    tot = O.TotalFunctional(grid, kedf="WT", pseudo=pp)
+    P = _plan(wl["nr"], wl["cell"])
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_vloc(torch.tensor(pp.vreal, device=P.device))
+    phi = np.sqrt(rho)
+    p = np.random.default_rng(5).standard_normal(rho.shape) * phi * 0.1
+    mu = 0.3
+    t = lambda a: torch.tensor(a, device=P.device)
+    V = P.empty()
+    aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+    P.eval_phi(t(phi), None, 1.0, 0.0, V, aux)
+    Ap = P.empty()
+    pAp = P.hvp(t(phi), t(p), V, aux, mu, Ap)
+    h = 1e-5
+    def F(ph):
+        _, Vv = tot.compute(ph * ph, ("V",))
+        return (Vv - mu) * ph
+    fd = (F(phi + h * p) - F(phi - h * p)) / (2 * h)
+    A = Ap.cpu().numpy()
+    err = np.abs(A - fd).max() / np.abs(fd).max()
+    print("hvp rel err", err)
+    assert err < 1e-7
+    assert abs(pAp - np.sum(p * A)) < 1e-9 * abs(pAp)
