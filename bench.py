@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""Benchmark: truncated-Newton density optimization of the 864-atom fcc Al
+cell (6x6x6 conventional cells, sigma = 0.05 A seeded displacements) on a
+192^3 grid with WT + TF + vW + LDA + Hartree + empty-core local PP, on one
+MI355X.
+
+A step is one complete TN solve in mixed precision (FP32 fields and
+transforms with FP64 reductions, then FP64 refinement to econv), from the
+seeded uniform-plus-noise density to convergence.  ``value`` is TN outer
+iterations per second over the timed solves.  The same solve in FP64 is run
+once after the timed region for the mixed/FP64 ratio and the energy error.
+
+Prints one JSON line (rank 0 of one GPU).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="al864_192")
+    ap.add_argument("--econv", type=float, default=1e-8)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-evals", type=int, default=3, help="oracle evaluations timed for the CPU baseline")
+    ap.add_argument("--quiet", action="store_true")
+    return ap.parse_args()
+
+
+def build_system(name, D, W):
+    wl = W.build(name)
+    grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
+    ions = D.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    pp = D.LocalPseudo(grid=grid, ions=ions, PP_list={s: D.empty_core(*W.EMPTY_CORE[s]) for s in wl["tables"]},
+                       PME=True)
+    ke = D.KEDF(wl["kedf"], **wl["kedf_options"])
+    ev = D.TotalFunctional(KEDF=ke, XC=D.XC("LDA"), HARTREE=D.Hartree(), PSEUDO=pp)
+    return wl, grid, ions, ev
+
+
+def solve(D, ev, grid, wl, precision, econv, verbose):
+    import torch
+    rho = D.DirectField(grid=grid, data=wl["rho_ini"])
+    opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
+                         optimization_options={"econv": econv, "precision": precision, "maxiter": 200})
+    opt.verbose = verbose
+    opt.optimize_rho(guess_rho=rho)
+    niter = sum(1 for l in opt.log[1:] if l["prec"] != "switch->fp64")
+    return opt, niter
+
+
+def cpu_baseline(wl, nevals, niter_evals):
+    """Oracle (numpy FP64, single-threaded pocketfft) evaluations on the
+    same 192^3 workload; converted to TN iterations/s with the evaluations
+    per TN iteration of the GPU FP64 solve."""
+    import numpy as np
+    import oracle as O
+    og = O.Grid(wl["cell"], wl["nr"], full=False)
+    oi = O.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    tabs = {s: O.empty_core_table(*__import__("dftpy_amd.workloads", fromlist=["x"]).EMPTY_CORE[s])
+            for s in wl["tables"]}
+    opp = O.LocalPseudo(og, oi, tabs, PME=True)
+    tot = O.TotalFunctional(og, kedf=wl["kedf"], kedf_options=wl["kedf_options"], pseudo=opp)
+    rho = wl["rho_ini"]
+    opp.vreal  # set-up outside the timed sample
+    tot.compute(rho)  # warm-up (kernel table)
+    t0 = time.perf_counter()
+    for _ in range(nevals):
+        tot.compute(rho)
+    dt = time.perf_counter() - t0
+    evals_per_s = nevals / dt
+    return dict(value=evals_per_s / niter_evals, unit="TN iterations/s", cores=1, kind="port",
+                sample="%d FP64 E+V evaluations of the oracle (numpy, 1 thread) on the same 192^3 WT workload in "
+                       "%.1f s = %.4f evals/s, divided by %.2f evaluations per TN iteration (GPU FP64 solve log)"
+                       % (nevals, dt, evals_per_s, niter_evals))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import dftpy_amd as D
+    from dftpy_amd import workloads as W
+
+    verbose = not args.quiet
+    wl, grid, ions, ev = build_system(args.workload, D, W)
+    nat = len(wl["symbols"])
+    # set-up (V_loc, Ewald, plans) outside the timed region
+    ev.ewald.energy
+    for _ in range(args.warmup):
+        solve(D, ev, grid, wl, "mixed", args.econv, False)
+    ev32 = ev.evaluator(grid, torch.float32)
+    ev64 = ev.evaluator(grid, torch.float64)
+    for e in (ev32, ev64):
+        e.P.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, energies, logs = 0, [], []
+    for _ in range(args.steps):
+        opt, n = solve(D, ev, grid, wl, "mixed", args.econv, verbose and not logs)
+        iters += n
+        energies.append(opt.energy)
+        logs.append(opt.log)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = {"fp32": ev32.P.get_profile(), "fp64": ev64.P.get_profile()}
+    for e in (ev32, ev64):
+        e.P.set_profiling(False)
+    value = iters / elapsed
+
+    # dominant kernel (largest total device time in the timed region)
+    best = None
+    for prec, pr in prof.items():
+        for cls, (ms, cnt, by) in pr.items():
+            if cnt and (best is None or ms > best[2]):
+                best = (prec, cls, ms, cnt, by)
+    prec_d, cls_d, ms_d, cnt_d, by_d = best
+    avg_s = ms_d / cnt_d / 1e3
+    bytes_per_launch = by_d / cnt_d
+    achieved = bytes_per_launch / avg_s / 1e9
+    total_dev_ms = sum(ms for pr in prof.values() for ms, _, _ in pr.values())
+    total_bytes = sum(by for pr in prof.values() for _, _, by in pr.values())
+
+    out = {
+        "metric": "TN iterations/sec at 192^3 grid, mixed vs FP64 GPU, at 1e-6 eV/atom",
+        "value": value,
+        "unit": "TN iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic: seeded fcc Al 6x6x6 supercell, sigma=0.05 A displacements, uniform+1% noise density",
+        "config": {"workload": args.workload, "grid": list(wl["nr"]), "atoms": nat, "kedf": "WT+TF+vW",
+                   "xc": "LDA(PZ)", "pp": "empty-core Al (Z=3, rc=1.12 bohr), PME order 10",
+                   "optimizer": "TN, exact HVP", "econv_Ha": args.econv, "precision": "mixed FP32->FP64",
+                   "tn_iterations_per_solve": iters / args.steps},
+        "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (cls_d, prec_d), "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launches": cnt_d, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "all_kernels_GBps": total_bytes / (total_dev_ms / 1e3) / 1e9 if total_dev_ms else None},
+        "profile": {p: {k: [round(v[0], 3), v[1], round(v[0] / v[1] * 1e3, 1) if v[1] else 0] for k, v in pr.items()}
+                    for p, pr in prof.items()},
+    }
+    if not args.no_fp64:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        o64, n64 = solve(D, ev, grid, wl, "fp64", args.econv, verbose)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t1
+        fp64_rate = n64 / t64
+        err = abs(energies[-1] - o64.energy) * W.HARTREE / nat
+        out["fp64_gpu"] = {"value": fp64_rate, "unit": "TN iterations/s", "iterations": n64, "seconds": t64,
+                           "energy_Ha": o64.energy, "converged": o64.converged}
+        out["mixed_over_fp64"] = value / fp64_rate
+        out["time_to_solution_ratio"] = t64 / (elapsed / args.steps)
+        out["mixed_energy_error_eV_per_atom"] = err
+        out["mixed_energy_Ha"] = energies[-1]
+        evals = sum(l["Nd"] + l["Nls"] for l in o64.log[1:]) / max(n64, 1)
+    else:
+        evals = sum(l["Nd"] + l["Nls"] for l in logs[-1][1:]) / max(iters / args.steps, 1)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_evals, evals)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

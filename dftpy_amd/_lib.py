@@ -35,6 +35,14 @@ class Functional(ctypes.Structure):
     ]
 
 
+class VLine(ctypes.Structure):
+    _fields_ = [
+        ("knots", ctypes.POINTER(ctypes.c_double)), ("coefs", ctypes.POINTER(ctypes.c_double)),
+        ("nknots", ctypes.c_int), ("q1", ctypes.c_double), ("dq", ctypes.c_double), ("qmax", ctypes.c_double),
+        ("v0", ctypes.c_double), ("v1", ctypes.c_double), ("v2", ctypes.c_double),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -61,6 +69,11 @@ _SIGS = {
     "ofdft_cg_update": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _dp]),
     "ofdft_dot_vphip": (_i, [_vp, _vp, _vp, _vp, _dp]),
     "ofdft_convert": (_i, [_vp, _vp, _i, _vp, _ll]),
+    "ofdft_vloc_build": (_i, [_vp, _i, ctypes.POINTER(VLine), _i, _dp, ctypes.POINTER(ctypes.c_int), _i, _i, _vp]),
+    "ofdft_local_forces": (_i, [_vp, _vp, _dp]),
+    "ofdft_ewald": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
+    "ofdft_plan_set_profiling": (_i, [_vp, _i]),
+    "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),
     "ofdft_version": (_i, []),
 }

@@ -99,6 +99,30 @@ __device__ __forceinline__ void dft5(cplx<T>* v, bool inv) {
   v[3] = csub(b2, r2);
 }
 
+// 8-point DFT as two 4-point DFTs (even / odd inputs) and a W8 twiddle stage.
+template <typename T>
+__device__ __forceinline__ void dft8(cplx<T>* v, bool inv) {
+  const T h = T(0.70710678118654752440);  // 1/sqrt(2)
+  cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4(e, inv);
+  dft4(o, inv);
+  // W8^1 = (1 -+ i)/sqrt2, W8^2 = -+ i, W8^3 = (-1 -+ i)/sqrt2
+  const cplx<T> o1 = inv ? cplx<T>{h * (o[1].x - o[1].y), h * (o[1].x + o[1].y)}
+                         : cplx<T>{h * (o[1].x + o[1].y), h * (o[1].y - o[1].x)};
+  const cplx<T> o2 = rotq(o[2], inv);
+  const cplx<T> o3 = inv ? cplx<T>{-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y)}
+                         : cplx<T>{h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y)};
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void bfly_load(const cplx<T>* __restrict__ x, int j, int nb, int k, int twstep,
                                           const cplx<T>* __restrict__ tw, bool inv, cplx<T>* v) {
@@ -116,6 +140,7 @@ __device__ __forceinline__ void bfly_load(const cplx<T>* __restrict__ x, int j, 
   if constexpr (R == 3) dft3(v, inv);
   if constexpr (R == 4) dft4(v, inv);
   if constexpr (R == 5) dft5(v, inv);
+  if constexpr (R == 8) dft8(v, inv);
 }
 
 template <typename T, int R>
@@ -138,7 +163,7 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
     const int nb = n / R;
     const int total = ntr * nb;
     const int twstep = n / (Ns * R);
-    cplx<T> v[KB][5];
+    cplx<T> v[KB][8];
     int ob[KB];
 #pragma unroll
     for (int kk = 0; kk < KB; ++kk) {
@@ -154,7 +179,8 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
           case 2: bfly_load<T, 2>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
           case 3: bfly_load<T, 3>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
           case 4: bfly_load<T, 4>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
-          default: bfly_load<T, 5>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          case 5: bfly_load<T, 5>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          default: bfly_load<T, 8>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
         }
       }
     }
@@ -166,7 +192,8 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
           case 2: bfly_store<T, 2>(lds, ob[kk], Ns, v[kk]); break;
           case 3: bfly_store<T, 3>(lds, ob[kk], Ns, v[kk]); break;
           case 4: bfly_store<T, 4>(lds, ob[kk], Ns, v[kk]); break;
-          default: bfly_store<T, 5>(lds, ob[kk], Ns, v[kk]); break;
+          case 5: bfly_store<T, 5>(lds, ob[kk], Ns, v[kk]); break;
+          default: bfly_store<T, 8>(lds, ob[kk], Ns, v[kk]); break;
         }
       }
     }
@@ -174,5 +201,92 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
     Ns *= R;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Compile-time schedules for the common lengths: radices, butterfly counts,
+// strides and twiddle steps become constants and every pass is unrolled.
+// ---------------------------------------------------------------------------
+struct Sched {
+  int np;
+  int r[kMaxPass];
+};
+
+constexpr Sched make_sched(int n) {
+  Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
+  int m = n;
+  while (m % 8 == 0) { s.r[s.np++] = 8; m /= 8; }
+  while (m % 4 == 0) { s.r[s.np++] = 4; m /= 4; }
+  while (m % 2 == 0) { s.r[s.np++] = 2; m /= 2; }
+  while (m % 3 == 0) { s.r[s.np++] = 3; m /= 3; }
+  while (m % 5 == 0) { s.r[s.np++] = 5; m /= 5; }
+  return s;
+}
+
+constexpr int sched_maxnb(int n) {
+  Sched s = make_sched(n);
+  int mx = 0;
+  for (int i = 0; i < s.np; ++i) mx = (n / s.r[i] > mx) ? n / s.r[i] : mx;
+  return mx;
+}
+
+template <typename T, int N, int P, int NS, int NTR, int TH>
+__device__ __forceinline__ void fixed_pass(cplx<T>* lds, int ld, int ntr, const cplx<T>* __restrict__ tw,
+                                           bool inv) {
+  constexpr Sched S = make_sched(N);
+  if constexpr (P < S.np) {
+    constexpr int R = S.r[P];
+    constexpr int nb = N / R;
+    constexpr int twstep = N / (NS * R);
+    constexpr int KB = (NTR * nb + TH - 1) / TH;  // butterfly slots per thread in this pass
+    const int total = ntr * nb;
+    cplx<T> v[KB][R];
+    int ob[KB];
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      const int t = threadIdx.x + kk * TH;
+      ob[kk] = -1;
+      if (t < total) {
+        const int b = t / nb;
+        const int j = t - b * nb;
+        const int k = j % NS;
+        ob[kk] = b * ld + (j - k) * R + k;
+        bfly_load<T, R>(lds + b * ld, j, nb, k, twstep, tw, inv, v[kk]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk)
+      if (ob[kk] >= 0) bfly_store<T, R>(lds, ob[kk], NS, v[kk]);
+    __syncthreads();
+    fixed_pass<T, N, P + 1, NS * R, NTR, TH>(lds, ld, ntr, tw, inv);
+  }
+}
+
+// FFT policy with a compile-time length N for up to NTR sequences per
+// workgroup of TH threads.
+template <int N, int NTR, int TH_>
+struct FFTFixed {
+  static constexpr int TH = TH_;
+  __host__ __device__ static constexpr int n() { return N; }
+  __device__ static int th() { return TH; }
+  template <typename T>
+  __device__ void run(cplx<T>* lds, int ld, int ntr, const cplx<T>* __restrict__ tw, bool inv) const {
+    fixed_pass<T, N, 0, 1, NTR, TH>(lds, ld, ntr, tw, inv);
+  }
+};
+
+// FFT policy for any 2^a 3^b 5^c length, schedule read at run time.
+template <int KB_>
+struct FFTRuntime {
+  static constexpr int KB = KB_;
+  static constexpr int TH = 1024;
+  Pass1D pl;
+  __host__ __device__ int n() const { return pl.n; }
+  __device__ static int th() { return blockDim.x; }
+  template <typename T>
+  __device__ void run(cplx<T>* lds, int ld, int ntr, const cplx<T>* __restrict__ tw, bool inv) const {
+    lds_fft<T, KB>(lds, ld, ntr, pl, tw, inv);
+  }
+};
 
 }  // namespace ofdft

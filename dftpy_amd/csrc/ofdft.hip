@@ -24,6 +24,7 @@
 
 #include "../../include/ofdft_amd.h"
 #include "fft_lds.h"
+#include "fft_reg.h"
 
 using namespace ofdft;
 
@@ -66,6 +67,7 @@ bool make_pass(int n, Pass1D* pl) {
     pl->radix[pl->npass++] = r;
     return true;
   };
+  while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
   while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
   while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
   while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
@@ -118,11 +120,59 @@ struct ofdft_plan_s {
   double* hred;     // pinned host mirror
   ofdft_functional fn;
   int have_fn, have_vloc, have_wtk;
+  // ion state of the last ofdft_vloc_build (local PP forces reuse it)
+  int nspecies, nat, pme, order;
+  double* d_knots;   // all species' knots, concatenated
+  double* d_coefs;   // all species' coefficients, concatenated
+  void* d_splines;   // SplineDev[nspecies]
+  double* d_frac;    // nat x 3 wrapped fractional coordinates
+  double* d_pos;     // nat x 3 Cartesian positions (bohr)
+  int* d_species;    // nat
+  double* d_bm;      // PME b-moduli: complex n0 + n1 + nbm2
+  double* d_work;    // N doubles: PME Q array / rhoPB
+  // per-kernel-class event timing (ofdft_plan_set_profiling)
+  int prof_on;
+  std::vector<hipEvent_t> ev_pool;  // reused across profile windows
+  size_t ev_used = 0;
+  struct ProfRec { int cls; long long bytes; int e0, e1; };
+  std::vector<ProfRec> prof_pending;
+  double prof_ms[8];
+  long long prof_count[8];
+  double prof_bytes[8];
 };
 
 namespace {
 
 constexpr int kMaxSum = 16;
+constexpr int kRowPairsC = 2;   // row pairs per rows_* workgroup
+constexpr int kColTileC = 16;   // W: contiguous columns per axis-1 / axis-0 tile
+
+// kernel classes reported by ofdft_plan_get_profile
+enum { PC_ROWS_FWD = 0, PC_AXIS = 1, PC_PENCIL = 2, PC_ROWS_INV = 3, PC_VEC = 4, PC_NCLS = 5 };
+
+// Event bracket around one launch (profiling only).
+struct ProfScope {
+  ofdft_plan_s* P;
+  int cls;
+  long long bytes;
+  int e0 = -1;
+  ProfScope(ofdft_plan_s* p, int c, long long b) : P(p), cls(c), bytes(b) {
+    if (!P->prof_on) return;
+    while (P->ev_pool.size() < P->ev_used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      P->ev_pool.push_back(e);
+    }
+    e0 = (int)P->ev_used;
+    P->ev_used += 2;
+    (void)hipEventRecord(P->ev_pool[e0], P->stream);
+  }
+  ~ProfScope() {
+    if (e0 < 0) return;
+    (void)hipEventRecord(P->ev_pool[e0 + 1], P->stream);
+    P->prof_pending.push_back({cls, bytes, e0, e0 + 1});
+  }
+};
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -199,6 +249,7 @@ __device__ __forceinline__ void put3(T* out, int f, T v) {
 // ---------------------------------------------------------------------------
 struct FnParams {
   int tf, vw, wt, lda, hartree, pseudo;
+  int beta56;  // beta == 5/6: rho^beta = sqrt(rho) cbrt(rho), no pow
   double x, y, alpha, beta;
   int fH, fV, fW;  // spectrum slot of each convolution field (-1 if absent)
   int F;
@@ -207,10 +258,10 @@ struct FnParams {
 // LDA exchange-correlation energy density per electron and potential:
 // xc/semilocal_xc.py:287-320.
 template <typename T>
-__device__ __forceinline__ void lda_point(T rho, T& ex, T& vxc) {
+__device__ __forceinline__ void lda_point(T rho, T rcbrt, T& ex, T& vxc) {
   const T a = T(0.0311), b = T(-0.048), c = T(0.0020), d = T(-0.0116);
   const T g = T(-0.1423), b1 = T(1.0529), b2 = T(0.3334);
-  T rc = dev_cbrt(rho);
+  T rc = rcbrt;
   if (rc < T(1e-30)) rc = T(1e-30);
   const T rs = T(0.62035049089940001667) / rc;  // cbrt(3/(4 pi)) / cbrt(rho)
   ex = T(-0.73855876638202240588) * rc;         // -3/4 cbrt(3/pi)
@@ -229,10 +280,10 @@ __device__ __forceinline__ void lda_point(T rho, T& ex, T& vxc) {
 
 // d V_xc / d rho: xc/semilocal_xc.py:321-333.
 template <typename T>
-__device__ __forceinline__ T lda_v2(T rho) {
+__device__ __forceinline__ T lda_v2(T rho, T rcbrt) {
   const T a = T(0.0311), c = T(0.0020), d = T(-0.0116);
   const T g = T(-0.1423), b1 = T(1.0529), b2 = T(0.3334);
-  T rc0 = dev_cbrt(rho);
+  T rc0 = rcbrt;
   T rc = rc0 < T(1e-30) ? T(1e-30) : rc0;
   const T rs = T(0.62035049089940001667) / rc;
   const T fx = T(-0.98474502184269654118) / T(3) * rc0 / rho;
@@ -274,6 +325,7 @@ struct GridGeom {
   int n0, n1, n2, nh;
   int full;  // 1: frequencies of the full complex grid (grid.py:584-620, full=True)
   double bg[9];
+  float bgf[9];
 };
 
 __device__ __forceinline__ double gsq_m(const GridGeom& g, double m0, double m1, double m2) {
@@ -305,6 +357,7 @@ __device__ __forceinline__ double gsq(const GridGeom& g, int k0, int k1, int k2,
 template <typename T>
 struct PreCopy {
   const T* __restrict__ in;
+  __host__ __device__ int nin() const { return 1; }
   __device__ __forceinline__ void operator()(long long i, T* out) const { out[0] = in[i]; }
 };
 
@@ -318,6 +371,7 @@ struct PreEval {
   const T* __restrict__ p;
   T c, s;
   FnParams fp;
+  __host__ __device__ int nin() const { return rho ? 1 : (p ? 2 : 1); }
   __device__ __forceinline__ T density(long long i) const {
     if (rho) return rho[i];
     T ph = phi[i];
@@ -327,8 +381,13 @@ struct PreEval {
   __device__ __forceinline__ void operator()(long long i, T* out) const {
     const T r = density(i);
     if (fp.fH >= 0) put3(out, fp.fH, r);
-    if (fp.fV >= 0) put3(out, fp.fV, dev_sqrt(r > T(0) ? r : T(kZero)));
-    if (fp.fW >= 0) put3(out, fp.fW, dev_pow(r < T(kZero) ? T(kZero) : r, T(fp.beta)));
+    const T s = dev_sqrt(r > T(0) ? r : T(kZero));
+    if (fp.fV >= 0) put3(out, fp.fV, s);
+    if (fp.fW >= 0) {
+      const T rz = r < T(kZero) ? T(kZero) : r;
+      const T rb = (fp.beta56 && r >= T(kZero)) ? s * dev_cbrt(rz) : dev_pow(rz, T(fp.beta));
+      put3(out, fp.fW, rb);
+    }
   }
 };
 
@@ -339,14 +398,18 @@ struct PreHvp {
   const T* __restrict__ phi;
   const T* __restrict__ p;
   FnParams fp;
+  __host__ __device__ int nin() const { return 2; }
   __device__ __forceinline__ void operator()(long long i, T* out) const {
     const T ph = phi[i], pp = p[i];
     const T r = ph * ph;
     const T dr = T(2) * ph * pp;
+    const T s = dev_sqrt(r);
     if (fp.fH >= 0) put3(out, fp.fH, dr);
-    if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dr / (T(2) * dev_sqrt(r)) : T(0));
-    if (fp.fW >= 0)
-      put3(out, fp.fW, r < T(kZero) ? T(0) : T(fp.beta) * dev_pow(r, T(fp.beta) - T(1)) * dr);
+    if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dr / (T(2) * s) : T(0));
+    if (fp.fW >= 0) {
+      const T rb1 = fp.beta56 ? dev_cbrt(r) / s : dev_pow(r, T(fp.beta) - T(1));  // rho^(beta-1)
+      put3(out, fp.fW, r < T(kZero) ? T(0) : T(fp.beta) * rb1 * dr);
+    }
   }
 };
 
@@ -358,6 +421,7 @@ template <typename T>
 struct PostStore {
   static constexpr int NS = 1;
   T* __restrict__ out;
+  __host__ __device__ int nio() const { return 1; }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { out[i] = c[0]; }
 };
 
@@ -369,6 +433,10 @@ struct PostEval {
   T* __restrict__ V;
   T* __restrict__ aux;  // [0,N) vW raw potential, [N,2N) WT convolution
   long long N;
+  // fields read (density inputs, V_loc) and written (V, aux)
+  __host__ __device__ int nio() const {
+    return pre.nin() + (pre.fp.pseudo ? 1 : 0) + (V ? 1 : 0) + (aux ? (pre.fp.fV >= 0) + (pre.fp.fW >= 0) : 0);
+  }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
     const FnParams& fp = pre.fp;
     T ph = T(0), dph = T(0);
@@ -383,16 +451,17 @@ struct PostEval {
       r = ph * ph;
     }
     T v = T(0);
+    // one cube root and one square root serve TF, LDA, vW and WT (beta = 5/6)
+    const T cb = dev_cbrt(r);
+    T s = dev_sqrt(r > T(0) ? r : T(kZero));
+    if (s < T(kZero)) s = T(kZero);
     if (fp.tf) {
-      // TF: energy density cbrt(rho^5), potential (5/3)C_TF cbrt(rho^2) (tf.py:13-35)
-      const T r2 = r * r;
-      acc[OFDFT_E_TF] += (double)dev_cbrt(r2 * r2 * r);
-      v += T(4.7853900003136530265) * dev_cbrt(r2) * T(fp.x);
+      // TF: energy density cbrt(rho^5) = rho cbrt(rho)^2, potential (5/3)C_TF cbrt(rho)^2 (tf.py:13-35)
+      acc[OFDFT_E_TF] += (double)(r * cb * cb);
+      v += T(4.7853900003136530265) * (cb * cb) * T(fp.x);
     }
     if (fp.fV >= 0) {
       // vW: 0.5 * IFFT[G^2 FFT(s)] / s, energy over rho > 0 (vw.py:29-53, :93)
-      T s = dev_sqrt(r > T(0) ? r : T(kZero));
-      if (s < T(kZero)) s = T(kZero);
       const T vraw = T(0.5) * pick3(c, fp.fV) / s;
       if (r > T(0)) acc[OFDFT_E_VW] += (double)(r * vraw);
       v += vraw * T(fp.y);
@@ -401,14 +470,15 @@ struct PostEval {
     if (fp.fW >= 0) {
       // WT: 2 alpha rho^beta / rho * conv, energy density V rho/(2 alpha) (wt.py:12-28, :125)
       const T rz = r < T(kZero) ? T(kZero) : r;
-      const T vw = T(2 * fp.alpha) * (dev_pow(rz, T(fp.beta)) / rz) * pick3(c, fp.fW);
+      const T rbm1 = (fp.beta56 && r >= T(kZero)) ? cb / s : dev_pow(rz, T(fp.beta)) / rz;
+      const T vw = T(2 * fp.alpha) * rbm1 * pick3(c, fp.fW);
       acc[OFDFT_E_NL] += (double)(vw * r / T(2 * fp.alpha));
       v += vw;
       if (aux) aux[N + i] = pick3(c, fp.fW);
     }
     if (fp.lda) {
       T ex, vx;
-      lda_point(r, ex, vx);
+      lda_point(r, cb, ex, vx);
       acc[OFDFT_E_XC] += (double)(ex * r);
       v += vx;
     }
@@ -441,21 +511,23 @@ struct PostHvp {
   T mu;
   long long N;
   FnParams fp;
+  __host__ __device__ int nio() const { return 4 + (fp.fV >= 0) + (fp.fW >= 0); }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
     const T ph = phi[i], pp = p[i];
     const T r = ph * ph;
     const T dr = T(2) * ph * pp;
+    const T cb = dev_cbrt(r);
+    const T s = dev_sqrt(r);
     T dv = T(0);
-    if (fp.tf && r > T(0)) dv += T(fp.x) * T(3.1902600002091020177) * dr / dev_cbrt(r);  // (2/3)(5/3)C_TF
-    if (fp.lda && r > T(0)) dv += lda_v2(r) * dr;
+    if (fp.tf && r > T(0)) dv += T(fp.x) * T(3.1902600002091020177) * dr / cb;  // (2/3)(5/3)C_TF
+    if (fp.lda && r > T(0)) dv += lda_v2(r, cb) * dr;
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
     if (fp.fV >= 0 && r > T(0)) {
-      const T s = dev_sqrt(r);
       const T ds = dr / (T(2) * s);
       dv += T(fp.y) * (T(0.5) * pick3(c, fp.fV) / s - aux[i] * ds / s);
     }
     if (fp.fW >= 0 && r >= T(kZero)) {
-      const T rb1 = dev_pow(r, T(fp.beta) - T(1));
+      const T rb1 = fp.beta56 ? cb / s : dev_pow(r, T(fp.beta) - T(1));
       dv += T(2 * fp.alpha) * ((T(fp.beta) - T(1)) * rb1 / r * dr * aux[N + i] + rb1 * pick3(c, fp.fW));
     }
     const T ap = (V[i] - mu) * pp + ph * dv;
@@ -470,14 +542,19 @@ struct PostHvp {
 // Axis-2 real-to-complex: block handles PB row pairs x F fields.  Rows r and
 // r+1 of field f share one complex sequence z = a + i b; after the FFT,
 // A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i).
-template <typename T, int KB, class Pre>
-__global__ void k_rows_fwd(Pre pre, cplx<T>* __restrict__ spec, long long nrows, int n2, int nh, long long Nh,
-                           int F, int PB, Pass1D pl, const cplx<T>* __restrict__ tw) {
+template <typename T, class FFT, class Pre>
+__global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
+                                                      long long nrows, long long Nh, int F,
+                                                      const cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int PB = kRowPairsC;
+  const int n2 = fft.n();
+  const int nh = n2 / 2 + 1;
+  const int th = fft.th();
   const long long pair0 = (long long)blockIdx.x * PB;
   const int ld = n2;
-  for (int t = threadIdx.x; t < PB * n2; t += blockDim.x) {
+  for (int t = threadIdx.x; t < PB * n2; t += th) {
     const int lp = t / n2, j = t - lp * n2;
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
     T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
@@ -487,9 +564,11 @@ __global__ void k_rows_fwd(Pre pre, cplx<T>* __restrict__ spec, long long nrows,
     for (int f = 0; f < 3; ++f)
       if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
   }
+  cplx<T>* ltw = lds + PB * F * ld;
+  for (int t = threadIdx.x; t < n2; t += th) ltw[t] = tw[t];
   __syncthreads();
-  lds_fft<T, KB>(lds, ld, PB * F, pl, tw, false);
-  for (int t = threadIdx.x; t < PB * F * nh; t += blockDim.x) {
+  fft.run(lds, ld, PB * F, ltw, false);
+  for (int t = threadIdx.x; t < PB * F * nh; t += th) {
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
@@ -504,16 +583,21 @@ __global__ void k_rows_fwd(Pre pre, cplx<T>* __restrict__ spec, long long nrows,
 // are combined into Z = A + iB on the full axis (Hermitian extension, DC and
 // Nyquist imaginary parts dropped as irfft does); after the inverse FFT the
 // real part is row r, the imaginary part row r+1.
-template <typename T, int KB, class Post>
-__global__ void k_rows_inv(Post post, const cplx<T>* __restrict__ spec, long long nrows, int n2, int nh,
-                           long long Nh, int F, int PB, Pass1D pl, const cplx<T>* __restrict__ tw,
-                           double* __restrict__ partials) {
+template <typename T, class FFT, class Post>
+__global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
+                                                      long long nrows, long long Nh, int F,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      double* __restrict__ partials) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int PB = kRowPairsC;
+  const int n2 = fft.n();
+  const int nh = n2 / 2 + 1;
+  const int th = fft.th();
   const long long pair0 = (long long)blockIdx.x * PB;
   const int ld = n2;
   const bool even = (n2 % 2) == 0;
-  for (int t = threadIdx.x; t < PB * F * nh; t += blockDim.x) {
+  for (int t = threadIdx.x; t < PB * F * nh; t += th) {
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
@@ -526,12 +610,14 @@ __global__ void k_rows_inv(Post post, const cplx<T>* __restrict__ spec, long lon
     lds[tr * ld + k] = cplx<T>{A.x - B.y, A.y + B.x};
     if (k != 0 && n2 - k != k) lds[tr * ld + (n2 - k)] = cplx<T>{A.x + B.y, B.x - A.y};
   }
+  cplx<T>* ltw = lds + PB * F * ld;
+  for (int t = threadIdx.x; t < n2; t += th) ltw[t] = tw[t];
   __syncthreads();
-  lds_fft<T, KB>(lds, ld, PB * F, pl, tw, true);
+  fft.run(lds, ld, PB * F, ltw, true);
   double acc[Post::NS];
 #pragma unroll
   for (int s = 0; s < Post::NS; ++s) acc[s] = 0.0;
-  for (int t = threadIdx.x; t < PB * n2; t += blockDim.x) {
+  for (int t = threadIdx.x; t < PB * n2; t += th) {
     const int lp = t / n2, j = t - lp * n2;
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
     T ca[3] = {T(0), T(0), T(0)}, cb[3] = {T(0), T(0), T(0)};
@@ -551,22 +637,28 @@ __global__ void k_rows_inv(Post post, const cplx<T>* __restrict__ spec, long lon
 
 // Strided complex transform along one axis: blockIdx.y selects the outer
 // index, blockIdx.x a tile of W contiguous columns, blockIdx.z the field.
-template <typename T, int KB>
-__global__ void k_axis_c2c(cplx<T>* __restrict__ spec, long long Nh, int n, long long stride, long long ostride,
-                           int ncol, int W, Pass1D pl, const cplx<T>* __restrict__ tw, int inv) {
+template <typename T, class FFT>
+__global__ __launch_bounds__(FFT::TH) void k_axis_c2c(FFT fft, cplx<T>* __restrict__ spec, long long Nh,
+                                                      long long stride, long long ostride, int ncol,
+                                                      const cplx<T>* __restrict__ tw, int inv) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
+  constexpr int W = kColTileC;
+  const int n = fft.n();
+  const int th = fft.th();
   cplx<T>* base = spec + (long long)blockIdx.z * Nh + (long long)blockIdx.y * ostride;
   const int c0 = blockIdx.x * W;
   const int ld = n + 1;
-  for (int t = threadIdx.x; t < n * W; t += blockDim.x) {
+  for (int t = threadIdx.x; t < n * W; t += th) {
     const int i = t / W, w = t - i * W;
     const int c = c0 + w;
     lds[w * ld + i] = c < ncol ? base[i * stride + c] : cplx<T>{T(0), T(0)};
   }
+  cplx<T>* ltw = lds + W * ld;
+  for (int t = threadIdx.x; t < n; t += th) ltw[t] = tw[t];
   __syncthreads();
-  lds_fft<T, KB>(lds, ld, W, pl, tw, inv != 0);
-  for (int t = threadIdx.x; t < n * W; t += blockDim.x) {
+  fft.run(lds, ld, W, ltw, inv != 0);
+  for (int t = threadIdx.x; t < n * W; t += th) {
     const int i = t / W, w = t - i * W;
     const int c = c0 + w;
     if (c < ncol) base[i * stride + c] = lds[w * ld + i];
@@ -578,57 +670,184 @@ struct ConvKinds {
   double scale;  // 1/N
 };
 
+// |G|^2 in the transform precision (FP32 plans use the FP32 lattice copy).
+template <typename T>
+__device__ __forceinline__ T gsq_t(const GridGeom& g, int k0, int k1, int k2, int partner) {
+  if constexpr (sizeof(T) == 8) {
+    return gsq(g, k0, k1, k2, partner);
+  } else {
+    int m0 = fftfreq_i(k0, g.n0), m1 = fftfreq_i(k1, g.n1);
+    int m2 = g.full ? fftfreq_i(k2, g.n2) : k2;
+    if (partner) {
+      m0 = fftfreq_i((g.n0 - k0) % g.n0, g.n0);
+      m1 = fftfreq_i((g.n1 - k1) % g.n1, g.n1);
+      m2 = fftfreq_i((g.n2 - k2) % g.n2, g.n2);
+    }
+    const float a = (float)m0, b = (float)m1, c = (float)m2;
+    const float gx = a * g.bgf[0] + b * g.bgf[3] + c * g.bgf[6];
+    const float gy = a * g.bgf[1] + b * g.bgf[4] + c * g.bgf[7];
+    const float gz = a * g.bgf[2] + b * g.bgf[5] + c * g.bgf[8];
+    return gx * gx + gy * gy + gz * gz;
+  }
+}
+
+// Reciprocal multiplier M(G)/N of one convolution kind at (k0, k1, c).
+template <typename T>
+__device__ __forceinline__ T conv_mult(const GridGeom& geo, int kind, int k0, int k1, int c, T scale,
+                                       const T* __restrict__ wtk) {
+  T m;
+  if (kind == OFDFT_CONV_HARTREE) {
+    const T g2 = gsq_t<T>(geo, k0, k1, c, 0);
+    m = g2 > T(0) ? T(4.0 * kPi) / g2 : T(0);
+    if (geo.full) {
+      const T h2 = gsq_t<T>(geo, k0, k1, c, 1);
+      m = T(0.5) * (m + (h2 > T(0) ? T(4.0 * kPi) / h2 : T(0)));
+    }
+  } else if (kind == OFDFT_CONV_GG) {
+    m = gsq_t<T>(geo, k0, k1, c, 0);
+    if (geo.full) m = T(0.5) * (m + gsq_t<T>(geo, k0, k1, c, 1));
+  } else if (kind == OFDFT_CONV_WT) {
+    m = wtk[((long long)k0 * geo.n1 + k1) * geo.nh + c];
+  } else {
+    m = T(1);
+  }
+  return m * scale;
+}
+
 // Axis-0 pencils: forward FFT, multiply by M_f(G)/N, inverse FFT.
-template <typename T, int KB>
-__global__ void k_pencil_conv(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo, Pass1D pl,
-                              const cplx<T>* __restrict__ tw, int W, ConvKinds ck, const T* __restrict__ wtk) {
+template <typename T, class FFT>
+__global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __restrict__ spec, long long Nh,
+                                                         GridGeom geo, const cplx<T>* __restrict__ tw,
+                                                         ConvKinds ck, const T* __restrict__ wtk) {
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int n = geo.n0;
+  constexpr int W = kColTileC;
+  const int n = fft.n();
+  const int th = fft.th();
   const int i1 = blockIdx.y;
   const int f = blockIdx.z;
   const long long stride = (long long)geo.n1 * geo.nh;
   cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nh;
   const int c0 = blockIdx.x * W;
   const int ld = n + 1;
-  for (int t = threadIdx.x; t < n * W; t += blockDim.x) {
+  for (int t = threadIdx.x; t < n * W; t += th) {
     const int i = t / W, w = t - i * W;
     const int c = c0 + w;
     lds[w * ld + i] = c < geo.nh ? base[i * stride + c] : cplx<T>{T(0), T(0)};
   }
+  cplx<T>* ltw = lds + W * ld;
+  for (int t = threadIdx.x; t < n; t += th) ltw[t] = tw[t];
   __syncthreads();
-  lds_fft<T, KB>(lds, ld, W, pl, tw, false);
+  fft.run(lds, ld, W, ltw, false);
   const int kind = ck.kind[f];
-  for (int t = threadIdx.x; t < n * W; t += blockDim.x) {
+  for (int t = threadIdx.x; t < n * W; t += th) {
     const int w = t / n, k0 = t - w * n;
     const int c = c0 + w;
     if (c >= geo.nh) continue;
-    double m;
-    if (kind == OFDFT_CONV_HARTREE) {
-      const double g2 = gsq(geo, k0, i1, c);
-      m = g2 > 0.0 ? 4.0 * kPi / g2 : 0.0;
-      if (geo.full) {
-        const double h2 = gsq(geo, k0, i1, c, 1);
-        m = 0.5 * (m + (h2 > 0.0 ? 4.0 * kPi / h2 : 0.0));
-      }
-    } else if (kind == OFDFT_CONV_GG) {
-      m = gsq(geo, k0, i1, c);
-      if (geo.full) m = 0.5 * (m + gsq(geo, k0, i1, c, 1));
-    } else if (kind == OFDFT_CONV_WT) {
-      m = (double)wtk[((long long)k0 * geo.n1 + i1) * geo.nh + c];
-    } else {
-      m = 1.0;
-    }
-    const T mm = T(m * ck.scale);
+    const T mm = conv_mult<T>(geo, kind, k0, i1, c, T(ck.scale), wtk);
     cplx<T> z = lds[w * ld + k0];
     lds[w * ld + k0] = cplx<T>{z.x * mm, z.y * mm};
   }
   __syncthreads();
-  lds_fft<T, KB>(lds, ld, W, pl, tw, true);
-  for (int t = threadIdx.x; t < n * W; t += blockDim.x) {
+  fft.run(lds, ld, W, ltw, true);
+  for (int t = threadIdx.x; t < n * W; t += th) {
     const int i = t / W, w = t - i * W;
     const int c = c0 + w;
     if (c < geo.nh) base[i * stride + c] = lds[w * ld + i];
+  }
+}
+
+
+// Four-step register transform of W columns along a strided axis (one
+// workgroup, W*N2 threads): x -> X in place, natural order in and out.
+template <typename T, int N1, int N2, int W>
+__global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+                                                     long long ostride, int ncol, const cplx<T>* __restrict__ tw,
+                                                     int inv) {
+  __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
+  const int t = threadIdx.x, w = t % W, q = t / W;
+  cplx<T>* base = spec + (long long)blockIdx.z * Nh + (long long)blockIdx.y * ostride;
+  const int c = blockIdx.x * W + w;
+  const bool ok = c < ncol;
+  const bool iv = inv != 0;
+  cplx<T> v[N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1)
+    v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
+  dftreg<T, N1>(v, iv);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) {
+    cplx<T> wv = tw[q * k1];
+    if (iv) wv.y = -wv.y;
+    v[k1] = cmul(v[k1], wv);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) lds[(w * N1 + k1) * (N2 + 1) + q] = v[k1];
+  __syncthreads();
+  if (q < N1) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
+    dftreg<T, N2>(v, iv);
+    if (ok) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) base[(long long)(q + N1 * k2) * stride + c] = v[k2];
+    }
+  }
+}
+
+// Four-step axis-0 pencils: forward transform, multiplier, inverse transform
+// with two LDS transposes and no intermediate global traffic.
+template <typename T, int N1, int N2, int W>
+__global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
+                                                       const cplx<T>* __restrict__ tw, ConvKinds ck,
+                                                       const T* __restrict__ wtk) {
+  __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
+  const int t = threadIdx.x, w = t % W, q = t / W;
+  const int i1 = blockIdx.y;
+  const int f = blockIdx.z;
+  const long long stride = (long long)geo.n1 * geo.nh;
+  cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nh;
+  const int c = blockIdx.x * W + w;
+  const bool ok = c < geo.nh;
+  cplx<T> v[N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1)
+    v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
+  dftreg<T, N1>(v, false);
+#pragma unroll
+  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) lds[(w * N1 + k1) * (N2 + 1) + q] = v[k1];
+  __syncthreads();
+  if (q < N1) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
+    dftreg<T, N2>(v, false);
+    const int kind = ck.kind[f];
+    const T scale = T(ck.scale);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const T m = ok ? conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk) : T(0);
+      v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+    }
+    dftreg<T, N2>(v, true);
+#pragma unroll
+    for (int n2 = 1; n2 < N2; ++n2) {
+      cplx<T> wv = tw[n2 * q];
+      wv.y = -wv.y;
+      v[n2] = cmul(v[n2], wv);
+    }
+    // this thread re-reads nothing it overwrites: row (w, q) is private to it
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) lds[(w * N1 + q) * (N2 + 1) + n2] = v[n2];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[(w * N1 + k1) * (N2 + 1) + q];
+  dftreg<T, N1>(v, true);
+  if (ok) {
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) base[(long long)(N2 * n1 + q) * stride + c] = v[n1];
   }
 }
 
@@ -677,7 +896,7 @@ __global__ void k_lincomb(T alpha, const T* x, T beta, const T* y, T* z, long lo
 }
 
 template <typename T>
-__global__ void k_residual(const T* __restrict__ V, const T* __restrict__ phi, T mu, T* __restrict__ res, long long N,
+__global__ void k_residual(const T* V, const T* phi, T mu, T* res, long long N,
                            double* __restrict__ partials) {
   double acc[1] = {0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
@@ -719,8 +938,8 @@ __global__ void k_convert(Td* __restrict__ dst, const Ts* __restrict__ src, long
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-constexpr int kRowPairs = 2;   // row pairs per rows_* block
-constexpr int kColTile = 16;   // W for axis-1 / axis-0 tiles
+constexpr int kRowPairs = kRowPairsC;
+constexpr int kColTile = kColTileC;
 
 int vec_blocks(long long N) {
   long long b = (N + 255) / 256;
@@ -741,6 +960,7 @@ FnParams make_fn(const ofdft_functional& f) {
   p.y = f.y;
   p.alpha = f.alpha;
   p.beta = f.beta;
+  p.beta56 = std::fabs(f.beta - 5.0 / 6.0) < 1e-15 ? 1 : 0;
   int F = 0;
   p.fH = f.hartree ? F++ : -1;
   p.fV = f.vw ? F++ : -1;
@@ -756,45 +976,91 @@ GridGeom geom(const ofdft_plan_s* P) {
   g.n2 = P->n[2];
   g.nh = P->nh;
   g.full = P->full;
-  for (int i = 0; i < 9; ++i) g.bg[i] = P->bg[i];
+  for (int i = 0; i < 9; ++i) {
+    g.bg[i] = P->bg[i];
+    g.bgf[i] = (float)P->bg[i];
+  }
   return g;
+}
+
+// Compile-time FFT lengths; any other 2^a 3^b 5^c length takes the runtime path.
+#define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(160) X(192)
+
+constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
+// columns per workgroup of the register column kernels
+template <typename T>
+struct RegTile {
+  static constexpr int W = sizeof(T) == 4 ? 32 : 16;
+};
+constexpr int kRowsTH = 256;
+constexpr int kColsTH = 256;
+template <int N>
+using RowsFFT = FFTFixed<N, kRowPairsC * 3, kRowsTH>;
+template <int N>
+using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
+
+template <typename T, class Pre, class FFT>
+void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
+  hipLaunchKernelGGL((k_rows_fwd<T, FFT, Pre>), grid, dim3(threads), lds, P->stream, pre, fft,
+                     reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nh, F,
+                     reinterpret_cast<const cplx<T>*>(P->tw[2]));
 }
 
 template <typename T, class Pre>
 int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
-  const int PB = kRowPairs;
-  FFTLaunch L = fft_launch(P->pl[2], PB * F);
-  const size_t lds = (size_t)PB * F * P->n[2] * sizeof(cplx<T>);
+  const int PB = kRowPairsC;
+  const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   dim3 grid((unsigned)((npairs + PB - 1) / PB));
-  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
-  auto* tw = reinterpret_cast<const cplx<T>*>(P->tw[2]);
-  switch (L.kb) {
-    case 1: hipLaunchKernelGGL((k_rows_fwd<T, 1, Pre>), grid, dim3(L.threads), lds, P->stream, pre, spec, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw); break;
-    case 2: hipLaunchKernelGGL((k_rows_fwd<T, 2, Pre>), grid, dim3(L.threads), lds, P->stream, pre, spec, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw); break;
-    default: hipLaunchKernelGGL((k_rows_fwd<T, 4, Pre>), grid, dim3(L.threads), lds, P->stream, pre, spec, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw); break;
+  ProfScope ps(P, PC_ROWS_FWD, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
+  switch (P->n[2]) {
+#define OFDFT_CASE(NN) \
+    case NN: go_rows_fwd<T>(P, pre, RowsFFT<NN>{}, F, grid, kRowsTH, lds); break;
+    OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+    default: {
+      FFTLaunch L = fft_launch(P->pl[2], PB * F);
+      if (L.kb == 1) go_rows_fwd<T>(P, pre, FFTRuntime<1>{P->pl[2]}, F, grid, L.threads, lds);
+      else if (L.kb == 2) go_rows_fwd<T>(P, pre, FFTRuntime<2>{P->pl[2]}, F, grid, L.threads, lds);
+      else go_rows_fwd<T>(P, pre, FFTRuntime<4>{P->pl[2]}, F, grid, L.threads, lds);
+    }
   }
   LAUNCHCHK();
   return 0;
+}
+
+template <typename T, class Post, class FFT>
+void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const cplx<T>* src, dim3 grid,
+                 int threads, size_t lds, double* part) {
+  hipLaunchKernelGGL((k_rows_inv<T, FFT, Post>), grid, dim3(threads), lds, P->stream, post, fft, src,
+                     (long long)P->n[0] * P->n[1], P->Nh, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part);
 }
 
 template <typename T, class Post>
 int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
-  const int PB = kRowPairs;
-  FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
-  const size_t lds = (size_t)PB * F * P->n[2] * sizeof(cplx<T>);
+  const int PB = kRowPairsC;
+  const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   const long long nblk = (npairs + PB - 1) / PB;
   if (reduce && nblk * Post::NS > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
   dim3 grid((unsigned)nblk);
-  auto* tw = reinterpret_cast<const cplx<T>*>(P->tw[2]);
   double* part = reduce ? P->partials : nullptr;
-  switch (L.kb) {
-    case 1: hipLaunchKernelGGL((k_rows_inv<T, 1, Post>), grid, dim3(L.threads), lds, P->stream, post, src, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw, part); break;
-    case 2: hipLaunchKernelGGL((k_rows_inv<T, 2, Post>), grid, dim3(L.threads), lds, P->stream, post, src, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw, part); break;
-    default: hipLaunchKernelGGL((k_rows_inv<T, 4, Post>), grid, dim3(L.threads), lds, P->stream, post, src, nrows, P->n[2], P->nh, P->Nh, F, PB, P->pl[2], tw, part); break;
+  {
+    ProfScope ps(P, PC_ROWS_INV, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
+    switch (P->n[2]) {
+#define OFDFT_CASE(NN) \
+      case NN: go_rows_inv<T>(P, post, RowsFFT<NN>{}, F, src, grid, kRowsTH, lds, part); break;
+      OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+      default: {
+        FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
+        if (L.kb == 1) go_rows_inv<T>(P, post, FFTRuntime<1>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else if (L.kb == 2) go_rows_inv<T>(P, post, FFTRuntime<2>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else go_rows_inv<T>(P, post, FFTRuntime<4>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+      }
+    }
   }
   LAUNCHCHK();
   if (reduce) {
@@ -804,13 +1070,18 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   return 0;
 }
 
+template <typename T, class FFT>
+void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid, int threads, size_t lds,
+             long long stride, long long ostride, bool inv) {
+  hipLaunchKernelGGL((k_axis_c2c<T, FFT>), grid, dim3(threads), lds, P->stream, fft, spec, P->Nh, stride, ostride,
+                     P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0);
+}
+
 template <typename T>
 int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
-  const int W = kColTile;
+  const int W = kColTileC;
   const int n = P->n[axis];
-  const Pass1D& pl = P->pl[axis];
-  FFTLaunch L = fft_launch(pl, W);
-  const size_t lds = (size_t)W * (n + 1) * sizeof(cplx<T>);
+  const size_t lds = (size_t)(W * (n + 1) + n) * sizeof(cplx<T>);
   long long stride, ostride;
   unsigned outer;
   if (axis == 1) {
@@ -823,35 +1094,69 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
     outer = P->n[1];
   }
   dim3 grid((unsigned)((P->nh + W - 1) / W), outer, (unsigned)F);
-  auto* tw = reinterpret_cast<const cplx<T>*>(P->tw[axis]);
-  switch (L.kb) {
-    case 1: hipLaunchKernelGGL((k_axis_c2c<T, 1>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, n, stride, ostride, P->nh, W, pl, tw, inv ? 1 : 0); break;
-    case 2: hipLaunchKernelGGL((k_axis_c2c<T, 2>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, n, stride, ostride, P->nh, W, pl, tw, inv ? 1 : 0); break;
-    default: hipLaunchKernelGGL((k_axis_c2c<T, 4>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, n, stride, ostride, P->nh, W, pl, tw, inv ? 1 : 0); break;
+  ProfScope ps(P, PC_AXIS, (long long)P->rs * 4LL * F * P->Nh);
+  switch (n) {
+#define OFDFT_CASE(NN)                                                                                       \
+    case NN:                                                                                                 \
+      hipLaunchKernelGGL((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
+                         dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),     \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nh, stride, ostride, P->nh, \
+                         reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0);                        \
+      break;
+    OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+    default: {
+      const Pass1D& pl = P->pl[axis];
+      FFTLaunch L = fft_launch(pl, W);
+      if (L.kb == 1) go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else if (L.kb == 2) go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+    }
   }
   LAUNCHCHK();
   return 0;
 }
 
+template <typename T, class FFT>
+void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t lds, const GridGeom& g,
+               const ConvKinds& ck) {
+  hipLaunchKernelGGL((k_pencil_conv<T, FFT>), grid, dim3(threads), lds, P->stream, fft,
+                     reinterpret_cast<cplx<T>*>(P->spec), P->Nh, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,
+                     reinterpret_cast<const T*>(P->wtk));
+}
+
 template <typename T>
 int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
-  const int W = kColTile;
-  FFTLaunch L = fft_launch(P->pl[0], W);
-  const size_t lds = (size_t)W * (P->n[0] + 1) * sizeof(cplx<T>);
+  const int W = kColTileC;
+  const size_t lds = (size_t)(W * (P->n[0] + 1) + P->n[0]) * sizeof(cplx<T>);
   dim3 grid((unsigned)((P->nh + W - 1) / W), (unsigned)P->n[1], (unsigned)F);
   ConvKinds ck;
   for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
   ck.scale = 1.0 / (double)P->N;
   for (int f = 0; f < F; ++f)
     if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
-  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
-  auto* tw = reinterpret_cast<const cplx<T>*>(P->tw[0]);
-  const T* wtk = reinterpret_cast<const T*>(P->wtk);
   GridGeom g = geom(P);
-  switch (L.kb) {
-    case 1: hipLaunchKernelGGL((k_pencil_conv<T, 1>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, g, P->pl[0], tw, W, ck, wtk); break;
-    case 2: hipLaunchKernelGGL((k_pencil_conv<T, 2>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, g, P->pl[0], tw, W, ck, wtk); break;
-    default: hipLaunchKernelGGL((k_pencil_conv<T, 4>), grid, dim3(L.threads), lds, P->stream, spec, P->Nh, g, P->pl[0], tw, W, ck, wtk); break;
+  long long nwt = 0;
+  for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
+  ProfScope ps(P, PC_PENCIL, (long long)P->rs * (4LL * F + nwt) * P->Nh);
+  switch (P->n[0]) {
+#define OFDFT_CASE(NN)                                                                                   \
+    case NN:                                                                                             \
+      hipLaunchKernelGGL((k_pencil_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                 \
+                         dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1],  \
+                              (unsigned)F),                                                              \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, reinterpret_cast<cplx<T>*>(P->spec), \
+                         P->Nh, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                       \
+                         reinterpret_cast<const T*>(P->wtk));                                            \
+      break;
+    OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+    default: {
+      FFTLaunch L = fft_launch(P->pl[0], W);
+      if (L.kb == 1) go_pencil<T>(P, FFTRuntime<1>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else if (L.kb == 2) go_pencil<T>(P, FFTRuntime<2>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else go_pencil<T>(P, FFTRuntime<4>{P->pl[0]}, grid, L.threads, lds, g, ck);
+    }
   }
   LAUNCHCHK();
   return 0;
@@ -997,6 +1302,495 @@ int reduce_blocks(ofdft_plan_s* P, int nblk, int ns, double* out) {
   return 0;
 }
 
+
+// ===========================================================================
+// ions: local pseudopotential (PME / exact structure factor), local forces,
+// Ewald ion-ion energy.  FP64 plans only (set-up and forces are computed in
+// FP64; an FP32 plan receives V_loc through ofdft_convert).
+// ===========================================================================
+struct SplineDev {
+  const double* t;
+  const double* c;
+  int nt;
+  double q1, dq, qmax, v0, v1, v2;
+};
+
+// splev of the cubic interpolating spline (FITPACK fpbspl recursion) on
+// [q1, qmax); quartic interpolation below q1; zero from qmax on
+// (calc_vlines, pseudo/__init__.py:398-423; math_utils.py:390-400).
+__device__ double vline_eval(const SplineDev& sp, double x) {
+  if (!(x < sp.qmax)) return 0.0;
+  if (x < sp.q1) {
+    const double f0 = sp.v2, f1 = sp.v1, f2 = sp.v0, f3 = sp.v1, f4 = sp.v2;
+    const double t1 = 2.0 * f0 - 16.0 * f1 + 16.0 * f3 - 2.0 * f4;
+    const double t2 = -f0 + 16.0 * f1 - 30.0 * f2 + 16.0 * f3 - f4;
+    const double t3 = -2.0 * f0 + 4.0 * f1 - 4.0 * f3 + 2.0 * f4;
+    const double t4 = f0 - 4.0 * f1 + 6.0 * f2 - 4.0 * f3 + f4;
+    const double dx = x / sp.dq;
+    return f2 + dx * (t1 + dx * (t2 + dx * (t3 + dx * t4))) / 24.0;
+  }
+  const int k = 3;
+  const double* t = sp.t;
+  const int hi = sp.nt - k - 2;
+  int l;
+  if (x >= t[hi + 1]) {
+    l = hi;
+  } else {
+    int a = k, b = hi;
+    while (a < b) {
+      const int m = (a + b + 1) >> 1;
+      if (t[m] <= x) a = m; else b = m - 1;
+    }
+    l = a;
+  }
+  double h[4] = {1.0, 0.0, 0.0, 0.0};
+  double hh[3];
+#pragma unroll
+  for (int j = 1; j <= k; ++j) {
+#pragma unroll
+    for (int i = 0; i < j; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= j; ++i) {
+      const int li = l + i, lj = li - j;
+      const double f = hh[i - 1] / (t[li] - t[lj]);
+      h[i - 1] += f * (t[li] - x);
+      h[i] = f * (x - t[lj]);
+    }
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j <= k; ++j) v += sp.c[l - k + j] * h[j];
+  return v;
+}
+
+// Frequency vector at half-spectrum index (k0,k1,k2) (partner: index -k).
+__device__ __forceinline__ void gvec(const GridGeom& g, int k0, int k1, int k2, int partner, double* G) {
+  int m0 = fftfreq_i(k0, g.n0), m1 = fftfreq_i(k1, g.n1);
+  int m2 = g.full ? fftfreq_i(k2, g.n2) : k2;
+  if (partner) {
+    m0 = fftfreq_i((g.n0 - k0) % g.n0, g.n0);
+    m1 = fftfreq_i((g.n1 - k1) % g.n1, g.n1);
+    m2 = fftfreq_i((g.n2 - k2) % g.n2, g.n2);
+  }
+  G[0] = m0 * g.bg[0] + m1 * g.bg[3] + m2 * g.bg[6];
+  G[1] = m0 * g.bg[1] + m1 * g.bg[4] + m2 * g.bg[7];
+  G[2] = m0 * g.bg[2] + m1 * g.bg[5] + m2 * g.bg[8];
+}
+
+// Half-space mask of ReciprocalGrid.mask (grid.py:658-695, serial offsets).
+__device__ bool in_mask(const GridGeom& g, int k0, int k1, int k2) {
+  const int D0 = g.n0 / 2, D1 = g.n1 / 2, D2 = g.n2 / 2;
+  const bool e0 = (g.n0 % 2) == 0, e1 = (g.n1 % 2) == 0, e2 = (g.n2 % 2) == 0;
+  if (g.full && k2 > D2) return false;
+  if (k2 == 0) {
+    if (k0 == 0 && k1 > D1) return false;
+    if (k0 > D0) return false;
+  }
+  if (e2 && k2 == D2) {
+    if (k0 == 0 && k1 == 0) return false;
+    if (k0 == 0 && k1 > D1) return false;
+    if (k0 > D0) return false;
+    if (e1 && k0 == 0 && k1 == D1) return false;
+    if (e0 && k0 == D0 && k1 == 0) return false;
+    if (e0 && k0 == D0 && k1 > D1) return false;
+  }
+  if (e0 && k2 == 0 && k0 == D0) {
+    if (k1 > D1) return false;
+    if (e1 && k1 == D1) return false;
+  }
+  if (e1 && k2 == 0 && k0 == 0 && k1 == D1) return false;
+  if (e0 && e1 && e2 && k0 == D0 && k1 == D1 && k2 == D2) return false;
+  return true;
+}
+
+// Cardinal B-spline values M_n(x + j), stored at index 1..order of an
+// (order_total+1) array (CBspline.calc_Mn, ewald.py:63-82).
+__device__ void calc_Mn(double x, int order, double* Mn, int order_total) {
+  for (int i = 0; i <= order_total; ++i) Mn[i] = 0.0;
+  Mn[1] = x;
+  Mn[2] = 1.0 - x;
+  for (int i = 3; i <= order; ++i) {
+    for (int j = 0; j < i; ++j) {
+      const int n = i - j;
+      Mn[n] = ((x + n - 1) * Mn[n] + (j + 1 - x) * Mn[n - 1]) / (i - 1);
+    }
+  }
+}
+
+constexpr int kMaxOrder = 12;
+
+// PME Q-array spread of the atoms with species == sel (sel < 0: all atoms,
+// weighted by w[i]) -- CBspline.get_PME_Qarray / _calc_PME_Qarray
+// (ewald.py:137-194).  One workgroup per atom, one thread per stencil point.
+__global__ void k_pme_spread(double* __restrict__ Q, const double* __restrict__ frac, const int* __restrict__ species,
+                             int sel, const double* __restrict__ w, int n0, int n1, int n2, int order) {
+  __shared__ double Mn[3][kMaxOrder + 1];
+  __shared__ int base[3];
+  const int i = blockIdx.x;
+  const bool take = sel < 0 || species[i] == sel;
+  if (!take) return;
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    const int n = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    const double up = frac[3 * i + d] * n;
+    const double fl = floor(up);
+    calc_Mn(up - fl, order, Mn[d], order);
+    base[d] = (int)fl;
+  }
+  __syncthreads();
+  const double wi = w ? w[i] : 1.0;
+  const int o = order;
+  for (int t = threadIdx.x; t < o * o * o; t += blockDim.x) {
+    const int a = t / (o * o), b = (t / o) % o, c = t % o;
+    int l0 = (1 + base[0] - a) % n0; if (l0 < 0) l0 += n0;
+    int l1 = (1 + base[1] - b) % n1; if (l1 < 0) l1 += n1;
+    int l2 = (1 + base[2] - c) % n2; if (l2 < 0) l2 += n2;
+    const double v = wi * Mn[0][a + 1] * Mn[1][b + 1] * Mn[2][c + 1];
+    atomicAdd(&Q[((long long)l0 * n1 + l1) * n2 + l2], v);
+  }
+}
+
+__device__ __forceinline__ cplx<double> bmod(const double* bm, int n0, int n1, int k0, int k1, int k2) {
+  const cplx<double>* b = reinterpret_cast<const cplx<double>*>(bm);
+  return cmul(cmul(b[k0], b[n0 + k1]), b[n0 + n1 + k2]);
+}
+
+// acc += Qhat * [v(G) B(k) (+ v(G') conj B(-k) in full mode)] * scale
+// (_PP_Reciprocal_PME, pseudo/__init__.py:437-452).
+__global__ void k_vloc_accum_pme(cplx<double>* __restrict__ acc, const cplx<double>* __restrict__ qhat, GridGeom geo,
+                                 const SplineDev* __restrict__ spl, int s, const double* __restrict__ bm,
+                                 double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const SplineDev sp = spl[s];
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    cplx<double> m = cscale(bmod(bm, geo.n0, geo.n1, k0, k1, c), vline_eval(sp, sqrt(gsq(geo, k0, k1, c))));
+    if (geo.full) {
+      const int p0 = (geo.n0 - k0) % geo.n0, p1 = (geo.n1 - k1) % geo.n1, p2 = (geo.n2 - c) % geo.n2;
+      cplx<double> bp = bmod(bm, geo.n0, geo.n1, p0, p1, p2);
+      bp.y = -bp.y;
+      const cplx<double> mp = cscale(bp, vline_eval(sp, sqrt(gsq(geo, k0, k1, c, 1))));
+      m = cscale(cadd(m, mp), 0.5);
+    }
+    const cplx<double> z = cmul(qhat[idx], m);
+    acc[idx].x += z.x * scale;
+    acc[idx].y += z.y * scale;
+  }
+}
+
+// acc = sum_s v_s(|G|) sum_{i in s} exp(-i G.R_i) * scale, Hermitian
+// projection in full mode (_PP_Reciprocal, pseudo/__init__.py:425-435).
+__global__ void k_vloc_exact(cplx<double>* __restrict__ acc, GridGeom geo, const SplineDev* __restrict__ spl,
+                             const double* __restrict__ pos, const int* __restrict__ species, int nat,
+                             double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    cplx<double> tot = {0.0, 0.0};
+    const int npart = geo.full ? 2 : 1;
+    for (int part = 0; part < npart; ++part) {
+      double G[3];
+      gvec(geo, k0, k1, c, part, G);
+      const double q = sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+      cplx<double> v = {0.0, 0.0};
+      int cur = -1;
+      double vl = 0.0;
+      for (int i = 0; i < nat; ++i) {
+        if (species[i] != cur) {
+          cur = species[i];
+          vl = vline_eval(spl[cur], q);
+        }
+        double sn, cs;
+        sincos(G[0] * pos[3 * i] + G[1] * pos[3 * i + 1] + G[2] * pos[3 * i + 2], &sn, &cs);
+        v.x += vl * cs;
+        v.y -= vl * sn;
+      }
+      if (part == 1) v.y = -v.y;  // conj(v(-k))
+      tot = cadd(tot, v);
+    }
+    const double f = scale / npart;
+    acc[idx] = cplx<double>{tot.x * f, tot.y * f};
+  }
+}
+
+// Per-species radial table v_s(|G|) on the half spectrum (for forces).
+__global__ void k_vline_table(double* __restrict__ out, GridGeom geo, const SplineDev* __restrict__ spl, int s) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const SplineDev sp = spl[s];
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    out[idx] = vline_eval(sp, sqrt(gsq(geo, k0, k1, c)));
+  }
+}
+
+// Exact local-PP forces, one workgroup per atom (_Force, pseudo/__init__.py:505-521):
+// F_i = (2/Omega) sum_{mask} G v_s(|G|) Im(rhoG exp(i G.R_i)), rhoG = fft(rho).
+__global__ void k_force_exact(const cplx<double>* __restrict__ rhog, GridGeom geo, const SplineDev* __restrict__ spl,
+                              const double* __restrict__ pos, const int* __restrict__ species, double dV,
+                              double* __restrict__ partials) {
+  const int i = blockIdx.x;
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const SplineDev sp = spl[species[i]];
+  const double R0 = pos[3 * i], R1 = pos[3 * i + 1], R2 = pos[3 * i + 2];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long idx = threadIdx.x; idx < Nh; idx += blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    if (!in_mask(geo, k0, k1, c)) continue;
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    const double v = vline_eval(sp, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]));
+    double sn, cs;
+    sincos(G[0] * R0 + G[1] * R1 + G[2] * R2, &sn, &cs);
+    const cplx<double> z = rhog[idx];
+    const double im = (z.y * cs + z.x * sn) * dV * v;
+    acc[0] += G[0] * im;
+    acc[1] += G[1] * im;
+    acc[2] += G[2] * im;
+  }
+  block_sums<3>(acc, partials + 3 * i);
+}
+
+// spec = rhoG * conj(B) * v_s(|G|), G=0 removed, Hermitian projection in
+// full mode (_ForcePME, pseudo/__init__.py:528-545).
+__global__ void k_force_pme_mult(cplx<double>* __restrict__ out, const cplx<double>* __restrict__ rhog, GridGeom geo,
+                                 const SplineDev* __restrict__ spl, int s, const double* __restrict__ bm) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const SplineDev sp = spl[s];
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    cplx<double> b = bmod(bm, geo.n0, geo.n1, k0, k1, c);
+    b.y = -b.y;
+    cplx<double> m = cscale(b, vline_eval(sp, sqrt(gsq(geo, k0, k1, c))));
+    if (geo.full) {
+      const int p0 = (geo.n0 - k0) % geo.n0, p1 = (geo.n1 - k1) % geo.n1, p2 = (geo.n2 - c) % geo.n2;
+      const cplx<double> bp = bmod(bm, geo.n0, geo.n1, p0, p1, p2);
+      m = cscale(cadd(m, cscale(bp, vline_eval(sp, sqrt(gsq(geo, k0, k1, c, 1))))), 0.5);
+    }
+    out[idx] = idx == 0 ? cplx<double>{0.0, 0.0} : cmul(rhog[idx], m);
+  }
+}
+
+// PME force gather for the atoms of species sel, one workgroup per atom:
+// F_i = -sum_stencil (dQ^T inv(L)) rhoPB[l] (pseudo/__init__.py:546-570).
+__global__ void k_force_pme_gather(const double* __restrict__ rhopb, const double* __restrict__ frac,
+                                   const int* __restrict__ species, int sel, int n0, int n1, int n2, int order,
+                                   const double* __restrict__ linv, double* __restrict__ partials) {
+  __shared__ double Mn[3][kMaxOrder + 1];
+  __shared__ double M2[3][kMaxOrder + 1];
+  __shared__ int base[3];
+  const int i = blockIdx.x;
+  if (species[i] != sel) return;
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    const int n = d == 0 ? n0 : (d == 1 ? n1 : n2);
+    const double up = frac[3 * i + d] * n;
+    const double fl = floor(up);
+    calc_Mn(up - fl, order, Mn[d], order);
+    calc_Mn(up - fl, order - 1, M2[d], order);
+    base[d] = (int)fl;
+  }
+  __syncthreads();
+  const int o = order;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < o * o * o; t += blockDim.x) {
+    const int a = t / (o * o), b = (t / o) % o, c = t % o;
+    int l0 = (1 + base[0] - a) % n0; if (l0 < 0) l0 += n0;
+    int l1 = (1 + base[1] - b) % n1; if (l1 < 0) l1 += n1;
+    int l2 = (1 + base[2] - c) % n2; if (l2 < 0) l2 += n2;
+    const double dq0 = n0 * (M2[0][a + 1] - M2[0][a]) * Mn[1][b + 1] * Mn[2][c + 1];
+    const double dq1 = n1 * Mn[0][a + 1] * (M2[1][b + 1] - M2[1][b]) * Mn[2][c + 1];
+    const double dq2 = n2 * Mn[0][a + 1] * Mn[1][b + 1] * (M2[2][c + 1] - M2[2][c]);
+    const double val = rhopb[((long long)l0 * n1 + l1) * n2 + l2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] -= (dq0 * linv[d] + dq1 * linv[3 + d] + dq2 * linv[6 + d]) * val;
+  }
+  block_sums<3>(acc, partials + 3 * i);
+}
+
+// Ewald real-space pair sum over images (ewald.py:280-322); one thread per
+// ordered pair (i, j), looping over the image shifts.
+__global__ void k_ewald_real(const double* __restrict__ pos, const double* __restrict__ q, int nat,
+                             const double* __restrict__ shifts, int nshift, double rmax, double eta_sqrt,
+                             double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  const long long npair = (long long)nat * nat;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npair;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / nat), j = (int)(t % nat);
+    const double dx0 = pos[3 * j] - pos[3 * i], dy0 = pos[3 * j + 1] - pos[3 * i + 1], dz0 = pos[3 * j + 2] - pos[3 * i + 2];
+    double s = 0.0;
+    for (int m = 0; m < nshift; ++m) {
+      const double dx = dx0 - shifts[3 * m], dy = dy0 - shifts[3 * m + 1], dz = dz0 - shifts[3 * m + 2];
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      if (d < rmax && d > 0.001) s += erfc(eta_sqrt * d) / d;
+    }
+    acc[0] += q[i] * q[j] * s;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// Ewald reciprocal sum over the half-space mask: sum |S(G)|^2 exp(-G^2/(4 eta))/G^2
+// with S = rfftn(Q) * B (PME, ewald.py:785-807) or the exact S (ewald.py:448-465).
+__global__ void k_ewald_rec(const cplx<double>* __restrict__ qhat, const double* __restrict__ bm, GridGeom geo,
+                            const double* __restrict__ pos, const double* __restrict__ q, int nat, double eta,
+                            double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    if (!in_mask(geo, k0, k1, c)) continue;
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    const double g2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    if (!(g2 > 0.0)) continue;
+    cplx<double> S;
+    if (qhat) {
+      S = cmul(qhat[idx], bmod(bm, geo.n0, geo.n1, k0, k1, c));
+    } else {
+      S = {0.0, 0.0};
+      for (int i = 0; i < nat; ++i) {
+        double sn, cs;
+        sincos(G[0] * pos[3 * i] + G[1] * pos[3 * i + 1] + G[2] * pos[3 * i + 2], &sn, &cs);
+        S.x += q[i] * cs;
+        S.y -= q[i] * sn;
+      }
+    }
+    acc[0] += (S.x * S.x + S.y * S.y) * exp(-g2 / (4.0 * eta)) / g2;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// Python's float modulo x % 1.0, applied twice as ase's wrap does.
+double pymod1(double x) {
+  double r = std::fmod(x, 1.0);
+  if (r != 0.0 && r < 0.0) r += 1.0;
+  return r;
+}
+
+// b-moduli of the cardinal B-spline (CBspline._calc_bm, ewald.py:84-101).
+std::vector<double> make_bm(const ofdft_plan_s* P, int order) {
+  std::vector<double> M(order + 1, 0.0);
+  M[1] = 1.0;
+  M[2] = 0.0;
+  for (int i = 3; i <= order; ++i)
+    for (int j = 0; j < i; ++j) {
+      const int n = i - j;
+      M[n] = ((1.0 + n - 1) * M[n] + (j + 1 - 1.0) * M[n - 1]) / (i - 1);
+    }
+  const int nb2 = P->full ? P->n[2] : P->nh;
+  std::vector<double> out;
+  for (int ax = 0; ax < 3; ++ax) {
+    const int cnt = ax < 2 ? P->n[ax] : nb2;
+    for (int k = 0; k < cnt; ++k) {
+      const double q = 2.0 * kPi * k / P->n[ax];
+      double tr = std::cos(-(order - 1.0) * q), ti = std::sin(-(order - 1.0) * q);
+      double fr = 0.0, fi = 0.0;
+      for (int j = 1; j < order; ++j) {
+        fr += M[j] * std::cos(-j * q);
+        fi += M[j] * std::sin(-j * q);
+      }
+      const double den = fr * fr + fi * fi;
+      out.push_back((tr * fr + ti * fi) / den);
+      out.push_back((ti * fr - tr * fi) / den);
+    }
+  }
+  return out;
+}
+
+// forward r2c of a real FP64 field into spectrum slot `slot` of the workspace
+int r2c_slot(ofdft_plan_s* P, const double* in, int slot) {
+  PreCopy<double> pre{in};
+  int rc;
+  auto* spec = reinterpret_cast<cplx<double>*>(P->spec) + (long long)slot * P->Nh;
+  // rows kernel writes slot 0; move it if another slot is requested
+  if ((rc = launch_rows_fwd<double>(P, pre, 1))) return rc;
+  auto* s0 = reinterpret_cast<cplx<double>*>(P->spec);
+  if ((rc = launch_axis<double>(P, s0, 1, 1, false))) return rc;
+  if ((rc = launch_axis<double>(P, s0, 0, 1, false))) return rc;
+  if (slot != 0) HIPCHK(hipMemcpyAsync(spec, s0, sizeof(cplx<double>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
+  return 0;
+}
+
+// c2r of spectrum slot 0 (consumed) into a real FP64 field, 1/N normalised.
+int c2r_slot0(ofdft_plan_s* P, double* out) {
+  auto* s0 = reinterpret_cast<cplx<double>*>(P->spec);
+  int rc;
+  hipLaunchKernelGGL((k_scale_cplx<double>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, s0, P->Nh,
+                     1.0 / (double)P->N);
+  LAUNCHCHK();
+  if ((rc = launch_axis<double>(P, s0, 0, 1, true))) return rc;
+  if ((rc = launch_axis<double>(P, s0, 1, 1, true))) return rc;
+  PostStore<double> post{out};
+  return launch_rows_inv<double>(P, post, 1, s0, false);
+}
+
+int upload_ions(ofdft_plan_s* P, int nat, const double* positions, const int* species) {
+  std::vector<double> frac(3 * (size_t)nat);
+  // frac = positions . inv(L) (rows of L are lattice vectors); inv(L) = bg^T/(2 pi)
+  for (int i = 0; i < nat; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int e = 0; e < 3; ++e) f += positions[3 * i + e] * P->bg[d * 3 + e] / (2.0 * kPi);
+      frac[3 * i + d] = pymod1(pymod1(f));
+    }
+  if (P->d_frac) (void)hipFree(P->d_frac);
+  if (P->d_pos) (void)hipFree(P->d_pos);
+  if (P->d_species) (void)hipFree(P->d_species);
+  P->d_frac = nullptr;
+  P->d_pos = nullptr;
+  P->d_species = nullptr;
+  HIPCHK(hipMalloc((void**)&P->d_frac, sizeof(double) * 3 * nat));
+  HIPCHK(hipMalloc((void**)&P->d_pos, sizeof(double) * 3 * nat));
+  HIPCHK(hipMalloc((void**)&P->d_species, sizeof(int) * nat));
+  HIPCHK(hipMemcpy(P->d_frac, frac.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->d_pos, positions, sizeof(double) * 3 * nat, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(P->d_species, species, sizeof(int) * nat, hipMemcpyHostToDevice));
+  P->nat = nat;
+  return 0;
+}
+
+int ensure_work(ofdft_plan_s* P) {
+  if (!P->d_work) HIPCHK(hipMalloc((void**)&P->d_work, sizeof(double) * P->N));
+  return 0;
+}
+
+int ensure_bm(ofdft_plan_s* P, int order) {
+  std::vector<double> bm = make_bm(P, order);
+  if (P->d_bm) (void)hipFree(P->d_bm);
+  P->d_bm = nullptr;
+  HIPCHK(hipMalloc((void**)&P->d_bm, sizeof(double) * bm.size()));
+  HIPCHK(hipMemcpy(P->d_bm, bm.data(), sizeof(double) * bm.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int reduce_to_host(ofdft_plan_s* P, long long nblk, int ns, double* out) {
+  // partial layout [blk][ns]; reuse k_reduce_partials
+  if (ns == 1)
+    hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red);
+  else
+    hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red);
+  LAUNCHCHK();
+  return fetch_sums(P, ns, out);
+}
+
 bool valid(ofdft_plan P) { return P != nullptr; }
 
 }  // namespace
@@ -1036,8 +1830,8 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   }
   // LDS limits (64 KiB per workgroup)
   const size_t cs = 2 * P->rs;
-  if ((size_t)kRowPairs * 3 * n2 * cs > 65536 || (size_t)kColTile * (n0 + 1) * cs > 65536 ||
-      (size_t)kColTile * (n1 + 1) * cs > 65536) {
+  if ((size_t)(kRowPairs * 3 + 1) * n2 * cs > 65536 || (size_t)(kColTile * (n0 + 1) + n0) * cs > 65536 ||
+      (size_t)(kColTile * (n1 + 1) + n1) * cs > 65536) {
     delete P;
     return fail(OFDFT_EINVAL, "grid dimension too large for the LDS tile sizes");
   }
@@ -1108,6 +1902,14 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->wtk) (void)hipFree(P->wtk);
   if (P->vloc) (void)hipFree(P->vloc);
   if (P->partials) (void)hipFree(P->partials);
+  if (P->d_knots) (void)hipFree(P->d_knots);
+  if (P->d_coefs) (void)hipFree(P->d_coefs);
+  if (P->d_splines) (void)hipFree(P->d_splines);
+  if (P->d_frac) (void)hipFree(P->d_frac);
+  if (P->d_pos) (void)hipFree(P->d_pos);
+  if (P->d_species) (void)hipFree(P->d_species);
+  if (P->d_bm) (void)hipFree(P->d_bm);
+  if (P->d_work) (void)hipFree(P->d_work);
   if (P->red) (void)hipFree(P->red);
   if (P->hred) (void)hipHostFree(P->hred);
   delete P;
@@ -1127,6 +1929,39 @@ long long ofdft_plan_workspace_bytes(ofdft_plan P) {
   if (P->vloc) b += (long long)P->rs * P->N;
   b += sizeof(double) * (P->npartials + kMaxSum);
   return b;
+}
+
+int ofdft_plan_set_profiling(ofdft_plan P, int on) {
+  if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
+  P->prof_on = on ? 1 : 0;
+  for (int i = 0; i < 8; ++i) {
+    P->prof_ms[i] = 0.0;
+    P->prof_count[i] = 0;
+    P->prof_bytes[i] = 0.0;
+  }
+  return 0;
+}
+
+int ofdft_plan_get_profile(ofdft_plan P, double* ms, long long* count, double* bytes) {
+  if (!valid(P) || !ms || !count || !bytes) return fail(OFDFT_EINVAL, "null argument");
+  if (!P->prof_pending.empty()) {
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (const auto& r : P->prof_pending) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, P->ev_pool[r.e0], P->ev_pool[r.e1]));
+      P->prof_ms[r.cls] += t;
+      P->prof_count[r.cls] += 1;
+      P->prof_bytes[r.cls] += (double)r.bytes;
+    }
+    P->prof_pending.clear();
+    P->ev_used = 0;
+  }
+  for (int i = 0; i < PC_NCLS; ++i) {
+    ms[i] = P->prof_ms[i];
+    count[i] = P->prof_count[i];
+    bytes[i] = P->prof_bytes[i];
+  }
+  return 0;
 }
 
 int ofdft_set_functional(ofdft_plan P, const ofdft_functional* f) {
@@ -1207,6 +2042,203 @@ int ofdft_convolve(ofdft_plan P, int kind, const void* in, void* out) {
   if (kind < OFDFT_CONV_NONE || kind > OFDFT_CONV_WT) return fail(OFDFT_EINVAL, "unknown convolution kind");
   if (P->prec == OFDFT_F64) return conv_impl<double>(P, kind, (const double*)in, (double*)out);
   return conv_impl<float>(P, kind, (const float*)in, (float*)out);
+}
+
+
+int ofdft_vloc_build(ofdft_plan P, int nspecies, const ofdft_vline* tables, int nat, const double* positions,
+                     const int* species, int pme, int order, void* vloc_out) {
+  if (!valid(P) || !tables || !positions || !species || !vloc_out) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "V_loc is built on an FP64 plan");
+  if (nspecies < 1 || nat < 1) return fail(OFDFT_EINVAL, "need at least one species and one atom");
+  if (pme && (order < 3 || order > kMaxOrder)) return fail(OFDFT_EINVAL, "B-spline order must be in [3, 12]");
+  for (int i = 0; i < nat; ++i)
+    if (species[i] < 0 || species[i] >= nspecies) return fail(OFDFT_EINVAL, "species index out of range");
+  for (int s = 0; s < nspecies; ++s)
+    if (tables[s].nknots < 8 || !tables[s].knots || !tables[s].coefs) return fail(OFDFT_EINVAL, "bad spline table");
+  int rc;
+  // spline tables
+  long long tot = 0;
+  for (int s = 0; s < nspecies; ++s) tot += tables[s].nknots;
+  if (P->d_knots) (void)hipFree(P->d_knots);
+  if (P->d_coefs) (void)hipFree(P->d_coefs);
+  if (P->d_splines) (void)hipFree(P->d_splines);
+  P->d_knots = P->d_coefs = nullptr;
+  P->d_splines = nullptr;
+  HIPCHK(hipMalloc((void**)&P->d_knots, sizeof(double) * tot));
+  HIPCHK(hipMalloc((void**)&P->d_coefs, sizeof(double) * tot));
+  std::vector<SplineDev> sd(nspecies);
+  long long off = 0;
+  for (int s = 0; s < nspecies; ++s) {
+    const ofdft_vline& t = tables[s];
+    HIPCHK(hipMemcpy(P->d_knots + off, t.knots, sizeof(double) * t.nknots, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P->d_coefs + off, t.coefs, sizeof(double) * t.nknots, hipMemcpyHostToDevice));
+    sd[s] = SplineDev{P->d_knots + off, P->d_coefs + off, t.nknots, t.q1, t.dq, t.qmax, t.v0, t.v1, t.v2};
+    off += t.nknots;
+  }
+  HIPCHK(hipMalloc(&P->d_splines, sizeof(SplineDev) * nspecies));
+  HIPCHK(hipMemcpy(P->d_splines, sd.data(), sizeof(SplineDev) * nspecies, hipMemcpyHostToDevice));
+  P->nspecies = nspecies;
+  P->pme = pme ? 1 : 0;
+  P->order = order;
+  if ((rc = upload_ions(P, nat, positions, species))) return rc;
+  GridGeom g = geom(P);
+  auto* acc = reinterpret_cast<cplx<double>*>(P->spec) + P->Nh;  // slot 1
+  const auto* spl = reinterpret_cast<const SplineDev*>(P->d_splines);
+  if (P->pme) {
+    if ((rc = ensure_work(P))) return rc;
+    if ((rc = ensure_bm(P, order))) return rc;
+    HIPCHK(hipMemsetAsync(acc, 0, sizeof(cplx<double>) * P->Nh, P->stream));
+    for (int s = 0; s < nspecies; ++s) {
+      HIPCHK(hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream));
+      hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(1024), 0, P->stream, P->d_work, P->d_frac, P->d_species, s,
+                         (const double*)nullptr, P->n[0], P->n[1], P->n[2], order);
+      LAUNCHCHK();
+      if ((rc = r2c_slot(P, P->d_work, 0))) return rc;
+      // v += vline * fft(Q) ; v *= B * N/Omega ; V = ifft(v): irfftn(rfftn(Q) vline B N/Omega)
+      hipLaunchKernelGGL(k_vloc_accum_pme, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, acc,
+                         reinterpret_cast<const cplx<double>*>(P->spec), g, spl, s, P->d_bm,
+                         (double)P->N / P->volume);
+      LAUNCHCHK();
+    }
+  } else {
+    // V = ifft(v) = irfftn(v)/dV
+    hipLaunchKernelGGL(k_vloc_exact, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, acc, g, spl, P->d_pos,
+                       P->d_species, nat, 1.0 / P->dV);
+    LAUNCHCHK();
+  }
+  HIPCHK(hipMemcpyAsync(P->spec, acc, sizeof(cplx<double>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
+  if ((rc = c2r_slot0(P, (double*)vloc_out))) return rc;
+  return 0;
+}
+
+int ofdft_local_forces(ofdft_plan P, const void* rho, double* forces_out) {
+  if (!valid(P) || !rho || !forces_out) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "forces are computed on an FP64 plan");
+  if (!P->d_splines || P->nat < 1) return fail(OFDFT_ESTATE, "call ofdft_vloc_build first");
+  if ((long long)3 * P->nat > P->npartials) return fail(OFDFT_EINVAL, "too many atoms for the partials buffer");
+  int rc;
+  GridGeom g = geom(P);
+  const auto* spl = reinterpret_cast<const SplineDev*>(P->d_splines);
+  auto* rhog = reinterpret_cast<cplx<double>*>(P->spec) + P->Nh;  // slot 1 = rfftn(rho)
+  if ((rc = r2c_slot(P, (const double*)rho, 1))) return rc;
+  std::vector<double> F(3 * (size_t)P->nat, 0.0);
+  if (!P->pme) {
+    hipLaunchKernelGGL(k_force_exact, dim3(P->nat), dim3(256), 0, P->stream, rhog, g, spl, P->d_pos, P->d_species,
+                       P->dV, P->partials);
+    LAUNCHCHK();
+    std::vector<double> h(3 * (size_t)P->nat);
+    HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (size_t i = 0; i < h.size(); ++i) F[i] = h[i] * 2.0 / P->volume;
+  } else {
+    if ((rc = ensure_work(P))) return rc;
+    if (!P->d_bm) return fail(OFDFT_ESTATE, "PME tables missing");
+    double linv[9];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) linv[a * 3 + b] = P->bg[b * 3 + a] / (2.0 * kPi);  // inv(L)
+    double* d_linv = P->red;  // 9 doubles of scratch (kMaxSum >= 9)
+    HIPCHK(hipMemcpyAsync(d_linv, linv, sizeof(linv), hipMemcpyHostToDevice, P->stream));
+    std::vector<double> h(3 * (size_t)P->nat);
+    for (int s = 0; s < P->nspecies; ++s) {
+      // rhoPB = ifft(fft(rho) conj(B) v_s) = irfftn(rfftn(rho) conj(B) v_s)
+      hipLaunchKernelGGL(k_force_pme_mult, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream,
+                         reinterpret_cast<cplx<double>*>(P->spec), rhog, g, spl, s, P->d_bm);
+      LAUNCHCHK();
+      if ((rc = c2r_slot0(P, P->d_work))) return rc;
+      HIPCHK(hipMemsetAsync(P->partials, 0, sizeof(double) * 3 * P->nat, P->stream));
+      hipLaunchKernelGGL(k_force_pme_gather, dim3(P->nat), dim3(1024), 0, P->stream, P->d_work, P->d_frac,
+                         P->d_species, s, P->n[0], P->n[1], P->n[2], P->order, d_linv, P->partials);
+      LAUNCHCHK();
+      HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+      HIPCHK(hipStreamSynchronize(P->stream));
+      for (int i = 0; i < P->nat; ++i) {
+        // partials of other species' atoms were zeroed and skipped
+        F[3 * i] += h[3 * i];
+        F[3 * i + 1] += h[3 * i + 1];
+        F[3 * i + 2] += h[3 * i + 2];
+      }
+    }
+  }
+  for (size_t i = 0; i < F.size(); ++i) forces_out[i] = F[i];
+  return 0;
+}
+
+int ofdft_ewald(ofdft_plan P, int nat, const double* positions, const double* charges, double eta, double rmax,
+                int pme, int order, double* e_real, double* e_rec) {
+  if (!valid(P) || !positions || !charges || !e_real || !e_rec) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "Ewald sums run on an FP64 plan");
+  if (nat < 1 || !(eta > 0) || !(rmax > 0)) return fail(OFDFT_EINVAL, "bad Ewald parameters");
+  if (pme && (order < 3 || order > kMaxOrder)) return fail(OFDFT_EINVAL, "B-spline order must be in [3, 12]");
+  int rc;
+  // image shifts T = (ix, iy, iz) . L with |i| <= ceil(rmax / |a_i|)
+  int Nimg[3];
+  for (int d = 0; d < 3; ++d) {
+    const double len = std::sqrt(P->lat[3 * d] * P->lat[3 * d] + P->lat[3 * d + 1] * P->lat[3 * d + 1] +
+                                 P->lat[3 * d + 2] * P->lat[3 * d + 2]);
+    Nimg[d] = (int)std::ceil(rmax / len);
+  }
+  std::vector<double> sh;
+  for (int a = -Nimg[0]; a <= Nimg[0]; ++a)
+    for (int b = -Nimg[1]; b <= Nimg[1]; ++b)
+      for (int c = -Nimg[2]; c <= Nimg[2]; ++c)
+        for (int d = 0; d < 3; ++d) sh.push_back(a * P->lat[d] + b * P->lat[3 + d] + c * P->lat[6 + d]);
+  const int nshift = (int)(sh.size() / 3);
+  double *d_pos = nullptr, *d_q = nullptr, *d_sh = nullptr, *d_frac = nullptr;
+  auto release = [&]() {
+    if (d_pos) (void)hipFree(d_pos);
+    if (d_q) (void)hipFree(d_q);
+    if (d_sh) (void)hipFree(d_sh);
+    if (d_frac) (void)hipFree(d_frac);
+  };
+  if (hipMalloc((void**)&d_pos, sizeof(double) * 3 * nat) != hipSuccess ||
+      hipMalloc((void**)&d_q, sizeof(double) * nat) != hipSuccess ||
+      hipMalloc((void**)&d_sh, sizeof(double) * sh.size()) != hipSuccess ||
+      hipMalloc((void**)&d_frac, sizeof(double) * 3 * nat) != hipSuccess) {
+    release();
+    return fail(OFDFT_ENOMEM, "Ewald buffers");
+  }
+  std::vector<double> frac(3 * (size_t)nat);
+  for (int i = 0; i < nat; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int e = 0; e < 3; ++e) f += positions[3 * i + e] * P->bg[d * 3 + e] / (2.0 * kPi);
+      frac[3 * i + d] = pymod1(pymod1(f));
+    }
+  hipError_t e1 = hipMemcpy(d_pos, positions, sizeof(double) * 3 * nat, hipMemcpyHostToDevice);
+  hipError_t e2 = hipMemcpy(d_q, charges, sizeof(double) * nat, hipMemcpyHostToDevice);
+  hipError_t e3 = hipMemcpy(d_sh, sh.data(), sizeof(double) * sh.size(), hipMemcpyHostToDevice);
+  hipError_t e4 = hipMemcpy(d_frac, frac.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    release();
+    return fail(OFDFT_EHIP, "Ewald upload");
+  }
+  const int nb = vec_blocks((long long)nat * nat);
+  hipLaunchKernelGGL(k_ewald_real, dim3(nb), dim3(256), 0, P->stream, d_pos, d_q, nat, d_sh, nshift, rmax,
+                     std::sqrt(eta), P->partials);
+  if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "Ewald real launch"); }
+  double er;
+  if ((rc = reduce_to_host(P, nb, 1, &er))) { release(); return rc; }
+  *e_real = er / 2.0;
+  GridGeom g = geom(P);
+  const cplx<double>* qhat = nullptr;
+  if (pme) {
+    if ((rc = ensure_work(P)) || (rc = ensure_bm(P, order))) { release(); return rc; }
+    if (hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream) != hipSuccess) { release(); return fail(OFDFT_EHIP, "memset"); }
+    hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(1024), 0, P->stream, P->d_work, d_frac, (const int*)nullptr, -1,
+                       d_q, P->n[0], P->n[1], P->n[2], order);
+    if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "spread launch"); }
+    if ((rc = r2c_slot(P, P->d_work, 0))) { release(); return rc; }
+    qhat = reinterpret_cast<const cplx<double>*>(P->spec);
+  }
+  const int nb2 = vec_blocks(P->Nh);
+  hipLaunchKernelGGL(k_ewald_rec, dim3(nb2), dim3(256), 0, P->stream, qhat, P->d_bm, g, d_pos, d_q, nat, eta,
+                     P->partials);
+  if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "Ewald rec launch"); }
+  double ek;
+  if ((rc = reduce_to_host(P, nb2, 1, &ek))) { release(); return rc; }
+  *e_rec = 4.0 * kPi * ek / P->volume;
+  release();
+  return 0;
 }
 
 int ofdft_dot3(ofdft_plan P, const void* a, const void* b, double* out3) {

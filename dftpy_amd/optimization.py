@@ -1,0 +1,435 @@
+"""Density optimization on phi = sqrt(rho), device resident.
+
+Mirrors ``Optimization`` of optimization/optimization.py:31-547 (defaults
+:64-78, TN :92-148, LBFGS :150-173 with math_utils.py:307-325, CG
+math_utils.py:327-361, orthogonalisation :205-226, value/derivative
+:228-280, main loop :283-525, convergence :527-540, chemical potential
+:542-544).  Every field operation is a HIP kernel on device buffers; only
+scalars (energies, dot products, line-search decisions) reach the host.
+
+With a TotalFunctional built from this package's functionals, each line-
+search trial is one fused evaluation at phi cos t + p sin t (the trial field
+is never materialised), and the TN inner step uses the exact Hessian-vector
+product (``hvp='exact'``, default) instead of the reference's forward
+difference (``hvp='fd'`` reproduces that one).
+
+Options beyond the reference's:
+  hvp            'exact' | 'fd'
+  precision      'fp64' | 'fp32' | 'mixed' (default: the dtype of guess_rho)
+  mixed_switch   mixed mode: leave FP32 once |dE| < mixed_switch (Ha);
+                 default 5e-10 * |E|
+  ls_first_step  'unit' (the reference code: first trial theta = 1) or
+                 'theta' (first trial = the running theta)
+"""
+import math
+import time
+
+import numpy as np
+import torch
+from scipy.optimize._dcsrch import DCSRCH
+
+from . import _lib as L
+from .field import DirectField
+from .functional import TotalFunctional, FunctionalOutput
+
+HARTREE_EV = 27.21138602436724
+
+
+class _Work:
+    """Device buffers of one precision."""
+
+    def __init__(self, ev):
+        self.ev = ev
+        self.P = P = ev.P
+        e = lambda: torch.empty(P.nr, dtype=P.dtype, device=P.device)
+        self.e = e
+        self.phi, self.V, self.Vt, self.res, self.res_prev, self.p = e(), e(), e(), e(), e(), e()
+        self.aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+        self.auxt = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+        self._tn = None
+
+    def tn(self):
+        if self._tn is None:
+            e = self.e
+            self._tn = dict(d=e(), r=e(), cgp=e(), Ap=e(), best=e(), tmp=e())
+        return self._tn
+
+
+class Optimization:
+    def __init__(self, optimization_method="CG-HS", optimization_options=None, EnergyEvaluator=None,
+                 guess_rho=None):
+        self.rho = guess_rho
+        self.optimization_options = {} if optimization_options is None else optimization_options
+        defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
+                    "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
+                    "ncheck": 2, "h0": 1.0, "hvp": "exact", "precision": None, "mixed_switch": None,
+                    "ls_first_step": "unit"}
+        for k, v in defaults.items():
+            self.optimization_options.setdefault(k, v)
+        if EnergyEvaluator is None:
+            raise AttributeError("Must provide an energy evaluator")
+        if not isinstance(EnergyEvaluator, TotalFunctional):
+            raise TypeError("the device optimizer needs a dftpy_amd TotalFunctional as EnergyEvaluator")
+        if self.optimization_options["algorithm"] != "EMM" or \
+                self.optimization_options["vector"] != "Orthogonalization":
+            raise AttributeError("only algorithm='EMM' with vector='Orthogonalization' is on the device path")
+        self.EnergyEvaluator = EnergyEvaluator
+        self.optimization_method = optimization_method
+        if optimization_method not in ("TN", "LBFGS", "GD") and not optimization_method.startswith("CG"):
+            raise AttributeError("The %s direction method not implemented." % optimization_method)
+        self.log = []
+        self.timings = {}
+        self.nfeval = 0
+        self.nhvp = 0
+        self.verbose = True
+
+    # ------------------------------------------------------------------
+    def _print(self, *a):
+        if self.verbose:
+            print(*a, flush=True)
+
+    def _eval(self, W, phi, p, t):
+        c, s = (1.0, 0.0) if p is None else (math.cos(t), math.sin(t))
+        self.nfeval += 1
+        return W.ev.eval_phi(phi, p, c, s, W.Vt, W.auxt)
+
+    def _ii(self):
+        ew = self.EnergyEvaluator.ewald
+        return ew.energy if ew else 0.0
+
+    def _energy(self, W, sums):
+        return float(sum(W.ev.energies(sums).values())) + self._ii_energy
+
+    # --- directions ---------------------------------------------------
+    def _direction_tn(self, W, mu):
+        """get_direction_TN (optimization.py:92-148) with an exact or a
+        forward-difference Hessian-vector product."""
+        P = W.P
+        T = W.tn()
+        d, r, cgp, Ap, best = T["d"], T["r"], T["cgp"], T["Ap"], T["best"]
+        d.zero_()
+        P.lincomb(-1.0, W.res, 0.0, W.res, r)
+        cgp.copy_(r)
+        r0 = P.dot3(r, r)[1]
+        rconv = r0 * 0.1
+        rl = [1e10]
+        eps = 1.0e-9
+        it = 0
+        for it in range(self.optimization_options["maxfun"]):
+            if self.optimization_options["hvp"] == "exact":
+                pAp = P.hvp(W.phi, cgp, W.V, W.aux, mu, Ap)
+                self.nhvp += 1
+            else:
+                # Ap = ((V(phi1) - mu) phi1 - res0) / eps, phi1 = phi + eps p
+                tmp = T["tmp"]
+                self.nfeval += 1
+                W.ev.eval_phi(W.phi, cgp, 1.0, eps, Ap, None)
+                P.lincomb(1.0, W.phi, eps, cgp, tmp)
+                P.residual(Ap, tmp, mu, Ap)
+                P.lincomb(1.0 / eps, Ap, -1.0 / eps, W.res, Ap)
+                pAp = P.dot3(cgp, Ap)[0]
+            if pAp < 0.0:
+                if it == 0:
+                    P.lincomb(r0 / pAp, cgp, 0.0, d, d)
+                else:
+                    self._print("!WARN : pAp small than zero :iter = ", it, pAp)
+                break
+            alpha = r0 / pAp
+            r1 = P.cg_update(alpha, cgp, Ap, d, r)
+            if r1 < min(rl):
+                best.copy_(d)
+            rl.append(r1)
+            if r1 < rconv:
+                break
+            elif r1 > 1000 * min(rl[:-1]):
+                d.copy_(best)
+                break
+            elif it > 10 and abs(r0 - r1) < 0.1 * r0:
+                d.copy_(best)
+                break
+            beta = r1 / r0
+            r0 = r1
+            P.lincomb(1.0, r, beta, cgp, cgp)
+        W.p.copy_(d)
+        return it + 1
+
+    def _direction_lbfgs(self, W):
+        """get_direction_LBFGS (optimization.py:150-173): two-loop recursion,
+        first loop skipping index 0 as the reference does."""
+        P = W.P
+        lb = self._lbfgs
+        q = W.p
+        P.lincomb(-1.0, W.res, 0.0, W.res, q)
+        n = len(lb["s"])
+        al = [0.0] * n
+        for i in range(n - 1, 0, -1):
+            a = lb["rho"][i] * P.dot3(lb["s"][i], q)[0]
+            al[i] = a
+            P.lincomb(-a, lb["y"][i], 1.0, q, q)
+        h0 = self.optimization_options["h0"]
+        if not h0:
+            gamma = 1.0 if n < 1 else P.dot3(lb["s"][-1], lb["y"][-1])[0] / P.dot3(lb["y"][-1], lb["y"][-1])[1]
+            P.lincomb(gamma, q, 0.0, q, q)
+        else:
+            P.lincomb(h0, q, 0.0, q, q)
+        for i in range(n):
+            b = lb["rho"][i] * P.dot3(lb["y"][i], q)[0]
+            P.lincomb(al[i] - b, lb["s"][i], 1.0, q, q)
+        return 1
+
+    def _direction_cg(self, W, first):
+        """get_direction_CG (math_utils.py:327-361)."""
+        P = W.P
+        m = self.optimization_method
+        if first or self._dir_prev is None:
+            W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
+            return 1
+        R, Rp, Dp = W.res, W.res_prev, self._dir_prev
+        t = W.tn()["tmp"]
+        P.lincomb(1.0, R, -1.0, Rp, t)  # R - Rp
+        if m in ("CG-HS", "CG"):
+            beta = P.dot3(R, t)[0] / P.dot3(Dp, t)[0]
+        elif m == "CG-PR":
+            beta = max(P.dot3(R, t)[0] / P.dot3(Rp, Rp)[1], 0.0)
+        elif m == "CG-DY":
+            beta = P.dot3(R, R)[1] / P.dot3(Dp, t)[0]
+        elif m == "CG-CD":
+            beta = -P.dot3(R, R)[1] / P.dot3(Dp, Rp)[0]
+        elif m == "CG-LS":
+            beta = P.dot3(R, t)[0] / P.dot3(Dp, Rp)[0]
+        else:
+            beta = P.dot3(R, R)[1] / P.dot3(Rp, Rp)[1]
+        P.lincomb(-1.0, R, beta, Dp, W.p)
+        return 1
+
+    # --- orthogonalisation and line search -----------------------------
+    def _orth_norm(self, W):
+        """OrthogonalNormalization (optimization.py:205-226)."""
+        P = W.P
+        dV = P.dV
+        pphi, _, phiphi = P.dot3(W.p, W.phi)
+        Ne = phiphi * dV
+        factor = pphi * dV / Ne
+        P.lincomb(-factor, W.phi, 1.0, W.p, W.p)
+        pn = P.dot3(W.p, W.p)[1] * dV
+        theta = math.sqrt(pn / Ne)
+        P.lincomb(math.sqrt(Ne / pn), W.p, 0.0, W.p, W.p)
+        return theta
+
+    def _line_search(self, W, E0, g0, theta):
+        o = self.optimization_options
+        cache = {}
+        count = [0]
+
+        def phi_(t):
+            count[0] += 1
+            sums = self._eval(W, W.phi, W.p, t)
+            cache["t"], cache["sums"] = t, sums
+            return self._energy(W, sums)
+
+        def dphi_(t):
+            if cache.get("t") != t:
+                phi_(t)
+            return float(cache["sums"][L.S_DTHETA])
+
+        a1 = theta if o["ls_first_step"] == "theta" else 1.0
+        ls = DCSRCH(phi_, dphi_, o["c1"], o["c2"], o["xtol"], 0.0, np.pi)
+        stp, _, _, task = ls(a1, phi0=E0, derphi0=g0, maxiter=100)
+        return stp, count[0], cache
+
+    # --- precision switching ------------------------------------------
+    def _work(self, prec):
+        grid = self.rho.grid
+        dt = torch.float64 if prec == "fp64" else torch.float32
+        key = (prec, id(grid))
+        if key not in self._works:
+            self._works[key] = _Work(self.EnergyEvaluator.evaluator(grid, dt))
+        return self._works[key]
+
+    def _start(self, W):
+        """Evaluate at W.phi and set V, aux, mu, residual."""
+        sums = self._eval(W, W.phi, None, 0.0)
+        W.V, W.Vt = W.Vt, W.V
+        W.aux, W.auxt = W.auxt, W.aux
+        E = self._energy(W, sums)
+        mu = sums[L.S_VRHO] / sums[L.S_RHO]
+        resN = W.P.residual(W.V, W.phi, mu, W.res) * W.P.dV
+        return E, mu, resN, sums
+
+    # ------------------------------------------------------------------
+    def optimize_rho(self, guess_rho=None, guess_phi=None, lphi=False):
+        if lphi:
+            raise AttributeError("lphi=True is not on the device path")
+        if guess_rho is None and self.rho is None and guess_phi is None:
+            raise AttributeError("Must provide a guess density")
+        if guess_rho is not None:
+            self.rho = guess_rho
+        if self.rho is None:
+            self.rho = guess_phi * guess_phi
+        o = self.optimization_options
+        prec = o["precision"]
+        if prec is None:
+            prec = "fp64" if self.rho.dtype == torch.float64 else "fp32"
+        if prec not in ("fp64", "fp32", "mixed"):
+            raise ValueError("precision must be fp64, fp32 or mixed")
+        self._works = {}
+        self._ii_energy = self._ii()
+        cur = "fp32" if prec in ("fp32", "mixed") else "fp64"
+        W = self._work(cur)
+        t0 = time.perf_counter()
+        with torch._C.DisableTorchFunctionSubclass():
+            if guess_phi is None:
+                rho = self.rho.to(torch.float64)
+                phi = torch.where(rho > 0, torch.sqrt(torch.where(rho > 0, rho, torch.ones_like(rho))),
+                                  torch.full_like(rho, 1e-300))
+                phi = phi * math.sqrt(float(rho.sum()) / float((phi * phi).sum()))
+            else:
+                phi = guess_phi.to(torch.float64)
+            W.phi.copy_(phi.to(W.P.dtype))
+            # electron count fixed in FP64 from the initial phi (optimization.py:305-315)
+            self._nelec = float((phi * phi).sum()) * W.P.dV
+        E, mu, resN, sums = self._start(W)
+        hist = [E]
+        self.log = [dict(step=0, energy=E, dE=E, dP=resN, Nd=1, Nls=1, time=time.perf_counter() - t0, prec=cur)]
+        self._print("{:8s}{:24s}{:16s}{:16s}{:8s}{:8s}{:16s}".format("Step", "Energy(a.u.)", "dE", "dP", "Nd", "Nls",
+                                                                      "Time(s)"))
+        self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(0, E, E, resN, 1, 1,
+                                                                                   self.log[-1]["time"]))
+        theta = 0.1
+        converged = 2
+        self._lbfgs = dict(s=[], y=[], rho=[])
+        self._dir_prev = None
+        first = True
+        switch = o["mixed_switch"]
+        if switch is None:
+            switch = 5e-10 * abs(E)
+        for it in range(1, o["maxiter"]):
+            m = self.optimization_method
+            if m == "TN":
+                nd = self._direction_tn(W, mu)
+            elif m == "LBFGS":
+                nd = self._direction_lbfgs(W)
+            elif m == "GD":
+                W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
+                nd = 1
+            else:
+                nd = self._direction_cg(W, first)
+            theta0 = self._orth_norm(W)
+            if W.P.dot_vphip(W.V, W.phi, W.p) * 2.0 > 0:
+                self._print("!WARN: Change to steepest decent")
+                W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
+                theta0 = self._orth_norm(W)
+            theta = min(theta0, theta)
+            g0 = W.P.dot_vphip(W.V, W.phi, W.p) * 2.0 * W.P.dV
+            stp, nls, cache = self._line_search(W, E, g0, theta)
+            failed = stp is None
+            if failed and not (prec == "mixed" and cur == "fp32"):
+                converged = 1
+                self._print("!!!ERROR : Line-Search Failed!!!")
+                self._print("!!!ERROR : Density Optimization NOT Converged  !!!")
+                break
+            if not failed:
+                theta = stp
+                t = cache["t"]
+                sums = cache["sums"]
+                old_phi = None
+                if m == "LBFGS":
+                    old_phi = W.phi.clone()
+                W.P.lincomb(math.cos(t), W.phi, math.sin(t), W.p, W.phi)
+                W.V, W.Vt = W.Vt, W.V
+                W.aux, W.auxt = W.auxt, W.aux
+                Enew = self._energy(W, sums)
+                mu = sums[L.S_VRHO] / sums[L.S_RHO]
+                W.res_prev.copy_(W.res)
+                resN = W.P.residual(W.V, W.phi, mu, W.res) * W.P.dV
+                if m == "LBFGS":
+                    self._lbfgs_update(W, old_phi)
+                if m.startswith("CG"):
+                    if self._dir_prev is None:
+                        self._dir_prev = W.e()
+                    self._dir_prev.copy_(W.p)
+                first = False
+                hist.append(Enew)
+                E = Enew
+                self.log.append(dict(step=it, energy=E, dE=hist[-1] - hist[-2], dP=resN, Nd=nd, Nls=nls,
+                                     time=time.perf_counter() - t0, prec=cur))
+                self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
+                    it, E, hist[-1] - hist[-2], resN, nd, nls, self.log[-1]["time"]))
+            # mixed precision: leave FP32 near its noise floor
+            if prec == "mixed" and cur == "fp32" and (failed or (len(hist) > 1 and abs(hist[-1] - hist[-2]) < switch)):
+                W32 = W
+                cur = "fp64"
+                W = self._work("fp64")
+                W.P.convert_from(W32.phi, W.phi)
+                # FP64 electron-count constraint: FP32 rotations drift the norm
+                nphi = W.P.dot3(W.phi, W.phi)[1] * W.P.dV
+                W.P.lincomb(math.sqrt(self._nelec / nphi), W.phi, 0.0, W.phi, W.phi)
+                E32 = hist[-1]
+                E, mu, resN, sums = self._start(W)
+                # the convergence test restarts on FP64 energies
+                hist = [E]
+                theta = 0.1
+                first = True
+                self._lbfgs = dict(s=[], y=[], rho=[])
+                self._dir_prev = None
+                self.log.append(dict(step=it, energy=E, dE=E - E32, dP=resN, Nd=0, Nls=0,
+                                     time=time.perf_counter() - t0, prec="switch->fp64"))
+                self._print("#### FP32 -> FP64 at step %d, E = %.12E" % (it, E))
+                continue
+            if self._check_converge(hist):
+                converged = 0
+                self._print("#### Density Optimization Converged ####")
+                break
+        else:
+            self._print("!WARN: Not converged, but reached max steps")
+        self.mu = mu
+        self._print("Chemical potential (a.u.):", mu)
+        self._print("Chemical potential (eV)  :", mu * HARTREE_EV)
+        grid = self.rho.grid
+        with torch._C.DisableTorchFunctionSubclass():
+            phi_out = W.phi.clone()
+            rho_out = phi_out * phi_out
+            Vout = W.V.clone()
+        self.phi = DirectField(grid=grid, data=phi_out, dtype=phi_out.dtype)
+        self.rho = DirectField(grid=grid, data=rho_out, dtype=rho_out.dtype)
+        self.functional = FunctionalOutput(name="TOTAL", energy=E,
+                                           potential=DirectField(grid=grid, data=Vout, dtype=Vout.dtype))
+        self.energy = E
+        self.converged = converged
+        self.elapsed = time.perf_counter() - t0
+        return self.rho
+
+    def _lbfgs_update(self, W, old_phi):
+        """LBFGS.update (math_utils.py:315-325): keep Bound+1 pairs."""
+        lb = self._lbfgs
+        P = W.P
+        if len(lb["s"]) > self.optimization_options["maxcor"]:
+            lb["s"].pop(0)
+            lb["y"].pop(0)
+            lb["rho"].pop(0)
+        s = W.e()
+        y = W.e()
+        P.lincomb(1.0, W.phi, -1.0, old_phi, s)
+        P.lincomb(1.0, W.res, -1.0, W.res_prev, y)
+        lb["s"].append(s)
+        lb["y"].append(y)
+        lb["rho"].append(1.0 / P.dot3(y, s)[0])
+
+    def _check_converge(self, hist):
+        """check_converge (optimization.py:527-540)."""
+        econv = self.optimization_options["econv"]
+        nc = self.optimization_options["ncheck"]
+        E = hist[-1]
+        if econv is not None:
+            if len(hist) - 1 < nc:
+                return False
+            for i in range(nc):
+                if abs(hist[-2 - i] - E) > econv:
+                    return False
+        return True
+
+    def get_chemical_potential(self, potential, rho, **kwargs):
+        return (potential * rho).integral() / rho.integral()
+
+    def __call__(self, guess_rho=None, calcType=("E", "V"), guess_phi=None, lphi=False):
+        return self.optimize_rho(guess_rho=guess_rho, guess_phi=guess_phi, lphi=lphi)

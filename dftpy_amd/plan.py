@@ -194,5 +194,18 @@ class Plan:
         L.check(self.lib.ofdft_convert(self._h, L.ptr(dst), sp, L.ptr(src), dst.numel()))
         return dst
 
+    PROFILE_CLASSES = ("rows_fwd", "axis_c2c", "pencil_conv", "rows_inv", "vector")
+
+    def set_profiling(self, on=True):
+        L.check(self.lib.ofdft_plan_set_profiling(self._h, 1 if on else 0))
+
+    def get_profile(self):
+        """{class: (ms_total, launches, algorithmic_bytes_total)}"""
+        ms = L.darr(5)
+        cnt = (ctypes.c_longlong * 5)()
+        by = L.darr(5)
+        L.check(self.lib.ofdft_plan_get_profile(self._h, ms, cnt, by))
+        return {k: (ms[i], int(cnt[i]), by[i]) for i, k in enumerate(self.PROFILE_CLASSES)}
+
     def workspace_bytes(self):
         return int(self.lib.ofdft_plan_workspace_bytes(self._h))

@@ -1,0 +1,271 @@
+"""Local pseudopotential and Ewald ion-ion energy on the GPU.
+
+LocalPseudo mirrors functional/pseudo/__init__.py:74-571: PP_list maps a
+symbol to a CASTEP .recpot file (pseudo/recpot.py:11-41) or to a
+``RadialPseudo`` table (the BasePseudo injection path,
+abstract_pseudo.py:49-111).  The radial table is fitted once on the host
+with the reference's own call (scipy ``splrep(gp[1:], vp[1:], k=3)``,
+pseudo/__init__.py:696); V_loc(r), forces and the Ewald sums run in HIP
+kernels (``ofdft_vloc_build``, ``ofdft_local_forces``, ``ofdft_ewald``).
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+from scipy.interpolate import splrep
+from scipy import special
+
+from . import _lib as L
+from .field import DirectField
+from .functional import AbstractFunctional, FunctionalOutput
+from .workloads import BOHR, HARTREE
+
+
+class RadialPseudo:
+    """Reciprocal-space radial local PP: q grid ``r`` (bohr^-1), values
+    ``v`` (Ha bohr^3), valence ``zval`` (BasePseudo attributes)."""
+
+    def __init__(self, r, v, zval, info=None):
+        self.r = np.asarray(r, dtype=np.float64)
+        self.v = np.asarray(v, dtype=np.float64)
+        self._zval = zval
+        self.info = info or {}
+
+    @property
+    def zval(self):
+        return self._zval
+
+    radial_grid = property(lambda self: self.r)
+    local_potential = property(lambda self: self.v)
+
+
+def read_recpot(fname):
+    """CASTEP recpot reader (pseudo/recpot.py:11-41)."""
+    with open(fname) as fh:
+        lines = fh.readlines()
+    ibegin, iend = 0, None
+    for i, line in enumerate(lines):
+        if "END COMMENT" in line:
+            ibegin = i + 3
+        elif ibegin > 1 and (line.strip() == "1000" or len(line.strip()) == 1):
+            iend = i
+            break
+    if iend is None:
+        raise AttributeError("Error : Check the PP file : {}".format(fname))
+    body = " ".join(l.strip() for l in lines[ibegin:iend])
+    gmax = float(lines[ibegin - 1].strip()) * BOHR
+    v = np.array(body.split()).astype(np.float64) / HARTREE / BOHR ** 3
+    r = np.linspace(0, gmax, num=len(v))
+    zval = round((v[0] - v[1]) * (r[1] ** 2) / (4.0 * np.pi))
+    return RadialPseudo(r, v, zval)
+
+
+def empty_core(Z, rc, qmax=30.0, npts=2001):
+    """Ashcroft empty-core table as a RadialPseudo (zval recovered as
+    recpot.py:41 does)."""
+    from .workloads import empty_core_table
+    q, v = empty_core_table(Z, rc, qmax, npts)
+    zval = round((v[0] - v[1]) * (q[1] ** 2) / (4.0 * np.pi))
+    return RadialPseudo(q, v, zval)
+
+
+def _load_pp(item):
+    if isinstance(item, RadialPseudo):
+        return item
+    if hasattr(item, "radial_grid") and hasattr(item, "local_potential"):
+        return RadialPseudo(item.radial_grid, item.local_potential, item.zval)
+    fname = os.fspath(item)
+    if not os.path.isfile(fname):
+        raise FileNotFoundError("'{}' PP file not found".format(fname))
+    if fname.lower().endswith(".recpot"):
+        return read_recpot(fname)
+    raise AttributeError("Pseudopotential '{}' is not supported on the device path".format(fname))
+
+
+class LocalPseudo(AbstractFunctional):
+    """Local pseudopotential term (pseudo/__init__.py:74-271)."""
+
+    def __init__(self, grid=None, ions=None, PP_list=None, PME=True, BsplineOrder=10, **kwargs):
+        self.type = "PSEUDO"
+        self.name = "PSEUDO"
+        if PP_list is None:
+            raise AttributeError("Must specify PP_list for Pseudopotentials")
+        if grid is None:
+            raise AttributeError("Must specify grid for Pseudopotentials")
+        if ions is None:
+            raise AttributeError("Must specify ions for Pseudopotentials")
+        self.PP_list = dict(PP_list)
+        self.pps = {k: _load_pp(v) for k, v in self.PP_list.items()}
+        for s in ions.symbols_uniq:
+            if s not in self.pps:
+                raise ValueError("There is no pseudopotential for {:4s} atom".format(s))
+        self.grid = grid
+        self.ions = ions
+        self.ions.set_charges({k: p.zval for k, p in self.pps.items()})
+        self.PME = PME
+        self.BsplineOrder = BsplineOrder
+        self._vreal = None
+        self._vreal32 = None
+        self._ewald = None
+        self._plan = None
+
+    @property
+    def zval(self):
+        return {k: p.zval for k, p in self.pps.items()}
+
+    def _fp64_plan(self):
+        if self._plan is None:
+            from .plan import Plan
+            g = self.grid
+            self._plan = Plan(tuple(int(x) for x in g.nrR), g.lattice, precision="fp64", full=g.full,
+                              device=g.get_device())
+        return self._plan
+
+    def _tables(self):
+        keys = sorted(self.pps)
+        ctabs, keep = [], []
+        for k in keys:
+            p = self.pps[k]
+            t, c, _ = splrep(p.r[1:], p.v[1:], k=3)
+            t = np.ascontiguousarray(t, dtype=np.float64)
+            c = np.ascontiguousarray(c, dtype=np.float64)
+            keep += [t, c]
+            ctabs.append(L.VLine(t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(t), p.r[1],
+                                 p.r[1] - p.r[0], p.r[-1], p.v[0], p.v[1], p.v[2]))
+        arr = (L.VLine * len(ctabs))(*ctabs)
+        return keys, arr, keep
+
+    def local_PP(self):
+        """V_loc(r) on the device (pseudo/__init__.py:237-246)."""
+        if self._vreal is not None:
+            return
+        P = self._fp64_plan()
+        keys, arr, keep = self._tables()
+        idx = {k: i for i, k in enumerate(keys)}
+        species = np.array([idx[s] for s in self.ions.symbols], dtype=np.int32)
+        pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
+        out = DirectField(grid=self.grid, dtype=torch.float64)
+        P._stream()
+        with torch._C.DisableTorchFunctionSubclass():
+            L.check(P.lib.ofdft_vloc_build(P._h, len(keys), arr, self.ions.nat,
+                                           pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           species.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           1 if self.PME else 0, int(self.BsplineOrder), L.ptr(out)))
+        torch.cuda.current_stream(P.device).synchronize()
+        del keep
+        self._vreal = out
+
+    @property
+    def vreal(self):
+        self.local_PP()
+        return self._vreal
+
+    def vreal_device(self, dtype):
+        """V_loc as a plain device tensor in the requested precision."""
+        v = self.vreal
+        with torch._C.DisableTorchFunctionSubclass():
+            if dtype == torch.float64:
+                return v.contiguous()
+            if self._vreal32 is None:
+                self._vreal32 = v.to(torch.float32).contiguous()
+            return self._vreal32
+
+    def _device_terms(self):
+        return dict(pseudo=True, obj=self, owner=self)
+
+    def compute(self, density, calcType=("E", "V"), **kwargs):
+        """E = int V_loc rho (pseudo/__init__.py:219-234)."""
+        from .functional import TotalFunctional
+        if getattr(self, "_tot", None) is None:
+            self._tot = TotalFunctional(PSEUDO=self)
+            self._tot._ewald = False
+        ev = self._tot.evaluator(density.grid, density.dtype)
+        with torch._C.DisableTorchFunctionSubclass():
+            sums = ev.eval_rho(density.contiguous(), None)
+        out = FunctionalOutput(name="eN", energy=float(sums[L.E_PP]))
+        if "V" in calcType:
+            out.potential = self.vreal if density.dtype == torch.float64 else \
+                DirectField(grid=self.grid, data=self.vreal_device(density.dtype))
+        return out
+
+    def forces(self, rho):
+        """Local-PP ionic forces (pseudo/__init__.py:260-270), Ha/bohr."""
+        if rho is None:
+            raise AttributeError("Must specify rho")
+        if not isinstance(rho, DirectField):
+            raise TypeError("rho must be DirectField")
+        self.local_PP()
+        P = self._fp64_plan()
+        out = np.zeros((self.ions.nat, 3))
+        with torch._C.DisableTorchFunctionSubclass():
+            r = rho.to(torch.float64).contiguous()
+            P._stream()
+            L.check(P.lib.ofdft_local_forces(P._h, L.ptr(r), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def get_ewald(self, PME=None):
+        if self._ewald is None:
+            self._ewald = Ewald(ions=self.ions, grid=self.grid, PME=self.PME if PME is None else PME,
+                                BsplineOrder=self.BsplineOrder, plan=self._fp64_plan())
+        return self._ewald
+
+
+class Ewald:
+    """Ion-ion energy (ewald.py:197-507): eta from Get_Best_eta (:256-278),
+    Energy_corr (:468-482) on the host; the real-space and reciprocal sums
+    in HIP kernels."""
+
+    def __init__(self, precision=1.0e-8, ions=None, grid=None, PME=False, BsplineOrder=10, plan=None, **kwargs):
+        if ions is None:
+            raise AttributeError("Must pass ions to Ewald")
+        if grid is None:
+            raise AttributeError("Must pass grid or rho to Ewald")
+        self.ions = ions
+        self.grid = grid
+        self.precision = precision
+        self.PME = PME
+        self.order = BsplineOrder
+        self._plan = plan
+        gmax = math.sqrt(grid.get_reciprocal().g2max)
+        self.eta = self.Get_Best_eta(precision, gmax, ions)
+        self._energy = None
+
+    def Get_Best_eta(self, precision, gmax, ions):
+        q2 = float(np.sum(ions.charges * ions.charges))
+        eta = 1.6
+        while True:
+            ub = 4.0 * np.pi * ions.nat * q2 * np.sqrt(eta / np.pi) * special.erfc(gmax / 2.0 * np.sqrt(1.0 / eta))
+            if ub < precision:
+                return eta
+            eta = eta - 0.01
+
+    def Energy_corr(self):
+        dc = -np.sqrt(self.eta / np.pi) * float(np.sum(self.ions.charges ** 2))
+        gz = -4.0 * np.pi * (1.0 / (4.0 * self.eta * self.grid.volume) / 2.0) * self.ions.get_ncharges() ** 2
+        return dc + gz
+
+    def _sums(self):
+        if self._plan is None:
+            from .plan import Plan
+            g = self.grid
+            self._plan = Plan(tuple(int(x) for x in g.nrR), g.lattice, "fp64", full=g.full, device=g.get_device())
+        P = self._plan
+        rmax = special.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+        pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
+        q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
+        er, ek = ctypes.c_double(), ctypes.c_double()
+        P._stream()
+        L.check(P.lib.ofdft_ewald(P._h, self.ions.nat, pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  q.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(self.eta), float(rmax),
+                                  1 if self.PME else 0, int(self.order), ctypes.byref(er), ctypes.byref(ek)))
+        return er.value, ek.value
+
+    @property
+    def energy(self):
+        if self._energy is None:
+            er, ek = self._sums()
+            self._energy = self.Energy_corr() + er + ek
+        return self._energy

@@ -84,6 +84,18 @@ int ofdft_plan_set_stream(ofdft_plan plan, void* hip_stream);
 /* Device bytes of workspace the plan owns. */
 long long ofdft_plan_workspace_bytes(ofdft_plan plan);
 
+/* Per-kernel-class timing with HIP events recorded on the plan stream
+ * around every launch (no reference counterpart: it replaces the
+ * wall-clock @timer labels of time_data.py:6-122 with device time).
+ * Classes: 0 rows_fwd (R2C + pre), 1 axis (strided C2C), 2 pencil
+ * (axis-0 C2C + multiplier + inverse), 3 rows_inv (C2R + post),
+ * 4 vector kernels.  get_profile synchronises and returns, per class
+ * (arrays of 5), total milliseconds, launch count and the algorithmic
+ * bytes (each launch's compulsory reads + writes) summed over launches.
+ * set_profiling resets the counters.                                     */
+int ofdft_plan_set_profiling(ofdft_plan plan, int on);
+int ofdft_plan_get_profile(ofdft_plan plan, double* ms, long long* count, double* bytes);
+
 /* Functional options; builds the WT kernel table on the device when
  * f->wt (get_ke_kernel_wt, kedf/wt.py:58-73; WTKernel kernel.py:325-334;
  * LindhardFunction kernel.py:14-136).                                      */
@@ -130,6 +142,44 @@ int ofdft_fft_c2r(ofdft_plan plan, const void* in_cplx, void* out_real);
 #define OFDFT_CONV_GG 2
 #define OFDFT_CONV_WT 3
 int ofdft_convolve(ofdft_plan plan, int kind, const void* in_real, void* out_real);
+
+/* ---- ions (FP64 plans) -------------------------------------------------
+ * Radial local-PP table of one species: the cubic interpolating spline of
+ * v(q) fitted without the q=0 point (ReadPseudo.get_vloc_interp,
+ * pseudo/__init__.py:689-697; knots/coefs as scipy splrep returns them,
+ * both of length nknots, degree 3), plus the q-grid and v values the
+ * quartic small-q branch uses (calc_vlines :413-420).                    */
+typedef struct {
+  const double* knots;  /* host */
+  const double* coefs;  /* host */
+  int nknots;
+  double q1;    /* gp[1]          */
+  double dq;    /* gp[1] - gp[0]  */
+  double qmax;  /* gp[-1]         */
+  double v0, v1, v2;  /* vp[0], vp[1], vp[2] */
+} ofdft_vline;
+
+/* V_loc(r) for nat ions (positions host, Cartesian bohr, nat*3; species
+ * index per ion) with PME (pme=1, B-spline order `order`,
+ * _PP_Reciprocal_PME pseudo/__init__.py:437-452) or the exact structure
+ * factor (pme=0, _PP_Reciprocal :425-435).  Writes N doubles to vloc_out
+ * (device) and keeps the ion state for ofdft_local_forces.              */
+int ofdft_vloc_build(ofdft_plan plan, int nspecies, const ofdft_vline* tables, int nat, const double* positions,
+                     const int* species, int pme, int order, void* vloc_out);
+
+/* Local-PP ionic forces at density rho (device, N doubles) for the ions
+ * of the last ofdft_vloc_build: _ForcePME (pseudo/__init__.py:523-571) or
+ * _Force (:505-521).  forces_out: host nat*3, Ha/bohr.                   */
+int ofdft_local_forces(ofdft_plan plan, const void* rho, double* forces_out);
+
+/* Ewald ion-ion sums (ewald.py:197-507): real-space pair sum with cutoff
+ * rmax over periodic images, and the reciprocal sum over the half-space
+ * mask with the PME structure factor (pme=1, Energy_rec_PME :785-807) or
+ * the exact one (pme=0, Energy_rec :448-465).  e_real already halved;
+ * e_rec includes 4 pi/Omega.  The G=0 / self terms (Energy_corr) are host
+ * arithmetic on the charges.                                             */
+int ofdft_ewald(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta, double rmax,
+                int pme, int order, double* e_real, double* e_rec);
 
 /* ---- vector kernels used by the optimizer (optimization.py:92-280) ---- */
 /* out[0] = sum(a*b), out[1] = sum(a*a), out[2] = sum(b*b) in FP64 (host).  */

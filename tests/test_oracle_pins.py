@@ -1,0 +1,269 @@
+"""Pins of the FP64 oracle against the reference's known answers, saved
+outputs and closed forms (CPU only).  Each check is chosen so that a
+plausible mistake in the restatement (a dropped term, a wrong sign or
+factor, a transposed operand) fails it; none of them re-types the oracle's
+own formula."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.conftest import DATA
+
+NOTEBOOK_E = [-8.090977710718, -8.273226052119, -8.280858141000, -8.281101062862, -8.281135978745,
+              -8.281138625688, -8.281138996216]
+NOTEBOOK_ND = [2, 7, 5, 6, 5, 8]
+NOTEBOOK_DP = [7.877088e-01, 7.033208e-02, 4.803522e-03, 3.640550e-04, 3.590863e-05, 2.500624e-06, 4.567996e-08]
+
+
+@pytest.fixture(scope="module")
+def fixture_rho():
+    return O.read_qepp(os.path.join(DATA, "Al_fde_rho.pp"), full=True)
+
+
+def test_wt_known_answer(fixture_rho):
+    ions, rho, grid = fixture_rho
+    e = O.tf_functional(rho, grid)[0] + O.vw_functional(rho, grid)[0] + O.wt_functional(rho, grid)[0]
+    assert abs(e - 2.916818700014412) < 1e-12  # test_functional.py:27
+
+
+def test_read_pp_known_answer_and_pme_vs_exact(fixture_rho):
+    ions, rho, grid = fixture_rho
+    pp = O.LocalPseudo(grid, ions, {"Al": O.read_recpot(os.path.join(DATA, "al.lda.recpot"))}, PME=True)
+    assert abs(pp.compute(rho)[0] - 1.3497046) < 1e-4  # test_read_pp.py:12
+    tab = O.read_recpot(os.path.join(DATA, "Al_lda.oe01.recpot"))
+    a = O.LocalPseudo(grid, ions, {"Al": tab}, PME=False)
+    b = O.LocalPseudo(grid, ions, {"Al": tab}, PME=True)
+    assert abs(a.compute(rho)[0] - b.compute(rho)[0]) < 1e-4  # test_pme.py:32
+    assert np.allclose(a.forces(rho), b.forces(rho), atol=1e-4)  # test_pme.py:34
+
+
+def test_exact_structure_factor_brute_force():
+    """V_loc from _PP_Reciprocal equals a direct sum over G and atoms (odd
+    sizes: no Nyquist planes, so the half and full grids coincide)."""
+    cell = np.diag([5.0, 6.0, 7.0])
+    nr = (7, 9, 11)
+    ions = O.Ions(["Al", "Al"], [[0.3, 0.2, 0.1], [2.0, 3.1, 4.7]], cell)
+    tab = O.empty_core_table(3.0, 1.12)
+    fg = O.Grid(cell, nr, full=True)
+    G = fg.g.reshape(3, -1).T
+    vq = O.ofdft_oracle.vline(np.sqrt((G ** 2).sum(1)), tab)
+    r = np.array([[5.0 * i / 7, 6.0 * j / 9, 7.0 * k / 11] for i in range(7) for j in range(9) for k in range(11)])
+    S = sum(np.exp(-1j * G @ R) for R in ions.positions)
+    Vd = (np.exp(1j * r @ G.T) @ (vq * S)).real / fg.volume
+    for full in (False, True):
+        grid = O.Grid(cell, nr, full=full)
+        V = O.LocalPseudo(grid, ions, {"Al": tab}, PME=False).vreal
+        assert np.abs(V.ravel() - Vd).max() < 1e-10
+
+
+def test_ewald_known_answers(fixture_rho):
+    ions, rho, grid = fixture_rho
+    ew = O.Ewald(grid, ions, PME=False)
+    assert abs(ew.energy_corr() - (-13.003839613564997)) < 1e-10  # test_ewald.py:33
+    assert abs(ew.energy_real() - 0.0) < 1e-5
+    assert abs(ew.energy_rec() - 7.606943756474895) < 1e-10
+    assert abs(O.Ewald(grid, ions, PME=True).energy_rec_pme() - 7.606943756474895) < 1e-5
+
+
+def test_ewald_real_space_brute_force():
+    """Real-space Ewald sum equals a direct image loop on a tiny cell."""
+    from scipy.special import erfc
+    cell = np.eye(3) * 4.0
+    grid = O.Grid(cell, (8, 8, 8))
+    ions = O.Ions(["Al", "Al"], [[0, 0, 0], [1.1, 2.0, 1.7]], cell, charges=[3.0, 3.0])
+    ew = O.Ewald(grid, ions, PME=False)
+    ew.eta = 0.3
+    rmax = O.ofdft_oracle.sp.erfcinv(ew.precision / 3.0) / np.sqrt(ew.eta)
+    tot = 0.0
+    n = int(np.ceil(rmax / 4.0)) + 1
+    for i in range(2):
+        for j in range(2):
+            for a in range(-n, n + 1):
+                for b in range(-n, n + 1):
+                    for c in range(-n, n + 1):
+                        d = np.linalg.norm(ions.positions[j] + 4.0 * np.array([a, b, c]) - ions.positions[i])
+                        if 0.001 < d < rmax:
+                            tot += 9.0 * erfc(np.sqrt(0.3) * d) / d
+    assert abs(ew.energy_real() - tot / 2) < 1e-12
+
+
+def _notebook(first):
+    a = 4.05 / O.BOHR
+    cell = np.eye(3) * a
+    pos = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a
+    ions = O.Ions(["Al"] * 4, pos, cell)
+    nr = O.ecut2nr(cell, ecut=35)
+    assert list(nr) == [20, 20, 20]
+    grid = O.Grid(cell, nr, full=True)
+    pp = O.LocalPseudo(grid, ions, {"Al": O.read_recpot(os.path.join(DATA, "Al_lda.oe01.recpot"))}, PME=True)
+    tot = O.TotalFunctional(grid, kedf="TFVW", pseudo=pp, ewald=O.Ewald(grid, ions, PME=True))
+    rho0 = np.full(tuple(nr), ions.get_ncharges() / grid.volume)
+    opt = O.Optimization(lambda r, c, p: tot.compute(r, c, phi=p), grid, "TN",
+                         {"econv": 4e-6, "ls_first_step": first})
+    opt.optimize_rho(rho0)
+    return opt
+
+
+def test_notebook_trajectory():
+    """density_optimization.ipynb:174-183: every logged iteration."""
+    opt = _notebook("theta")
+    E = [l[1] for l in opt.log]
+    assert [l[4] for l in opt.log[1:]] == NOTEBOOK_ND
+    assert all(l[5] == 1 for l in opt.log)
+    assert np.allclose(E, NOTEBOOK_E, rtol=0, atol=2e-9)
+    assert np.allclose([l[3] for l in opt.log], NOTEBOOK_DP, rtol=2e-5)
+    assert abs(opt.mu - 0.30115196532053357) < 1e-9
+
+
+def test_notebook_minimum_with_reference_line_search():
+    opt = _notebook("unit")
+    assert abs(opt.energy - (-8.2811389962163)) < 1e-9
+
+
+def test_gaas_density_optimization_known_answer():
+    """test_density_opt.py:16-46: WT+LDA+H+PME PP on random GaAs."""
+    ions = O.read_poscar(os.path.join(DATA, "GaAs_random.vasp"))
+    nr = O.ecut2nr(ions.cell, spacing=0.4)
+    grid = O.Grid(ions.cell, nr, full=False)
+    tabs = {"Ga": O.read_recpot(os.path.join(DATA, "Ga_lda.oe04.recpot")),
+            "As": O.read_recpot(os.path.join(DATA, "As_lda.oe04.recpot"))}
+    pp = O.LocalPseudo(grid, ions, tabs, PME=True)
+    tot = O.TotalFunctional(grid, kedf="WT", pseudo=pp, ewald=O.Ewald(grid, ions, PME=True))
+    rho0 = np.full(tuple(nr), ions.get_ncharges() / grid.volume)
+    opt = O.Optimization(lambda r, c, p: tot.compute(r, c, phi=p), grid, "TN",
+                         {"econv": 1e-6 * ions.nat, "maxfun": 50, "maxiter": 100})
+    opt.optimize_rho(rho0)
+    assert np.isclose(tot.compute(opt.rho, ("E",))[0], -43.3046)
+
+
+# ----------------------------- closed forms ---------------------------------
+def _cos_grid():
+    cell = np.diag([6.0, 7.0, 8.0])
+    grid = O.Grid(cell, (12, 14, 16), full=False)
+    x = np.arange(12)[:, None, None] / 12.0
+    return cell, grid, x
+
+
+def test_hartree_single_cosine():
+    cell, grid, x = _cos_grid()
+    G = 2 * np.pi * 2 / 6.0
+    d = 0.01
+    rho = 0.05 + d * np.cos(2 * np.pi * 2 * x) * np.ones((12, 14, 16))
+    E, V = O.hartree_functional(rho, grid)
+    assert abs(E - np.pi * grid.volume * d ** 2 / G ** 2) < 1e-12
+    assert np.abs(V - 4 * np.pi * d / G ** 2 * np.cos(2 * np.pi * 2 * x)).max() < 1e-12
+
+
+def test_vw_closed_forms():
+    cell, grid, x = _cos_grid()
+    rho = np.full((12, 14, 16), 0.03)
+    E, V = O.vw_functional(rho, grid)
+    assert abs(E) < 1e-14 and np.abs(V).max() < 1e-12
+    a, b, G = 0.2, 0.01, 2 * np.pi * 3 / 6.0
+    s = a + b * np.cos(2 * np.pi * 3 * x) * np.ones((12, 14, 16))
+    E, V = O.vw_functional(s * s, grid)
+    assert abs(E - b * b * G * G * grid.volume / 4) < 1e-12
+    assert np.abs(V - 0.5 * b * G * G * np.cos(2 * np.pi * 3 * x) / s).max() < 1e-12
+
+
+def test_tf_uniform_gas():
+    grid = O.Grid(np.eye(3) * 5.0, (8, 8, 8))
+    r = 0.02
+    E, V = O.tf_functional(np.full((8, 8, 8), r), grid)
+    ctf = 0.3 * (3 * np.pi ** 2) ** (2 / 3)
+    assert abs(E - ctf * r ** (5 / 3) * 125.0) < 1e-13
+    assert np.allclose(V, 5 / 3 * ctf * r ** (2 / 3), rtol=1e-14)
+
+
+def test_wt_uniform_density_has_no_energy():
+    """F(eta=0; lambda=1) = 0: the WT term vanishes for uniform rho."""
+    grid = O.Grid(np.eye(3) * 5.0, (8, 8, 8))
+    E, V = O.wt_functional(np.full((8, 8, 8), 0.02), grid)
+    assert abs(E) < 1e-15
+    assert np.abs(V).max() < 1e-13
+
+
+def test_lindhard_branch_limits():
+    lam, mu = 1.0, 1.0
+    e = np.array([1e-12, 1e-4, 1.0, 1.0 + 1e-6, 3.6499999, 3.6500001, 10.0])
+    F = O.lindhard(e, lam, mu)
+    assert abs(F[0] - (1 - lam)) < 1e-15
+    assert abs(F[1] - (1 - lam + 1e-8 * (1 / 3 - 3 * mu))) < 1e-9
+    assert abs(F[2] - (2 - lam - 3 * mu)) < 1e-12
+    assert abs(F[3] - F[2]) < 1e-4
+    assert abs(F[4] - F[5]) < 1e-6  # series and closed form agree at the switch
+    # large eta: 1/F_Lindhard -> 3 eta^2 + ... so F - (3(1-mu) eta^2 - lam) -> -0.6
+    assert abs(O.lindhard(np.array([1e3]), 0.0, 0.0)[0] - 3e6 + 0.6) < 1e-6
+
+
+def _fd_potential(fun, rho, grid, h=1e-6):
+    rng = np.random.default_rng(7)
+    d = rng.standard_normal(rho.shape) * rho * 0.1
+    Ep = fun(rho + h * d)[0]
+    Em = fun(rho - h * d)[0]
+    V = fun(rho)[1]
+    return (Ep - Em) / (2 * h), np.sum(V * d) * grid.dV
+
+
+@pytest.mark.parametrize("name", ["tf", "vw", "wt", "lda", "hartree"])
+def test_potential_is_energy_derivative(name):
+    cell = np.diag([6.0, 6.5, 7.0])
+    grid = O.Grid(cell, (12, 14, 16), full=False)
+    rng = np.random.default_rng(1)
+    rho = 0.03 * (1 + 0.2 * rng.random((12, 14, 16)))
+    cache = O.ofdft_oracle.KernelCache()
+    fun = {"tf": lambda r: O.tf_functional(r, grid), "vw": lambda r: O.vw_functional(r, grid),
+           "wt": lambda r: O.wt_functional(r, grid, rho0=0.033, cache=cache),
+           "lda": lambda r: O.lda_functional(r, grid), "hartree": lambda r: O.hartree_functional(r, grid)}[name]
+    fd, an = _fd_potential(fun, rho, grid)
+    assert abs(fd - an) < 1e-7 * max(1.0, abs(an))
+
+
+def test_lda_second_derivative():
+    rho = np.linspace(0.001, 2.0, 200).reshape(200, 1, 1)
+    grid = O.Grid(np.eye(3), (200, 1, 1))
+    h = 1e-6 * rho
+    Vp = O.lda_functional(rho + h, grid)[1]
+    Vm = O.lda_functional(rho - h, grid)[1]
+    assert np.allclose((Vp - Vm) / (2 * h), O.lda_v2rho2(rho), rtol=1e-6)
+
+
+def test_fft_conventions():
+    grid = O.Grid(np.eye(3) * 10.0, (10, 12, 15), full=False)
+    f = np.random.default_rng(0).random((10, 12, 15))
+    F = O.fft(f, grid)
+    assert abs(F[0, 0, 0] - f.sum() * grid.dV) < 1e-12
+    assert np.abs(O.ifft(F, grid) - f).max() < 1e-13
+
+
+def test_gaussian_transform_pair():
+    """test_field.py:254-292: analytic Gaussian pair on an odd grid."""
+    grid = O.Grid(np.eye(3) * 10.0, (51, 51, 51), full=True)
+    sigma = 0.5
+    ax = np.arange(51) / 51 * 10.0
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    c = ax[25]
+    g = (sigma * np.sqrt(2 * np.pi)) ** -3 * np.exp(-0.5 * ((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2) / sigma ** 2)
+    G = grid.g
+    ref = np.exp(-sigma ** 2 * grid.gg / 2) * np.exp(-1j * (G[0] * c + G[1] * c + G[2] * c))
+    assert np.isclose(g.sum() * grid.dV, 1.0)
+    assert np.allclose(O.fft(g, grid), ref, atol=1e-8)
+
+
+def test_lbfgs_reaches_tn_minimum():
+    a = 4.05 / O.BOHR
+    cell = np.eye(3) * a
+    pos = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]]) * a
+    ions = O.Ions(["Al"] * 4, pos, cell)
+    grid = O.Grid(cell, (16, 16, 16))
+    pp = O.LocalPseudo(grid, ions, {"Al": O.empty_core_table(3.0, 1.12)}, PME=True)
+    tot = O.TotalFunctional(grid, kedf="TFVW", kedf_options=dict(y=1 / 9), pseudo=pp)
+    rho0 = np.full((16, 16, 16), 12 / grid.volume)
+    E = {}
+    for m in ("TN", "LBFGS"):
+        opt = O.Optimization(lambda r, c, p: tot.compute(r, c, phi=p), grid, m, {"econv": 1e-11, "maxiter": 400})
+        opt.optimize_rho(rho0)
+        E[m] = opt.energy
+    assert abs(E["TN"] - E["LBFGS"]) < 1e-9
