@@ -21,6 +21,10 @@ NSUM = 9
 
 CONV_NONE, CONV_HARTREE, CONV_GG, CONV_WT = range(4)
 
+# ofdft_tn_direction outcomes: the `stat` strings of get_direction_TN
+TN_CONV, TN_WARN, TN_FAILED, TN_NOTREDUCE, TN_SMALL, TN_MAXFUN = range(6)
+TN_STATUS = ("CONV", "WARN", "FAILED", "WARN : Not reduce", "WARN : Change too small", "NOTCONV")
+
 
 class OfdftError(RuntimeError):
     """Error reported by the HIP library."""
@@ -59,6 +63,10 @@ _SIGS = {
     "ofdft_eval": (_i, [_vp, _vp, _vp, _dp]),
     "ofdft_eval_phi": (_i, [_vp, _vp, _vp, _d, _d, _vp, _vp, _dp]),
     "ofdft_hvp": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _dp]),
+    "ofdft_hvp_prepare": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _vp, _dp]),
+    "ofdft_hvp_fast": (_i, [_vp, _vp, _vp, _vp, _vp, _dp]),
+    "ofdft_tn_inner": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
+    "ofdft_tn_direction": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp]),
     "ofdft_fft_r2c": (_i, [_vp, _vp, _vp]),
     "ofdft_fft_c2r": (_i, [_vp, _vp, _vp]),
     "ofdft_convolve": (_i, [_vp, _i, _vp, _vp]),

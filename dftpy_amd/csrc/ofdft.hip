@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -102,8 +103,9 @@ FFTLaunch fft_launch(const Pass1D& pl, int ntr, int min_threads = 64) {
 // ---------------------------------------------------------------------------
 struct ofdft_plan_s {
   int n[3];
-  int nh;
-  long long N, Nh;
+  int nh;            // n2/2 + 1 logical half-spectrum length
+  int nhp;           // row stride of the internal spectrum (nh rounded up to even)
+  long long N, Nh, Nhp;
   int prec;
   int full;   // reference full=True semantics (Hermitian-symmetrised multipliers)
   size_t rs;  // bytes per real
@@ -118,6 +120,9 @@ struct ofdft_plan_s {
   long long npartials;
   double* red;      // device, kMaxSum doubles
   double* hred;     // pinned host mirror
+  void* tnst;       // device TnState of ofdft_tn_direction
+  void* htnst;      // pinned host mirror of it
+  const int* skip;  // while set, FFT and reduction kernels return at once when *skip != 0
   ofdft_functional fn;
   int have_fn, have_vloc, have_wtk;
   // ion state of the last ofdft_vloc_build (local PP forces reuse it)
@@ -134,8 +139,9 @@ struct ofdft_plan_s {
   int prof_on;
   std::vector<hipEvent_t> ev_pool;  // reused across profile windows
   size_t ev_used = 0;
-  struct ProfRec { int cls; long long bytes; int e0, e1; };
+  struct ProfRec { int cls; long long bytes; int e0, e1, tag; };
   std::vector<ProfRec> prof_pending;
+  int prof_tag = -1;  // CG iteration a launch belongs to (-1: none)
   double prof_ms[8];
   long long prof_count[8];
   double prof_bytes[8];
@@ -157,7 +163,7 @@ struct ProfScope {
   long long bytes;
   int e0 = -1;
   ProfScope(ofdft_plan_s* p, int c, long long b) : P(p), cls(c), bytes(b) {
-    if (!P->prof_on) return;
+    if (!(P->prof_on & (1 << c))) return;
     while (P->ev_pool.size() < P->ev_used + 2) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;
@@ -170,7 +176,7 @@ struct ProfScope {
   ~ProfScope() {
     if (e0 < 0) return;
     (void)hipEventRecord(P->ev_pool[e0 + 1], P->stream);
-    P->prof_pending.push_back({cls, bytes, e0, e0 + 1});
+    P->prof_pending.push_back({cls, bytes, e0, e0 + 1, P->prof_tag});
   }
 };
 
@@ -323,6 +329,7 @@ __device__ double lindhard_dev(double eta, double lbda, double mu) {
 
 struct GridGeom {
   int n0, n1, n2, nh;
+  int nhp;   // row stride of the internal spectrum
   int full;  // 1: frequencies of the full complex grid (grid.py:584-620, full=True)
   double bg[9];
   float bgf[9];
@@ -536,16 +543,80 @@ struct PostHvp {
   }
 };
 
+// Ap = W0 p + phi * (H[drho] + y/2 GG[ds]/s + 2 alpha rho^(beta-1) K[beta rho^(beta-1) drho]),
+// with W0 = V - mu + 2 rho a(rho) precomputed by k_hvp_prepare (a: the local
+// second derivative of TF, LDA, vW and WT); acc[0] = sum p*Ap.
+template <typename T>
+struct PostHvpFast {
+  static constexpr int NS = 1;
+  const T* __restrict__ phi;
+  const T* __restrict__ p;
+  const T* __restrict__ W0;
+  T* __restrict__ Ap;
+  FnParams fp;
+  __host__ __device__ int nio() const { return 4; }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
+    const T ph = phi[i], pp = p[i];
+    const T r = ph * ph;
+    T dv = T(0);
+    if (fp.fH >= 0) dv += pick3(c, fp.fH);
+    if (fp.fV >= 0 || fp.fW >= 0) {
+      const T s = dev_sqrt(r);
+      if (fp.fV >= 0 && r > T(0)) dv += T(fp.y) * T(0.5) * pick3(c, fp.fV) / s;
+      if (fp.fW >= 0 && r >= T(kZero)) {
+        const T rb1 = fp.beta56 ? dev_cbrt(r) / s : dev_pow(r, T(fp.beta) - T(1));
+        dv += T(2 * fp.alpha) * rb1 * pick3(c, fp.fW);
+      }
+    }
+    const T ap = W0[i] * pp + ph * dv;
+    Ap[i] = ap;
+    acc[0] += (double)(pp * ap);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+// Residual (V - mu) phi (optimization.py:323, :469) and the HVP diagonal
+// W0 = V - mu + 2 rho a(rho) at phi; partial sums of res^2.
+template <typename T>
+__global__ void k_hvp_prepare(const T* __restrict__ phi, const T* __restrict__ V, const T* __restrict__ aux, T mu,
+                              T* __restrict__ res, T* __restrict__ W0, long long N, FnParams fp,
+                              double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const T ph = phi[i];
+    const T r = ph * ph;
+    const T vm = V[i] - mu;
+    const T rr = vm * ph;
+    res[i] = rr;
+    acc[0] += (double)rr * (double)rr;
+    T a = T(0);
+    if (r > T(0)) {
+      const T cb = dev_cbrt(r);
+      if (fp.tf) a += T(fp.x) * T(3.1902600002091020177) / cb;  // (2/3)(5/3) C_TF rho^(-1/3)
+      if (fp.lda) a += lda_v2(r, cb);
+      if (fp.fV >= 0) a -= T(fp.y) * T(0.5) * aux[i] / r;   // -V_vW / (2 rho)
+      if (fp.fW >= 0 && r >= T(kZero)) {
+        const T s = dev_sqrt(r);
+        const T rb1 = fp.beta56 ? cb / s : dev_pow(r, T(fp.beta) - T(1));
+        a += T(2 * fp.alpha * (fp.beta - 1.0)) * rb1 / r * aux[N + i];
+      }
+    }
+    W0[i] = vm + T(2) * r * a;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
 // Axis-2 real-to-complex: block handles PB row pairs x F fields.  Rows r and
 // r+1 of field f share one complex sequence z = a + i b; after the FFT,
 // A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i).
 template <typename T, class FFT, class Pre>
 __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int F,
-                                                      const cplx<T>* __restrict__ tw) {
+                                                      long long nrows, long long Nh, int nhp, int F,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPairsC;
@@ -574,8 +645,8 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
     const cplx<T> zk = lds[tr * ld + k];
     const cplx<T> zm = lds[tr * ld + (k == 0 ? 0 : n2 - k)];
-    if (r0 < nrows) spec[f * Nh + r0 * nh + k] = cplx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-    if (r1 < nrows) spec[f * Nh + r1 * nh + k] = cplx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+    if (r0 < nrows) spec[f * Nh + r0 * nhp + k] = cplx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+    if (r1 < nrows) spec[f * Nh + r1 * nhp + k] = cplx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
   }
 }
 
@@ -585,9 +656,10 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
 // real part is row r, the imaginary part row r+1.
 template <typename T, class FFT, class Post>
 __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int F,
+                                                      long long nrows, long long Nh, int nhp, int F,
                                                       const cplx<T>* __restrict__ tw,
-                                                      double* __restrict__ partials) {
+                                                      double* __restrict__ partials, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPairsC;
@@ -601,8 +673,8 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const 
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
     const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
-    cplx<T> A = r0 < nrows ? spec[f * Nh + r0 * nh + k] : cplx<T>{T(0), T(0)};
-    cplx<T> B = r1 < nrows ? spec[f * Nh + r1 * nh + k] : cplx<T>{T(0), T(0)};
+    cplx<T> A = r0 < nrows ? spec[f * Nh + r0 * nhp + k] : cplx<T>{T(0), T(0)};
+    cplx<T> B = r1 < nrows ? spec[f * Nh + r1 * nhp + k] : cplx<T>{T(0), T(0)};
     if (k == 0 || (even && k == n2 / 2)) {
       A.y = T(0);
       B.y = T(0);
@@ -640,7 +712,9 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const 
 template <typename T, class FFT>
 __global__ __launch_bounds__(FFT::TH) void k_axis_c2c(FFT fft, cplx<T>* __restrict__ spec, long long Nh,
                                                       long long stride, long long ostride, int ncol,
-                                                      const cplx<T>* __restrict__ tw, int inv) {
+                                                      const cplx<T>* __restrict__ tw, int inv,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int W = kColTileC;
@@ -718,7 +792,9 @@ __device__ __forceinline__ T conv_mult(const GridGeom& geo, int kind, int k0, in
 template <typename T, class FFT>
 __global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __restrict__ spec, long long Nh,
                                                          GridGeom geo, const cplx<T>* __restrict__ tw,
-                                                         ConvKinds ck, const T* __restrict__ wtk) {
+                                                         ConvKinds ck, const T* __restrict__ wtk,
+                                                         const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int W = kColTileC;
@@ -726,8 +802,8 @@ __global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __res
   const int th = fft.th();
   const int i1 = blockIdx.y;
   const int f = blockIdx.z;
-  const long long stride = (long long)geo.n1 * geo.nh;
-  cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nh;
+  const long long stride = (long long)geo.n1 * geo.nhp;
+  cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nhp;
   const int c0 = blockIdx.x * W;
   const int ld = n + 1;
   for (int t = threadIdx.x; t < n * W; t += th) {
@@ -763,7 +839,8 @@ __global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __res
 template <typename T, int N1, int N2, int W>
 __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
                                                      long long ostride, int ncol, const cplx<T>* __restrict__ tw,
-                                                     int inv) {
+                                                     int inv, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
   const int t = threadIdx.x, w = t % W, q = t / W;
   cplx<T>* base = spec + (long long)blockIdx.z * Nh + (long long)blockIdx.y * ostride;
@@ -800,13 +877,14 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec,
 template <typename T, int N1, int N2, int W>
 __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
                                                        const cplx<T>* __restrict__ tw, ConvKinds ck,
-                                                       const T* __restrict__ wtk) {
+                                                       const T* __restrict__ wtk, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
   const int t = threadIdx.x, w = t % W, q = t / W;
   const int i1 = blockIdx.y;
   const int f = blockIdx.z;
-  const long long stride = (long long)geo.n1 * geo.nh;
-  cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nh;
+  const long long stride = (long long)geo.n1 * geo.nhp;
+  cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nhp;
   const int c = blockIdx.x * W + w;
   const bool ok = c < geo.nh;
   cplx<T> v[N2];
@@ -851,6 +929,7 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
   }
 }
 
+
 // WT kernel table K(|G|) (WTKernel kernel.py:325-334), evaluated in FP64.
 template <typename T>
 __global__ void k_wt_table(T* __restrict__ out, GridGeom geo, double tkf, double factor, double lbda) {
@@ -868,7 +947,9 @@ __global__ void k_wt_table(T* __restrict__ out, GridGeom geo, double tkf, double
 }
 
 template <int NS>
-__global__ void k_reduce_partials(const double* __restrict__ partials, long long nblk, double* __restrict__ out) {
+__global__ void k_reduce_partials(const double* __restrict__ partials, long long nblk, double* __restrict__ out,
+                                  const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   const int s = blockIdx.x;
   double acc = 0.0;
   for (long long b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[b * NS + s];
@@ -918,6 +999,179 @@ __global__ void k_cg_update(T a, const T* __restrict__ p, const T* __restrict__ 
     acc[0] += (double)rr * (double)rr;
   }
   block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// TN inner update with the step length computed on the device from the
+// pAp just reduced into red[0]: if pAp > 0, d += a p and r -= a Ap with
+// a = r0 / pAp; otherwise d and r are left untouched (the reference leaves
+// the loop before updating).  Partial sums of r^2 either way.
+template <typename T>
+__global__ void k_cg_update_dev(const double* __restrict__ red, double r0, const T* __restrict__ p,
+                                const T* __restrict__ Ap, T* __restrict__ d, T* __restrict__ r, long long N,
+                                double* __restrict__ partials) {
+  const double pAp = red[0];
+  const bool upd = pAp > 0.0;
+  const T a = upd ? T(r0 / pAp) : T(0);
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    T rr = r[i];
+    if (upd) {
+      d[i] += a * p[i];
+      rr -= a * Ap[i];
+      r[i] = rr;
+    }
+    acc[0] += (double)rr * (double)rr;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Device-resident TN direction loop: get_direction_TN (optimization.py:92-148)
+// with the exact HVP.  The host queues CG iterations without waiting; a
+// one-thread control kernel applies the reference's tests to the reductions
+// and sets `done`, after which every queued kernel returns at once.
+// ---------------------------------------------------------------------------
+struct TnState {
+  double r0, rconv, rmin, r1, pAp, beta, scale, pad;
+  int done;    // loop finished
+  int it;      // index of the CG iteration being run
+  int niter;   // iterations taken (the reference's `number`)
+  int status;  // OFDFT_TN_*
+  int act;     // kAct* bits for k_tn_apply of this iteration
+  int maxit;
+  int alias;   // `Best` still aliases `direction` (no copy taken yet)
+  int pad2;
+};
+constexpr int kActKeep = 1;     // Best = direction.copy()
+constexpr int kActRestore = 2;  // direction = Best
+constexpr int kActScale = 4;    // direction = r0 / pAp * p
+constexpr int kActNext = 8;     // p = res + beta * p
+
+// d = 0, r = -res, p = r; partial sums of r^2
+template <typename T>
+__global__ void k_tn_init(const T* __restrict__ res, T* __restrict__ d, T* __restrict__ r, T* __restrict__ p,
+                          long long N, double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const T v = -res[i];
+    d[i] = T(0);
+    r[i] = v;
+    p[i] = v;
+    acc[0] += (double)v * (double)v;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
+__global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double r0 = rr[0];
+  st->r0 = r0;
+  st->rconv = 0.1 * r0;
+  st->rmin = 1e10;  // rLists = [1e10]
+  st->r1 = r0;
+  st->pAp = 0.0;
+  st->beta = 0.0;
+  st->scale = 0.0;
+  st->it = 0;
+  st->niter = 0;
+  st->status = OFDFT_TN_MAXFUN;
+  st->act = 0;
+  st->maxit = maxit;
+  st->alias = 1;
+  st->done = maxit > 0 ? 0 : 1;
+}
+
+// alpha = r0 / pAp; d += alpha p; r -= alpha Ap; partial sums of r^2.
+// Nothing happens when the loop is done or pAp < 0 (the reference leaves
+// the loop before the update).
+template <typename T>
+__global__ void k_tn_step(const TnState* __restrict__ st, const double* __restrict__ red, const T* __restrict__ p,
+                          const T* __restrict__ Ap, T* __restrict__ d, T* __restrict__ r, long long N,
+                          double* __restrict__ partials) {
+  if (st->done) return;
+  const double pAp = red[0];
+  if (pAp < 0.0) return;
+  const T a = T(st->r0 / pAp);
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    d[i] += a * p[i];
+    const T rr = r[i] - a * Ap[i];
+    r[i] = rr;
+    acc[0] += (double)rr * (double)rr;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// The branch tests of optimization.py:116-145 on pAp = red[0], r1 = rr[0].
+__global__ void k_tn_ctrl(TnState* st, const double* __restrict__ red, const double* __restrict__ rr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (st->done) {
+    st->act = 0;
+    return;
+  }
+  const double pAp = red[0];
+  st->pAp = pAp;
+  int act = 0;
+  if (pAp < 0.0) {
+    if (st->it == 0) {
+      st->scale = st->r0 / pAp;
+      act = kActScale;
+      st->status = OFDFT_TN_WARN;
+    } else {
+      st->status = OFDFT_TN_FAILED;
+    }
+    st->done = 1;
+  } else {
+    const double r1 = rr[0], r0 = st->r0, rmin = st->rmin;  // rmin = min(rLists[:-1])
+    st->r1 = r1;
+    if (r1 < rmin) {
+      act |= kActKeep;
+      st->rmin = r1;
+    }
+    if (r1 < st->rconv) {
+      st->status = OFDFT_TN_CONV;
+      st->done = 1;
+    } else if (r1 > 1000.0 * rmin) {
+      act |= kActRestore;
+      st->status = OFDFT_TN_NOTREDUCE;
+      st->done = 1;
+    } else if (st->it > 10 && fabs(r0 - r1) < 0.1 * r0) {
+      act |= kActRestore;
+      st->status = OFDFT_TN_SMALL;
+      st->done = 1;
+    } else {
+      st->beta = r1 / r0;
+      st->r0 = r1;
+      act |= kActNext;
+    }
+    // restoring Best while it still aliases direction leaves direction as is;
+    // a fresh copy followed by a restore is a no-op too
+    if ((act & kActRestore) && (st->alias || (act & kActKeep))) act &= ~kActRestore;
+    if (act & kActKeep) st->alias = 0;
+  }
+  if (st->done) {
+    st->niter = st->it + 1;
+  } else if (st->it + 1 >= st->maxit) {
+    st->done = 1;
+    st->niter = st->maxit;
+    st->status = OFDFT_TN_MAXFUN;
+  }
+  st->it += 1;
+  st->act = act;
+}
+
+template <typename T>
+__global__ void k_tn_apply(const TnState* __restrict__ st, T* __restrict__ d, const T* __restrict__ r,
+                           T* __restrict__ p, T* __restrict__ best, long long N) {
+  const int act = st->act;
+  if (!act) return;
+  const T beta = T(st->beta), scale = T(st->scale);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    if (act & kActKeep) best[i] = d[i];
+    if (act & kActRestore) d[i] = best[i];
+    if (act & kActScale) d[i] = scale * p[i];
+    if (act & kActNext) p[i] = r[i] + beta * p[i];
+  }
 }
 
 template <typename T>
@@ -975,6 +1229,7 @@ GridGeom geom(const ofdft_plan_s* P) {
   g.n1 = P->n[1];
   g.n2 = P->n[2];
   g.nh = P->nh;
+  g.nhp = P->nhp;
   g.full = P->full;
   for (int i = 0; i < 9; ++i) {
     g.bg[i] = P->bg[i];
@@ -990,7 +1245,7 @@ constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // columns per workgroup of the register column kernels
 template <typename T>
 struct RegTile {
-  static constexpr int W = sizeof(T) == 4 ? 32 : 16;
+  static constexpr int W = 16;
 };
 constexpr int kRowsTH = 256;
 constexpr int kColsTH = 256;
@@ -1002,8 +1257,8 @@ using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 template <typename T, class Pre, class FFT>
 void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
   hipLaunchKernelGGL((k_rows_fwd<T, FFT, Pre>), grid, dim3(threads), lds, P->stream, pre, fft,
-                     reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nh, F,
-                     reinterpret_cast<const cplx<T>*>(P->tw[2]));
+                     reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
+                     reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
 }
 
 template <typename T, class Pre>
@@ -1034,7 +1289,7 @@ template <typename T, class Post, class FFT>
 void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const cplx<T>* src, dim3 grid,
                  int threads, size_t lds, double* part) {
   hipLaunchKernelGGL((k_rows_inv<T, FFT, Post>), grid, dim3(threads), lds, P->stream, post, fft, src,
-                     (long long)P->n[0] * P->n[1], P->Nh, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part);
+                     (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
 }
 
 template <typename T, class Post>
@@ -1064,7 +1319,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   }
   LAUNCHCHK();
   if (reduce) {
-    hipLaunchKernelGGL((k_reduce_partials<Post::NS>), dim3(Post::NS), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red);
+    hipLaunchKernelGGL((k_reduce_partials<Post::NS>), dim3(Post::NS), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
     LAUNCHCHK();
   }
   return 0;
@@ -1073,8 +1328,8 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
 template <typename T, class FFT>
 void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid, int threads, size_t lds,
              long long stride, long long ostride, bool inv) {
-  hipLaunchKernelGGL((k_axis_c2c<T, FFT>), grid, dim3(threads), lds, P->stream, fft, spec, P->Nh, stride, ostride,
-                     P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0);
+  hipLaunchKernelGGL((k_axis_c2c<T, FFT>), grid, dim3(threads), lds, P->stream, fft, spec, P->Nhp, stride, ostride,
+                     P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
 template <typename T>
@@ -1085,12 +1340,12 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   long long stride, ostride;
   unsigned outer;
   if (axis == 1) {
-    stride = P->nh;
-    ostride = (long long)P->n[1] * P->nh;
+    stride = P->nhp;
+    ostride = (long long)P->n[1] * P->nhp;
     outer = P->n[0];
   } else {
-    stride = (long long)P->n[1] * P->nh;
-    ostride = P->nh;
+    stride = (long long)P->n[1] * P->nhp;
+    ostride = P->nhp;
     outer = P->n[1];
   }
   dim3 grid((unsigned)((P->nh + W - 1) / W), outer, (unsigned)F);
@@ -1100,8 +1355,8 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
     case NN:                                                                                                 \
       hipLaunchKernelGGL((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
                          dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),     \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nh, stride, ostride, P->nh, \
-                         reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0);                        \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nhp, stride, ostride, P->nh, \
+                         reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);               \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -1121,8 +1376,8 @@ template <typename T, class FFT>
 void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t lds, const GridGeom& g,
                const ConvKinds& ck) {
   hipLaunchKernelGGL((k_pencil_conv<T, FFT>), grid, dim3(threads), lds, P->stream, fft,
-                     reinterpret_cast<cplx<T>*>(P->spec), P->Nh, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,
-                     reinterpret_cast<const T*>(P->wtk));
+                     reinterpret_cast<cplx<T>*>(P->spec), P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,
+                     reinterpret_cast<const T*>(P->wtk), P->skip);
 }
 
 template <typename T>
@@ -1146,8 +1401,8 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
                          dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1],  \
                               (unsigned)F),                                                              \
                          dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, reinterpret_cast<cplx<T>*>(P->spec), \
-                         P->Nh, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                       \
-                         reinterpret_cast<const T*>(P->wtk));                                            \
+                         P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                      \
+                         reinterpret_cast<const T*>(P->wtk), P->skip);                                   \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -1244,6 +1499,34 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
 }
 
 template <typename T>
+int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap, double* pAp) {
+  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
+  FnParams fp = make_fn(P->fn);
+  const int F = fp.F;
+  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
+  int rc;
+  if (F > 0) {
+    PreHvp<T> pre{phi, p, fp};
+    if ((rc = launch_rows_fwd<T>(P, pre, F))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+    int kinds[3];
+    if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
+    if (fp.fV >= 0) kinds[fp.fV] = OFDFT_CONV_GG;
+    if (fp.fW >= 0) kinds[fp.fW] = OFDFT_CONV_WT;
+    if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+  }
+  PostHvpFast<T> post{phi, p, W0, Ap, fp};
+  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true))) return rc;
+  if (pAp) {
+    double t;
+    if ((rc = fetch_sums(P, 1, &t))) return rc;
+    *pAp = t;
+  }
+  return 0;
+}
+
+template <typename T>
 int r2c_impl(ofdft_plan_s* P, const T* in, cplx<T>* out) {
   // rows -> axis1 -> axis0 in the workspace, then copy the spectrum out.
   PreCopy<T> pre{in};
@@ -1252,7 +1535,8 @@ int r2c_impl(ofdft_plan_s* P, const T* in, cplx<T>* out) {
   if ((rc = launch_rows_fwd<T>(P, pre, 1))) return rc;
   if ((rc = launch_axis<T>(P, spec, 1, 1, false))) return rc;
   if ((rc = launch_axis<T>(P, spec, 0, 1, false))) return rc;
-  HIPCHK(hipMemcpyAsync(out, spec, sizeof(cplx<T>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
+  HIPCHK(hipMemcpy2DAsync(out, sizeof(cplx<T>) * P->nh, spec, sizeof(cplx<T>) * P->nhp, sizeof(cplx<T>) * P->nh,
+                          (size_t)P->n[0] * P->n[1], hipMemcpyDeviceToDevice, P->stream));
   return 0;
 }
 
@@ -1268,8 +1552,9 @@ template <typename T>
 int c2r_impl(ofdft_plan_s* P, const cplx<T>* in, T* out) {
   auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
   int rc;
-  HIPCHK(hipMemcpyAsync(spec, in, sizeof(cplx<T>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
-  hipLaunchKernelGGL((k_scale_cplx<T>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nh,
+  HIPCHK(hipMemcpy2DAsync(spec, sizeof(cplx<T>) * P->nhp, in, sizeof(cplx<T>) * P->nh, sizeof(cplx<T>) * P->nh,
+                          (size_t)P->n[0] * P->n[1], hipMemcpyDeviceToDevice, P->stream));
+  hipLaunchKernelGGL((k_scale_cplx<T>), dim3(vec_blocks(P->Nhp)), dim3(256), 0, P->stream, spec, P->Nhp,
                      T(1.0 / (double)P->N));
   LAUNCHCHK();
   if ((rc = launch_axis<T>(P, spec, 0, 1, true))) return rc;
@@ -1294,8 +1579,8 @@ int conv_impl(ofdft_plan_s* P, int kind, const T* in, T* out) {
 int reduce_blocks(ofdft_plan_s* P, int nblk, int ns, double* out) {
   int rc;
   switch (ns) {
-    case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red); break;
-    default: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red); break;
+    case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
+    default: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
   }
   LAUNCHCHK();
   if ((rc = fetch_sums(P, ns, out))) return rc;
@@ -1468,6 +1753,7 @@ __global__ void k_vloc_accum_pme(cplx<double>* __restrict__ acc, const cplx<doub
     const int c = (int)(idx % geo.nh);
     const long long r = idx / geo.nh;
     const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const long long ph = r * geo.nhp + c;
     cplx<double> m = cscale(bmod(bm, geo.n0, geo.n1, k0, k1, c), vline_eval(sp, sqrt(gsq(geo, k0, k1, c))));
     if (geo.full) {
       const int p0 = (geo.n0 - k0) % geo.n0, p1 = (geo.n1 - k1) % geo.n1, p2 = (geo.n2 - c) % geo.n2;
@@ -1476,9 +1762,9 @@ __global__ void k_vloc_accum_pme(cplx<double>* __restrict__ acc, const cplx<doub
       const cplx<double> mp = cscale(bp, vline_eval(sp, sqrt(gsq(geo, k0, k1, c, 1))));
       m = cscale(cadd(m, mp), 0.5);
     }
-    const cplx<double> z = cmul(qhat[idx], m);
-    acc[idx].x += z.x * scale;
-    acc[idx].y += z.y * scale;
+    const cplx<double> z = cmul(qhat[ph], m);
+    acc[ph].x += z.x * scale;
+    acc[ph].y += z.y * scale;
   }
 }
 
@@ -1516,7 +1802,7 @@ __global__ void k_vloc_exact(cplx<double>* __restrict__ acc, GridGeom geo, const
       tot = cadd(tot, v);
     }
     const double f = scale / npart;
-    acc[idx] = cplx<double>{tot.x * f, tot.y * f};
+    acc[r * geo.nhp + c] = cplx<double>{tot.x * f, tot.y * f};
   }
 }
 
@@ -1553,7 +1839,7 @@ __global__ void k_force_exact(const cplx<double>* __restrict__ rhog, GridGeom ge
     const double v = vline_eval(sp, sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]));
     double sn, cs;
     sincos(G[0] * R0 + G[1] * R1 + G[2] * R2, &sn, &cs);
-    const cplx<double> z = rhog[idx];
+    const cplx<double> z = rhog[r * geo.nhp + c];
     const double im = (z.y * cs + z.x * sn) * dV * v;
     acc[0] += G[0] * im;
     acc[1] += G[1] * im;
@@ -1581,7 +1867,8 @@ __global__ void k_force_pme_mult(cplx<double>* __restrict__ out, const cplx<doub
       const cplx<double> bp = bmod(bm, geo.n0, geo.n1, p0, p1, p2);
       m = cscale(cadd(m, cscale(bp, vline_eval(sp, sqrt(gsq(geo, k0, k1, c, 1))))), 0.5);
     }
-    out[idx] = idx == 0 ? cplx<double>{0.0, 0.0} : cmul(rhog[idx], m);
+    const long long ph = r * geo.nhp + c;
+    out[ph] = idx == 0 ? cplx<double>{0.0, 0.0} : cmul(rhog[ph], m);
   }
 }
 
@@ -1663,7 +1950,7 @@ __global__ void k_ewald_rec(const cplx<double>* __restrict__ qhat, const double*
     if (!(g2 > 0.0)) continue;
     cplx<double> S;
     if (qhat) {
-      S = cmul(qhat[idx], bmod(bm, geo.n0, geo.n1, k0, k1, c));
+      S = cmul(qhat[r * geo.nhp + c], bmod(bm, geo.n0, geo.n1, k0, k1, c));
     } else {
       S = {0.0, 0.0};
       for (int i = 0; i < nat; ++i) {
@@ -1719,13 +2006,13 @@ std::vector<double> make_bm(const ofdft_plan_s* P, int order) {
 int r2c_slot(ofdft_plan_s* P, const double* in, int slot) {
   PreCopy<double> pre{in};
   int rc;
-  auto* spec = reinterpret_cast<cplx<double>*>(P->spec) + (long long)slot * P->Nh;
+  auto* spec = reinterpret_cast<cplx<double>*>(P->spec) + (long long)slot * P->Nhp;
   // rows kernel writes slot 0; move it if another slot is requested
   if ((rc = launch_rows_fwd<double>(P, pre, 1))) return rc;
   auto* s0 = reinterpret_cast<cplx<double>*>(P->spec);
   if ((rc = launch_axis<double>(P, s0, 1, 1, false))) return rc;
   if ((rc = launch_axis<double>(P, s0, 0, 1, false))) return rc;
-  if (slot != 0) HIPCHK(hipMemcpyAsync(spec, s0, sizeof(cplx<double>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
+  if (slot != 0) HIPCHK(hipMemcpyAsync(spec, s0, sizeof(cplx<double>) * P->Nhp, hipMemcpyDeviceToDevice, P->stream));
   return 0;
 }
 
@@ -1733,7 +2020,7 @@ int r2c_slot(ofdft_plan_s* P, const double* in, int slot) {
 int c2r_slot0(ofdft_plan_s* P, double* out) {
   auto* s0 = reinterpret_cast<cplx<double>*>(P->spec);
   int rc;
-  hipLaunchKernelGGL((k_scale_cplx<double>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, s0, P->Nh,
+  hipLaunchKernelGGL((k_scale_cplx<double>), dim3(vec_blocks(P->Nhp)), dim3(256), 0, P->stream, s0, P->Nhp,
                      1.0 / (double)P->N);
   LAUNCHCHK();
   if ((rc = launch_axis<double>(P, s0, 0, 1, true))) return rc;
@@ -1784,9 +2071,9 @@ int ensure_bm(ofdft_plan_s* P, int order) {
 int reduce_to_host(ofdft_plan_s* P, long long nblk, int ns, double* out) {
   // partial layout [blk][ns]; reuse k_reduce_partials
   if (ns == 1)
-    hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red);
+    hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
   else
-    hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red);
+    hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
   LAUNCHCHK();
   return fetch_sums(P, ns, out);
 }
@@ -1798,6 +2085,83 @@ bool valid(ofdft_plan P) { return P != nullptr; }
 // ===========================================================================
 // C ABI
 // ===========================================================================
+namespace {
+
+template <typename T>
+int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, T* d, T* work, int maxfun, int chunk,
+                      int* niter, int* status, double* pAp_last) {
+  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
+  const long long N = P->N;
+  T* r = work;
+  T* p = work + N;
+  T* Ap = work + 2 * N;
+  T* best = work + 3 * N;
+  auto* st = reinterpret_cast<TnState*>(P->tnst);
+  const int nb = vec_blocks(N);
+  double* rr = P->red + 8;  // r^2 lands in red[8]; pAp in red[0]
+  P->skip = nullptr;
+  {
+    ProfScope ps(P, PC_VEC, (long long)P->rs * 4 * N);
+    hipLaunchKernelGGL((k_tn_init<T>), dim3(nb), dim3(256), 0, P->stream, res, d, r, p, N, P->partials);
+  }
+  LAUNCHCHK();
+  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr,
+                     (const int*)nullptr);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_tn_start, dim3(1), dim3(1), 0, P->stream, st, (const double*)rr, maxfun);
+  LAUNCHCHK();
+  auto* h = reinterpret_cast<TnState*>(P->htnst);
+  const size_t prof_mark = P->prof_pending.size();
+  P->skip = &st->done;
+  int rc = 0, queued = 0, k = std::max(1, chunk);
+  while (queued < maxfun) {
+    for (int j = 0; j < k && queued < maxfun; ++j, ++queued) {
+      P->prof_tag = queued;
+      if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr))) break;
+      {
+        ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * N);
+        hipLaunchKernelGGL((k_tn_step<T>), dim3(nb), dim3(256), 0, P->stream, st, (const double*)P->red,
+                           (const T*)p, (const T*)Ap, d, r, N, P->partials);
+      }
+      hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials,
+                         (long long)nb, rr, P->skip);
+      hipLaunchKernelGGL(k_tn_ctrl, dim3(1), dim3(1), 0, P->stream, st, (const double*)P->red, (const double*)rr);
+      {
+        ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
+        hipLaunchKernelGGL((k_tn_apply<T>), dim3(nb), dim3(256), 0, P->stream, (const TnState*)st, d, (const T*)r,
+                           p, best, N);
+      }
+      if (hipGetLastError() != hipSuccess) {
+        rc = fail(OFDFT_EHIP, "kernel launch failed in the TN loop");
+        break;
+      }
+    }
+    if (rc) break;
+    P->prof_tag = -1;
+    if (hipMemcpyAsync(h, st, sizeof(TnState), hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
+        hipStreamSynchronize(P->stream) != hipSuccess) {
+      rc = fail(OFDFT_EHIP, "TN state read-back");
+      break;
+    }
+    if (h->done) break;
+    k = std::max(2, k / 2);
+  }
+  P->prof_tag = -1;
+  P->skip = nullptr;
+  if (rc) return rc;
+  if (!h->done) return fail(OFDFT_ESTATE, "TN loop did not finish");
+  // launches queued after the loop finished did no work: drop their timings
+  P->prof_pending.erase(std::remove_if(P->prof_pending.begin() + prof_mark, P->prof_pending.end(),
+                                       [&](const ofdft_plan_s::ProfRec& x) { return x.tag >= h->niter; }),
+                        P->prof_pending.end());
+  *niter = h->niter;
+  *status = h->status;
+  *pAp_last = h->pAp;
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ofdft_last_error(void) { return g_err.c_str(); }
@@ -1816,8 +2180,10 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   P->n[1] = n1;
   P->n[2] = n2;
   P->nh = n2 / 2 + 1;
+  P->nhp = P->nh + (P->nh & 1);
   P->N = (long long)n0 * n1 * n2;
   P->Nh = (long long)n0 * n1 * P->nh;
+  P->Nhp = (long long)n0 * n1 * P->nhp;
   P->prec = precision;
   P->full = full ? 1 : 0;
   P->rs = precision == OFDFT_F64 ? sizeof(double) : sizeof(float);
@@ -1881,7 +2247,8 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
         return cleanup(OFDFT_EHIP, "twiddle upload");
     }
   }
-  if (hipMalloc(&P->spec, cs * 3 * P->Nh) != hipSuccess) return cleanup(OFDFT_ENOMEM, "spectrum workspace");
+  if (hipMalloc(&P->spec, cs * 3 * P->Nhp) != hipSuccess) return cleanup(OFDFT_ENOMEM, "spectrum workspace");
+  if (hipMemset(P->spec, 0, cs * 3 * P->Nhp) != hipSuccess) return cleanup(OFDFT_EHIP, "spectrum workspace clear");
   const long long npairs = ((long long)n0 * n1 + 1) / 2;
   const long long nblk = (npairs + kRowPairs - 1) / kRowPairs;
   P->npartials = std::max<long long>(nblk * OFDFT_NSUM, 2048LL * 3);
@@ -1890,6 +2257,10 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   if (hipMalloc((void**)&P->red, sizeof(double) * kMaxSum) != hipSuccess) return cleanup(OFDFT_ENOMEM, "red");
   if (hipHostMalloc((void**)&P->hred, sizeof(double) * kMaxSum, hipHostMallocDefault) != hipSuccess)
     return cleanup(OFDFT_ENOMEM, "pinned host buffer");
+  if (hipMalloc(&P->tnst, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_ENOMEM, "TN state");
+  if (hipMemset(P->tnst, 0, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_EHIP, "TN state clear");
+  if (hipHostMalloc(&P->htnst, sizeof(TnState), hipHostMallocDefault) != hipSuccess)
+    return cleanup(OFDFT_ENOMEM, "pinned TN state");
   *out = P;
   return 0;
 }
@@ -1912,6 +2283,8 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->d_work) (void)hipFree(P->d_work);
   if (P->red) (void)hipFree(P->red);
   if (P->hred) (void)hipHostFree(P->hred);
+  if (P->tnst) (void)hipFree(P->tnst);
+  if (P->htnst) (void)hipHostFree(P->htnst);
   delete P;
   return 0;
 }
@@ -1924,7 +2297,7 @@ int ofdft_plan_set_stream(ofdft_plan P, void* s) {
 
 long long ofdft_plan_workspace_bytes(ofdft_plan P) {
   if (!valid(P)) return -1;
-  long long b = 2 * (long long)P->rs * (3 * P->Nh + P->n[0] + P->n[1] + P->n[2]);
+  long long b = 2 * (long long)P->rs * (3 * P->Nhp + P->n[0] + P->n[1] + P->n[2]);
   if (P->wtk) b += (long long)P->rs * P->Nh;
   if (P->vloc) b += (long long)P->rs * P->N;
   b += sizeof(double) * (P->npartials + kMaxSum);
@@ -1933,7 +2306,7 @@ long long ofdft_plan_workspace_bytes(ofdft_plan P) {
 
 int ofdft_plan_set_profiling(ofdft_plan P, int on) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
-  P->prof_on = on ? 1 : 0;
+  P->prof_on = on & ((1 << PC_NCLS) - 1);
   for (int i = 0; i < 8; ++i) {
     P->prof_ms[i] = 0.0;
     P->prof_count[i] = 0;
@@ -2025,6 +2398,76 @@ int ofdft_hvp(ofdft_plan P, const void* phi, const void* p, const void* V, const
                          pAp);
 }
 
+int ofdft_hvp_prepare(ofdft_plan P, const void* phi, const void* V, const void* aux, double mu, void* res,
+                      void* W0, double* res2) {
+  if (!valid(P) || !phi || !V || !res || !W0) return fail(OFDFT_EINVAL, "null argument");
+  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
+  if ((P->fn.vw || P->fn.wt) && !aux) return fail(OFDFT_EINVAL, "aux fields required for vW/WT");
+  FnParams fp = make_fn(P->fn);
+  const int nb = vec_blocks(P->N);
+  {
+    ProfScope ps(P, PC_VEC, (long long)P->rs * 5 * P->N);
+    if (P->prec == OFDFT_F64)
+      hipLaunchKernelGGL((k_hvp_prepare<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)phi,
+                         (const double*)V, (const double*)aux, mu, (double*)res, (double*)W0, P->N, fp, P->partials);
+    else
+      hipLaunchKernelGGL((k_hvp_prepare<float>), dim3(nb), dim3(256), 0, P->stream, (const float*)phi,
+                         (const float*)V, (const float*)aux, (float)mu, (float*)res, (float*)W0, P->N, fp,
+                         P->partials);
+  }
+  LAUNCHCHK();
+  if (!res2) return 0;
+  return reduce_blocks(P, nb, 1, res2);
+}
+
+int ofdft_hvp_fast(ofdft_plan P, const void* phi, const void* p, const void* W0, void* Ap, double* pAp) {
+  if (!valid(P) || !phi || !p || !W0 || !Ap) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec == OFDFT_F64)
+    return hvp_fast_impl<double>(P, (const double*)phi, (const double*)p, (const double*)W0, (double*)Ap, pAp);
+  return hvp_fast_impl<float>(P, (const float*)phi, (const float*)p, (const float*)W0, (float*)Ap, pAp);
+}
+
+int ofdft_tn_inner(ofdft_plan P, const void* phi, const void* p, const void* W0, void* Ap, void* d, void* r,
+                   double r0, double* out2) {
+  if (!valid(P) || !phi || !p || !W0 || !Ap || !d || !r || !out2) return fail(OFDFT_EINVAL, "null argument");
+  int rc;
+  const int nb = vec_blocks(P->N);
+  double* red2 = P->red + 8;  // red[8]: r1 (red[0] holds pAp)
+  if (P->prec == OFDFT_F64) {
+    if ((rc = hvp_fast_impl<double>(P, (const double*)phi, (const double*)p, (const double*)W0, (double*)Ap, nullptr)))
+      return rc;
+    ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * P->N);
+    hipLaunchKernelGGL((k_cg_update_dev<double>), dim3(nb), dim3(256), 0, P->stream, P->red, r0, (const double*)p,
+                       (const double*)Ap, (double*)d, (double*)r, P->N, P->partials);
+  } else {
+    if ((rc = hvp_fast_impl<float>(P, (const float*)phi, (const float*)p, (const float*)W0, (float*)Ap, nullptr)))
+      return rc;
+    ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * P->N);
+    hipLaunchKernelGGL((k_cg_update_dev<float>), dim3(nb), dim3(256), 0, P->stream, P->red, r0, (const float*)p,
+                       (const float*)Ap, (float*)d, (float*)r, P->N, P->partials);
+  }
+  LAUNCHCHK();
+  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, red2, P->skip);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(P->hred, P->red, sizeof(double) * 9, hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(hipStreamSynchronize(P->stream));
+  out2[0] = P->hred[0];
+  out2[1] = P->hred[8];
+  return 0;
+}
+
+int ofdft_tn_direction(ofdft_plan P, const void* phi, const void* res, const void* W0, void* d, void* work,
+                       int maxfun, int chunk, int* niter, int* status, double* pAp_last) {
+  if (!valid(P) || !phi || !res || !W0 || !d || !work || !niter || !status || !pAp_last)
+    return fail(OFDFT_EINVAL, "null argument");
+  if (maxfun < 1) return fail(OFDFT_EINVAL, "maxfun must be positive");
+  if (P->prec == OFDFT_F64)
+    return tn_direction_impl<double>(P, (const double*)phi, (const double*)res, (const double*)W0, (double*)d,
+                                     (double*)work, maxfun, chunk, niter, status, pAp_last);
+  return tn_direction_impl<float>(P, (const float*)phi, (const float*)res, (const float*)W0, (float*)d, (float*)work,
+                                  maxfun, chunk, niter, status, pAp_last);
+}
+
 int ofdft_fft_r2c(ofdft_plan P, const void* in, void* out) {
   if (!valid(P) || !in || !out) return fail(OFDFT_EINVAL, "null argument");
   if (P->prec == OFDFT_F64) return r2c_impl<double>(P, (const double*)in, (cplx<double>*)out);
@@ -2082,12 +2525,12 @@ int ofdft_vloc_build(ofdft_plan P, int nspecies, const ofdft_vline* tables, int 
   P->order = order;
   if ((rc = upload_ions(P, nat, positions, species))) return rc;
   GridGeom g = geom(P);
-  auto* acc = reinterpret_cast<cplx<double>*>(P->spec) + P->Nh;  // slot 1
+  auto* acc = reinterpret_cast<cplx<double>*>(P->spec) + P->Nhp;  // slot 1
   const auto* spl = reinterpret_cast<const SplineDev*>(P->d_splines);
   if (P->pme) {
     if ((rc = ensure_work(P))) return rc;
     if ((rc = ensure_bm(P, order))) return rc;
-    HIPCHK(hipMemsetAsync(acc, 0, sizeof(cplx<double>) * P->Nh, P->stream));
+    HIPCHK(hipMemsetAsync(acc, 0, sizeof(cplx<double>) * P->Nhp, P->stream));
     for (int s = 0; s < nspecies; ++s) {
       HIPCHK(hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream));
       hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(1024), 0, P->stream, P->d_work, P->d_frac, P->d_species, s,
@@ -2106,7 +2549,7 @@ int ofdft_vloc_build(ofdft_plan P, int nspecies, const ofdft_vline* tables, int 
                        P->d_species, nat, 1.0 / P->dV);
     LAUNCHCHK();
   }
-  HIPCHK(hipMemcpyAsync(P->spec, acc, sizeof(cplx<double>) * P->Nh, hipMemcpyDeviceToDevice, P->stream));
+  HIPCHK(hipMemcpyAsync(P->spec, acc, sizeof(cplx<double>) * P->Nhp, hipMemcpyDeviceToDevice, P->stream));
   if ((rc = c2r_slot0(P, (double*)vloc_out))) return rc;
   return 0;
 }
@@ -2119,7 +2562,7 @@ int ofdft_local_forces(ofdft_plan P, const void* rho, double* forces_out) {
   int rc;
   GridGeom g = geom(P);
   const auto* spl = reinterpret_cast<const SplineDev*>(P->d_splines);
-  auto* rhog = reinterpret_cast<cplx<double>*>(P->spec) + P->Nh;  // slot 1 = rfftn(rho)
+  auto* rhog = reinterpret_cast<cplx<double>*>(P->spec) + P->Nhp;  // slot 1 = rfftn(rho)
   if ((rc = r2c_slot(P, (const double*)rho, 1))) return rc;
   std::vector<double> F(3 * (size_t)P->nat, 0.0);
   if (!P->pme) {

@@ -15,9 +15,12 @@ difference (``hvp='fd'`` reproduces that one).
 
 Options beyond the reference's:
   hvp            'exact' | 'fd'
-  precision      'fp64' | 'fp32' | 'mixed' (default: the dtype of guess_rho)
-  mixed_switch   mixed mode: leave FP32 once |dE| < mixed_switch (Ha);
-                 default 5e-10 * |E|
+  precision      'fp64' | 'fp32' | 'mixed' (default: the dtype of guess_rho).
+                 'mixed': FP64 energies, potentials, line search and phi;
+                 the TN inner conjugate-gradient Hessian products in FP32
+  mixed_phase1   mixed mode: run FP32 outer iterations first and switch to
+                 the FP64 outer loop once |dE| < mixed_switch (Ha, default
+                 5e-10 * |E|) or the FP32 line search fails
   ls_first_step  'unit' (the reference code: first trial theta = 1) or
                  'theta' (first trial = the running theta)
 """
@@ -46,12 +49,14 @@ class _Work:
         self.phi, self.V, self.Vt, self.res, self.res_prev, self.p = e(), e(), e(), e(), e(), e()
         self.aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
         self.auxt = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+        self.W0 = None  # HVP diagonal (V - mu + 2 rho a(rho)) at phi
         self._tn = None
 
     def tn(self):
         if self._tn is None:
             e = self.e
-            self._tn = dict(d=e(), r=e(), cgp=e(), Ap=e(), best=e(), tmp=e())
+            work = torch.empty((4,) + tuple(self.P.nr), dtype=self.P.dtype, device=self.P.device)
+            self._tn = dict(d=e(), work=work, r=work[0], cgp=work[1], Ap=work[2], best=work[3], tmp=e())
         return self._tn
 
 
@@ -81,6 +86,7 @@ class Optimization:
         self.timings = {}
         self.nfeval = 0
         self.nhvp = 0
+        self._tn_chunk = 4
         self.verbose = True
 
     # ------------------------------------------------------------------
@@ -101,11 +107,39 @@ class Optimization:
         return float(sum(W.ev.energies(sums).values())) + self._ii_energy
 
     # --- directions ---------------------------------------------------
-    def _direction_tn(self, W, mu):
+    def _direction_tn(self, W, mu, Wi=None):
         """get_direction_TN (optimization.py:92-148) with an exact or a
-        forward-difference Hessian-vector product."""
+        forward-difference Hessian-vector product.  With ``Wi`` (mixed
+        precision) the inner CG runs on FP32 copies of phi, W0 and the
+        residual; the direction is returned to W.p in W's precision."""
+        if Wi is not None:
+            if W.ev.configured_rho0 is not None:
+                Wi.ev.ensure_kernel(W.ev.configured_rho0)  # same WT kernel as the outer evaluator
+            Wi.P.convert_from(W.phi, Wi.phi)
+            Wi.P.convert_from(W.res, Wi.res)
+            if Wi.W0 is None:
+                Wi.W0 = Wi.e()
+            Wi.P.convert_from(W.W0, Wi.W0)
+            nd = self._tn_loop(Wi, mu)
+            W.P.convert_from(Wi.tn()["d"], W.p)
+            return nd
+        nd = self._tn_loop(W, mu)
+        W.p.copy_(W.tn()["d"])
+        return nd
+
+    def _tn_loop(self, W, mu):
         P = W.P
         T = W.tn()
+        if self.optimization_options["hvp"] == "exact":
+            # the whole CG loop on the device; chunk = iterations queued per
+            # host synchronisation, predicted from the previous call
+            n, stat, pAp = P.tn_direction(W.phi, W.res, W.W0, T["d"], T["work"],
+                                          self.optimization_options["maxfun"], self._tn_chunk)
+            self.nhvp += n
+            self._tn_chunk = max(2, n)
+            if stat == L.TN_FAILED:
+                self._print("!WARN : pAp small than zero :iter = ", n - 1, pAp)
+            return n
         d, r, cgp, Ap, best = T["d"], T["r"], T["cgp"], T["Ap"], T["best"]
         d.zero_()
         P.lincomb(-1.0, W.res, 0.0, W.res, r)
@@ -116,26 +150,22 @@ class Optimization:
         eps = 1.0e-9
         it = 0
         for it in range(self.optimization_options["maxfun"]):
-            if self.optimization_options["hvp"] == "exact":
-                pAp = P.hvp(W.phi, cgp, W.V, W.aux, mu, Ap)
-                self.nhvp += 1
-            else:
-                # Ap = ((V(phi1) - mu) phi1 - res0) / eps, phi1 = phi + eps p
-                tmp = T["tmp"]
-                self.nfeval += 1
-                W.ev.eval_phi(W.phi, cgp, 1.0, eps, Ap, None)
-                P.lincomb(1.0, W.phi, eps, cgp, tmp)
-                P.residual(Ap, tmp, mu, Ap)
-                P.lincomb(1.0 / eps, Ap, -1.0 / eps, W.res, Ap)
-                pAp = P.dot3(cgp, Ap)[0]
+            # Ap = ((V(phi1) - mu) phi1 - res0) / eps, phi1 = phi + eps p
+            tmp = T["tmp"]
+            self.nfeval += 1
+            W.ev.eval_phi(W.phi, cgp, 1.0, eps, Ap, None)
+            P.lincomb(1.0, W.phi, eps, cgp, tmp)
+            P.residual(Ap, tmp, mu, Ap)
+            P.lincomb(1.0 / eps, Ap, -1.0 / eps, W.res, Ap)
+            pAp = P.dot3(cgp, Ap)[0]
+            if pAp > 0.0:
+                r1 = P.cg_update(r0 / pAp, cgp, Ap, d, r)
             if pAp < 0.0:
                 if it == 0:
                     P.lincomb(r0 / pAp, cgp, 0.0, d, d)
                 else:
                     self._print("!WARN : pAp small than zero :iter = ", it, pAp)
                 break
-            alpha = r0 / pAp
-            r1 = P.cg_update(alpha, cgp, Ap, d, r)
             if r1 < min(rl):
                 best.copy_(d)
             rl.append(r1)
@@ -150,7 +180,6 @@ class Optimization:
             beta = r1 / r0
             r0 = r1
             P.lincomb(1.0, r, beta, cgp, cgp)
-        W.p.copy_(d)
         return it + 1
 
     def _direction_lbfgs(self, W):
@@ -246,6 +275,14 @@ class Optimization:
             self._works[key] = _Work(self.EnergyEvaluator.evaluator(grid, dt))
         return self._works[key]
 
+    def _residual(self, W, mu):
+        """(V - mu) phi into W.res; with the exact TN also the HVP diagonal."""
+        if self.optimization_method == "TN" and self.optimization_options["hvp"] == "exact":
+            if W.W0 is None:
+                W.W0 = W.e()
+            return W.P.hvp_prepare(W.phi, W.V, W.aux, mu, W.res, W.W0) * W.P.dV
+        return W.P.residual(W.V, W.phi, mu, W.res) * W.P.dV
+
     def _start(self, W):
         """Evaluate at W.phi and set V, aux, mu, residual."""
         sums = self._eval(W, W.phi, None, 0.0)
@@ -253,7 +290,7 @@ class Optimization:
         W.aux, W.auxt = W.auxt, W.aux
         E = self._energy(W, sums)
         mu = sums[L.S_VRHO] / sums[L.S_RHO]
-        resN = W.P.residual(W.V, W.phi, mu, W.res) * W.P.dV
+        resN = self._residual(W, mu)
         return E, mu, resN, sums
 
     # ------------------------------------------------------------------
@@ -274,8 +311,15 @@ class Optimization:
             raise ValueError("precision must be fp64, fp32 or mixed")
         self._works = {}
         self._ii_energy = self._ii()
-        cur = "fp32" if prec in ("fp32", "mixed") else "fp64"
+        phase1 = prec == "mixed" and bool(o.get("mixed_phase1"))
+        cur = "fp32" if (prec == "fp32" or phase1) else "fp64"
         W = self._work(cur)
+        # mixed precision: FP64 outer iterations with the TN inner CG in FP32
+        Wi = None
+        if prec == "mixed" and not phase1 and self.optimization_method == "TN" and o["hvp"] == "exact":
+            Wi = self._work("fp32")
+        for w in ([W, Wi] if Wi is not None else [W]):
+            w.P.bind_stream(True)
         t0 = time.perf_counter()
         with torch._C.DisableTorchFunctionSubclass():
             if guess_phi is None:
@@ -306,7 +350,7 @@ class Optimization:
         for it in range(1, o["maxiter"]):
             m = self.optimization_method
             if m == "TN":
-                nd = self._direction_tn(W, mu)
+                nd = self._direction_tn(W, mu, Wi if cur == "fp64" else None)
             elif m == "LBFGS":
                 nd = self._direction_lbfgs(W)
             elif m == "GD":
@@ -323,7 +367,7 @@ class Optimization:
             g0 = W.P.dot_vphip(W.V, W.phi, W.p) * 2.0 * W.P.dV
             stp, nls, cache = self._line_search(W, E, g0, theta)
             failed = stp is None
-            if failed and not (prec == "mixed" and cur == "fp32"):
+            if failed and not (phase1 and cur == "fp32"):
                 converged = 1
                 self._print("!!!ERROR : Line-Search Failed!!!")
                 self._print("!!!ERROR : Density Optimization NOT Converged  !!!")
@@ -341,7 +385,7 @@ class Optimization:
                 Enew = self._energy(W, sums)
                 mu = sums[L.S_VRHO] / sums[L.S_RHO]
                 W.res_prev.copy_(W.res)
-                resN = W.P.residual(W.V, W.phi, mu, W.res) * W.P.dV
+                resN = self._residual(W, mu)
                 if m == "LBFGS":
                     self._lbfgs_update(W, old_phi)
                 if m.startswith("CG"):
@@ -356,10 +400,12 @@ class Optimization:
                 self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
                     it, E, hist[-1] - hist[-2], resN, nd, nls, self.log[-1]["time"]))
             # mixed precision: leave FP32 near its noise floor
-            if prec == "mixed" and cur == "fp32" and (failed or (len(hist) > 1 and abs(hist[-1] - hist[-2]) < switch)):
+            if phase1 and cur == "fp32" and (failed or (len(hist) > 1 and abs(hist[-1] - hist[-2]) < switch)):
                 W32 = W
                 cur = "fp64"
                 W = self._work("fp64")
+                W.P.bind_stream(True)
+                Wi = self._work("fp32") if (self.optimization_method == "TN" and o["hvp"] == "exact") else None
                 W.P.convert_from(W32.phi, W.phi)
                 # FP64 electron-count constraint: FP32 rotations drift the norm
                 nphi = W.P.dot3(W.phi, W.phi)[1] * W.P.dV
@@ -382,6 +428,8 @@ class Optimization:
                 break
         else:
             self._print("!WARN: Not converged, but reached max steps")
+        for w in self._works.values():
+            w.P.bind_stream(False)
         self.mu = mu
         self._print("Chemical potential (a.u.):", mu)
         self._print("Chemical potential (eV)  :", mu * HARTREE_EV)

@@ -48,6 +48,7 @@ class Plan:
             L.check(self.lib.ofdft_plan_create(ctypes.byref(self._h), self.nr[0], self.nr[1], self.nr[2], lat,
                                                self.prec_code, 1 if self.full else 0, ctypes.c_void_p(stream)))
         self.functional = None
+        self.bound = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -60,8 +61,39 @@ class Plan:
 
     # ------------------------------------------------------------------
     def _stream(self):
+        if self.bound:
+            return
         s = torch.cuda.current_stream(self.device).cuda_stream
         L.check(self.lib.ofdft_plan_set_stream(self._h, ctypes.c_void_p(s)))
+
+    def bind_stream(self, on=True):
+        """Pin the plan to the current torch stream and skip the per-call
+        stream lookup (inner loops); bind_stream(False) restores it."""
+        self.bound = False
+        if on:
+            self._stream()
+            self.bound = True
+
+    def tn_inner(self, phi, p, W0, Ap, d, r, r0):
+        """Fused inner TN step on buffers owned by the caller (no argument
+        checks: the optimizer allocates them from this plan)."""
+        out = (ctypes.c_double * 2)()
+        self._stream()
+        L.check(self.lib.ofdft_tn_inner(self._h, phi.data_ptr(), p.data_ptr(), W0.data_ptr(), Ap.data_ptr(),
+                                        d.data_ptr(), r.data_ptr(), float(r0), out))
+        return out[0], out[1]
+
+    def tn_direction(self, phi, res, W0, d, work, maxfun, chunk=4):
+        """get_direction_TN with the exact HVP as one device-resident loop.
+        ``work`` holds 4 fields (r, p, Ap, Best).  Returns (number, status,
+        pAp of the last iteration)."""
+        n, st, pAp = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._stream()
+        L.check(self.lib.ofdft_tn_direction(self._h, self._real(phi, "phi"), self._real(res, "res"),
+                                            self._real(W0, "W0"), self._real(d, "d"),
+                                            self._realn(work, "work", 4 * self.N), int(maxfun), int(chunk),
+                                            ctypes.byref(n), ctypes.byref(st), ctypes.byref(pAp)))
+        return n.value, st.value, pAp.value
 
     def _real(self, t, name):
         if t is None:
@@ -126,6 +158,21 @@ class Plan:
         auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
         L.check(self.lib.ofdft_hvp(self._h, self._real(phi, "phi"), self._real(p, "p"), self._real(V, "V"), auxp,
                                    float(mu), self._real(Ap, "Ap"), ctypes.byref(out)))
+        return out.value
+
+    def hvp_prepare(self, phi, V, aux, mu, res, W0):
+        out = ctypes.c_double()
+        self._stream()
+        auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
+        L.check(self.lib.ofdft_hvp_prepare(self._h, self._real(phi, "phi"), self._real(V, "V"), auxp, float(mu),
+                                           self._real(res, "res"), self._real(W0, "W0"), ctypes.byref(out)))
+        return out.value
+
+    def hvp_fast(self, phi, p, W0, Ap):
+        out = ctypes.c_double()
+        self._stream()
+        L.check(self.lib.ofdft_hvp_fast(self._h, self._real(phi, "phi"), self._real(p, "p"), self._real(W0, "W0"),
+                                        self._real(Ap, "Ap"), ctypes.byref(out)))
         return out.value
 
     def r2c(self, f, out=None):
@@ -196,8 +243,18 @@ class Plan:
 
     PROFILE_CLASSES = ("rows_fwd", "axis_c2c", "pencil_conv", "rows_inv", "vector")
 
-    def set_profiling(self, on=True):
-        L.check(self.lib.ofdft_plan_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on=True, classes=None):
+        """Time kernel launches with HIP events: all classes, or only the
+        named ones (PROFILE_CLASSES); resets the counters."""
+        if not on:
+            mask = 0
+        elif classes is None:
+            mask = (1 << len(self.PROFILE_CLASSES)) - 1
+        else:
+            mask = 0
+            for c in classes:
+                mask |= 1 << self.PROFILE_CLASSES.index(c)
+        L.check(self.lib.ofdft_plan_set_profiling(self._h, mask))
 
     def get_profile(self):
         """{class: (ms_total, launches, algorithmic_bytes_total)}"""

@@ -52,6 +52,15 @@ extern "C" {
 #define OFDFT_S_DTHETA 8  /* 2*integral V*phi(t)*dphi/dt dV (optimization.py:270-277) */
 #define OFDFT_NSUM 9
 
+/* Outcome of ofdft_tn_direction, the `stat` strings of
+ * get_direction_TN (optimization.py:102-142).                           */
+#define OFDFT_TN_CONV 0       /* "CONV": r1 < 0.1 r0                         */
+#define OFDFT_TN_WARN 1       /* "WARN": pAp < 0 at the first iteration      */
+#define OFDFT_TN_FAILED 2     /* "FAILED": pAp < 0 later                     */
+#define OFDFT_TN_NOTREDUCE 3  /* "WARN : Not reduce": r1 > 1000 min(r)       */
+#define OFDFT_TN_SMALL 4      /* "WARN : Change too small"                   */
+#define OFDFT_TN_MAXFUN 5     /* "NOTCONV": maxfun iterations taken          */
+
 typedef struct ofdft_plan_s* ofdft_plan;
 
 /* Functional selection and options.  Replaces the construction of
@@ -92,8 +101,10 @@ long long ofdft_plan_workspace_bytes(ofdft_plan plan);
  * 4 vector kernels.  get_profile synchronises and returns, per class
  * (arrays of 5), total milliseconds, launch count and the algorithmic
  * bytes (each launch's compulsory reads + writes) summed over launches.
- * set_profiling resets the counters.                                     */
-int ofdft_plan_set_profiling(ofdft_plan plan, int on);
+ * set_profiling resets the counters; `mask` selects the classes timed
+ * (bit c for class c, 0 = off, 31 = all), so that a caller can time one
+ * kernel class without adding events around every other launch.        */
+int ofdft_plan_set_profiling(ofdft_plan plan, int mask);
 int ofdft_plan_get_profile(ofdft_plan plan, double* ms, long long* count, double* bytes);
 
 /* Functional options; builds the WT kernel table on the device when
@@ -126,6 +137,35 @@ int ofdft_eval_phi(ofdft_plan plan, const void* phi, const void* p, double c, do
  * Synchronises the stream.                                               */
 int ofdft_hvp(ofdft_plan plan, const void* phi, const void* p, const void* V, const void* aux, double mu,
               void* Ap_out, double* pAp_out);
+
+/* The same product in two steps for the TN inner loop.  hvp_prepare, once
+ * per outer iteration, writes the residual res = (V - mu) phi
+ * (optimization.py:323, :469; *res2 = sum res^2) and the local part
+ * W0 = V - mu + 2 rho a(rho), a = d(V_TF + V_XC + V_vW,local + V_WT,local)/drho.
+ * hvp_fast then forms Ap from phi, p and W0 plus the three convolutions;
+ * it equals ofdft_hvp up to rounding.                                    */
+int ofdft_hvp_prepare(ofdft_plan plan, const void* phi, const void* V, const void* aux, double mu, void* res,
+                      void* W0, double* res2);
+int ofdft_hvp_fast(ofdft_plan plan, const void* phi, const void* p, const void* W0, void* Ap, double* pAp_out);
+
+/* One inner step of get_direction_TN (optimization.py:105-145) with the
+ * exact HVP: Ap = hvp_fast(p); pAp = sum p*Ap; if pAp > 0: a = r0/pAp,
+ * d += a p, r -= a Ap (on the device); r1 = sum r^2.  out2 = {pAp, r1};
+ * one stream synchronisation.                                            */
+int ofdft_tn_inner(ofdft_plan plan, const void* phi, const void* p, const void* W0, void* Ap, void* d, void* r,
+                   double r0, double* out2);
+
+/* The whole of get_direction_TN (optimization.py:92-148) with the exact
+ * HVP, run on the device: d = direction (N reals, output), res = residual
+ * (phi-gradient, input), work = 4*N reals of scratch (r, p, Ap, Best).
+ * The host queues `chunk` CG iterations at a time without waiting; a
+ * one-thread control kernel applies the reference's branch tests (pAp < 0,
+ * r1 < rConv, r1 > 1000 min(rLists), stagnation after 10 iterations) and
+ * ends the loop, after which queued kernels return at once.  One stream
+ * synchronisation per chunk.  niter = the reference's `number`, status =
+ * OFDFT_TN_*, pAp_last = pAp of the last iteration.                      */
+int ofdft_tn_direction(ofdft_plan plan, const void* phi, const void* res, const void* W0, void* d, void* work,
+                       int maxfun, int chunk, int* niter, int* status, double* pAp_last);
 
 /* Raw transforms (DirectField.fft field.py:413-430 without the dV factor;
  * ReciprocalField.ifft field.py:779-804 without the 1/dV factor):

@@ -154,3 +154,119 @@ def test_hvp_matches_central_difference():
     print("hvp rel err", err)
     assert err < 1e-7
     assert abs(pAp - np.sum(p * A)) < 1e-9 * abs(pAp)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_hvp_fast_equals_hvp(prec):
+    """hvp_prepare + hvp_fast (precomputed local part) == ofdft_hvp."""
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    P = _plan(wl["nr"], wl["cell"], prec)
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_vloc(torch.tensor(pp.vreal, dtype=P.dtype, device=P.device))
+    t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
+    phi = np.sqrt(rho)
+    p = np.random.default_rng(5).standard_normal(rho.shape) * phi * 0.1
+    V = P.empty()
+    aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+    P.eval_phi(t(phi), None, 1.0, 0.0, V, aux)
+    mu = 0.31
+    A1, A2, res, W0 = P.empty(), P.empty(), P.empty(), P.empty()
+    q1 = P.hvp(t(phi), t(p), V, aux, mu, A1)
+    r2 = P.hvp_prepare(t(phi), V, aux, mu, res, W0)
+    q2 = P.hvp_fast(t(phi), t(p), W0, A2)
+    tol = 1e-12 if prec == "fp64" else 2e-5
+    a1, a2 = A1.cpu().double().numpy(), A2.cpu().double().numpy()
+    assert np.abs(a1 - a2).max() <= tol * np.abs(a1).max()
+    assert abs(q1 - q2) <= tol * abs(q1) * 10
+    ref = ((V.cpu().double().numpy() - mu) * phi)
+    assert np.abs(res.cpu().double().numpy() - ref).max() <= tol * np.abs(ref).max() * 10
+    assert abs(r2 - np.sum(ref ** 2)) <= 1e-4 * r2
+
+
+def _tn_host(hvp, res0, maxfun):
+    """get_direction_TN (optimization.py:92-148) on the host, step for step,
+    with Ap from the given product; `Best` aliases `direction` until the
+    first copy, as in the reference."""
+    direction = np.zeros_like(res0)
+    res = -res0.copy()
+    p = res.copy()
+    r0 = np.sum(res * res)
+    rconv = r0 * 0.1
+    best, alias = None, True
+    rl = [1e10]
+    stat, pAp = "NOTCONV", 0.0
+    for it in range(maxfun):
+        Ap = hvp(p)
+        pAp = np.sum(p * Ap)
+        if pAp < 0.0:
+            if it == 0:
+                direction = r0 / pAp * p
+                stat = "WARN"
+            else:
+                stat = "FAILED"
+            break
+        alpha = r0 / pAp
+        direction = direction + alpha * p
+        res = res - alpha * Ap
+        r1 = np.sum(res * res)
+        if r1 < min(rl):
+            best, alias = direction.copy(), False
+        rl.append(r1)
+        if r1 < rconv:
+            stat = "CONV"
+            break
+        elif r1 > 1000 * min(rl[:-1]):
+            stat = "WARN : Not reduce"
+            direction = direction if alias else best
+            break
+        elif it > 10 and abs(r0 - r1) < 0.1 * r0:
+            stat = "WARN : Change too small"
+            direction = direction if alias else best
+            break
+        beta = r1 / r0
+        r0 = r1
+        p = res + beta * p
+    return direction, it + 1, stat, pAp
+
+
+@pytest.mark.parametrize("mu,maxfun", [(0.31, 50), (0.31, 2), (0.31, 1), (40.0, 50)])
+@pytest.mark.parametrize("chunk", [1, 3, 16])
+def test_tn_direction_matches_host_loop(mu, maxfun, chunk):
+    """The device-resident CG loop takes the branches of the reference loop
+    (convergence, maxfun, negative curvature at the first iteration) and
+    returns its direction and iteration count."""
+    from dftpy_amd import _lib as L
+    print("case tn_direction", mu, maxfun, chunk, flush=True)
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    P = _plan(wl["nr"], wl["cell"], "fp64")
+    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_vloc(torch.tensor(pp.vreal, dtype=P.dtype, device=P.device))
+    t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
+    phi = t(np.sqrt(rho))
+    V = P.empty()
+    aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+    P.eval_phi(phi, None, 1.0, 0.0, V, aux)
+    res, W0 = P.empty(), P.empty()
+    P.hvp_prepare(phi, V, aux, mu, res, W0)
+    Ap = P.empty()
+
+    def hvp(pv):
+        P.hvp_fast(phi, t(pv), W0, Ap)
+        return Ap.cpu().numpy()
+
+    dref, nref, sref, pref = _tn_host(hvp, res.cpu().numpy(), maxfun)
+    d = P.zeros()
+    work = torch.full((4,) + tuple(P.nr), 7.0, dtype=P.dtype, device=P.device)  # stale scratch must not matter
+    n, st, pAp = P.tn_direction(phi, res, W0, d, work, maxfun, chunk)
+    assert (n, L.TN_STATUS[st]) == (nref, sref)
+    if mu > 1.0:
+        assert sref == "WARN" and n == 1
+    assert abs(pAp - pref) <= 1e-10 * abs(pref)
+    dd = d.cpu().numpy()
+    assert np.abs(dd - dref).max() <= 1e-9 * np.abs(dref).max()
+    # a second call on the same plan starts from a clean state
+    n2, st2, _ = P.tn_direction(phi, res, W0, d, work, maxfun, chunk)
+    assert (n2, st2) == (n, st)
+    assert np.abs(d.cpu().numpy() - dd).max() <= 1e-12 * np.abs(dd).max()

@@ -111,15 +111,17 @@ def test_al4_32_lbfgs_and_cg_reach_tn_minimum():
         assert abs(opt.energy - ref) < 1e-8
 
 
-def test_al4_32_mixed_precision_within_1e6_ev_per_atom():
+@pytest.mark.parametrize("phase1", [False, True])
+def test_al4_32_mixed_precision_within_1e6_ev_per_atom(phase1):
     wl, og, otot, grid, ev = _al4_pair()
     rho = D.DirectField(grid=grid, data=wl["rho_ini"])
     o64 = D.Optimization(EnergyEvaluator=ev, optimization_method="TN", optimization_options={"econv": 1e-9})
     o64.optimize_rho(guess_rho=rho)
     rho = D.DirectField(grid=grid, data=wl["rho_ini"])
     omx = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
-                         optimization_options={"econv": 1e-9, "precision": "mixed"})
+                         optimization_options={"econv": 1e-9, "precision": "mixed", "mixed_phase1": phase1})
     omx.optimize_rho(guess_rho=rho)
     err = abs(omx.energy - o64.energy) * W.HARTREE / 4
-    print("mixed", omx.energy, o64.energy, err, [l["prec"] for l in omx.log])
+    print("mixed", phase1, omx.energy, o64.energy, err, [l["prec"] for l in omx.log])
+    assert omx.converged == 0
     assert err < 1e-6
