@@ -104,7 +104,7 @@ FFTLaunch fft_launch(const Pass1D& pl, int ntr, int min_threads = 64) {
 struct ofdft_plan_s {
   int n[3];
   int nh;            // n2/2 + 1 logical half-spectrum length
-  int nhp;           // row stride of the internal spectrum (nh rounded up to even)
+  int nhp;           // row stride of the internal spectrum (nh rounded up to a multiple of 16)
   long long N, Nh, Nhp;
   int prec;
   int full;   // reference full=True semantics (Hermitian-symmetrised multipliers)
@@ -886,7 +886,8 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
   const long long stride = (long long)geo.n1 * geo.nhp;
   cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nhp;
   const int c = blockIdx.x * W + w;
-  const bool ok = c < geo.nh;
+  const bool ok = c < geo.nhp;     // padded columns are transformed (they stay zero)
+  const bool live = c < geo.nh;
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1)
@@ -905,7 +906,7 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
     const T scale = T(ck.scale);
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
-      const T m = ok ? conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk) : T(0);
+      const T m = live ? conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk) : T(0);
       v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
     }
     dftreg<T, N2>(v, true);
@@ -1354,8 +1355,8 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
 #define OFDFT_CASE(NN)                                                                                       \
     case NN:                                                                                                 \
       hipLaunchKernelGGL((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
-                         dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),     \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nhp, stride, ostride, P->nh, \
+                         dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),    \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nhp, stride, ostride, P->nhp, \
                          reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);               \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
@@ -1398,7 +1399,7 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
 #define OFDFT_CASE(NN)                                                                                   \
     case NN:                                                                                             \
       hipLaunchKernelGGL((k_pencil_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                 \
-                         dim3((unsigned)((P->nh + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1],  \
+                         dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1], \
                               (unsigned)F),                                                              \
                          dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, reinterpret_cast<cplx<T>*>(P->spec), \
                          P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                      \
@@ -2180,7 +2181,9 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   P->n[1] = n1;
   P->n[2] = n2;
   P->nh = n2 / 2 + 1;
-  P->nhp = P->nh + (P->nh & 1);
+  // rows padded to a multiple of 16 entries: every 16-column tile of the
+  // column kernels starts on a 128-byte boundary and covers whole lines
+  P->nhp = (P->nh + 15) / 16 * 16;
   P->N = (long long)n0 * n1 * n2;
   P->Nh = (long long)n0 * n1 * P->nh;
   P->Nhp = (long long)n0 * n1 * P->nhp;
