@@ -15,6 +15,7 @@
 // plus a tiny reduction launch.  All fields of one evaluation (up to three
 // spectra) travel through every launch together.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdio>
@@ -142,6 +143,7 @@ struct ofdft_plan_s {
   struct ProfRec { int cls; long long bytes; int e0, e1, tag; };
   std::vector<ProfRec> prof_pending;
   int prof_tag = -1;  // CG iteration a launch belongs to (-1: none)
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;  // events of the launch being profiled
   double prof_ms[8];
   long long prof_count[8];
   double prof_bytes[8];
@@ -153,10 +155,15 @@ constexpr int kMaxSum = 16;
 constexpr int kRowPairsC = 2;   // row pairs per rows_* workgroup
 constexpr int kColTileC = 16;   // W: contiguous columns per axis-1 / axis-0 tile
 
-// kernel classes reported by ofdft_plan_get_profile
-enum { PC_ROWS_FWD = 0, PC_AXIS = 1, PC_PENCIL = 2, PC_ROWS_INV = 3, PC_VEC = 4, PC_NCLS = 5 };
+// kernel classes reported by ofdft_plan_get_profile (OFDFT_NPROF of them):
+// one per kernel variant that a solve runs
+enum { PC_ROWS_FWD = 0, PC_ROWS_FWD_HVP = 1, PC_AXIS = 2, PC_PENCIL = 3, PC_ROWS_INV = 4, PC_ROWS_INV_HVP = 5,
+       PC_VEC = 6, PC_NCLS = 7 };
 
 // Event bracket around one launch (profiling only).
+// Times one kernel launch: the launch made inside the scope passes
+// P->ev_start / P->ev_stop to hipExtLaunchKernel, which stamps them from the
+// dispatch itself (the kernel's own start and end, no gap around it).
 struct ProfScope {
   ofdft_plan_s* P;
   int cls;
@@ -171,14 +178,19 @@ struct ProfScope {
     }
     e0 = (int)P->ev_used;
     P->ev_used += 2;
-    (void)hipEventRecord(P->ev_pool[e0], P->stream);
+    P->ev_start = P->ev_pool[e0];
+    P->ev_stop = P->ev_pool[e0 + 1];
   }
   ~ProfScope() {
     if (e0 < 0) return;
-    (void)hipEventRecord(P->ev_pool[e0 + 1], P->stream);
+    P->ev_start = P->ev_stop = nullptr;
     P->prof_pending.push_back({cls, bytes, e0, e0 + 1, P->prof_tag});
   }
 };
+
+// launch on the plan stream, timed when a ProfScope is open
+#define OFDFT_LAUNCH(K, G, B, S, P, ...) \
+  hipExtLaunchKernelGGL(K, G, B, S, (P)->stream, (P)->ev_start, (P)->ev_stop, 0u, __VA_ARGS__)
 
 // ---------------------------------------------------------------------------
 // device helpers
@@ -363,6 +375,7 @@ __device__ __forceinline__ double gsq(const GridGeom& g, int k0, int k1, int k2,
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PreCopy {
+  static constexpr int kProf = PC_ROWS_FWD;
   const T* __restrict__ in;
   __host__ __device__ int nin() const { return 1; }
   __device__ __forceinline__ void operator()(long long i, T* out) const { out[0] = in[i]; }
@@ -373,6 +386,7 @@ struct PreCopy {
 // (WT, wt.py:15-19).
 template <typename T>
 struct PreEval {
+  static constexpr int kProf = PC_ROWS_FWD;
   const T* __restrict__ rho;
   const T* __restrict__ phi;
   const T* __restrict__ p;
@@ -402,6 +416,7 @@ struct PreEval {
 // (vW), beta*rho^(beta-1)*drho (WT).
 template <typename T>
 struct PreHvp {
+  static constexpr int kProf = PC_ROWS_FWD_HVP;
   const T* __restrict__ phi;
   const T* __restrict__ p;
   FnParams fp;
@@ -427,6 +442,7 @@ struct PreHvp {
 template <typename T>
 struct PostStore {
   static constexpr int NS = 1;
+  static constexpr int kProf = PC_ROWS_INV;
   T* __restrict__ out;
   __host__ __device__ int nio() const { return 1; }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { out[i] = c[0]; }
@@ -435,6 +451,7 @@ struct PostStore {
 template <typename T>
 struct PostEval {
   static constexpr int NS = OFDFT_NSUM;
+  static constexpr int kProf = PC_ROWS_INV;
   PreEval<T> pre;
   const T* __restrict__ vloc;
   T* __restrict__ V;
@@ -510,6 +527,7 @@ struct PostEval {
 template <typename T>
 struct PostHvp {
   static constexpr int NS = 1;
+  static constexpr int kProf = PC_ROWS_INV_HVP;
   const T* __restrict__ phi;
   const T* __restrict__ p;
   const T* __restrict__ V;
@@ -549,6 +567,7 @@ struct PostHvp {
 template <typename T>
 struct PostHvpFast {
   static constexpr int NS = 1;
+  static constexpr int kProf = PC_ROWS_INV_HVP;
   const T* __restrict__ phi;
   const T* __restrict__ p;
   const T* __restrict__ W0;
@@ -1257,7 +1276,7 @@ using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
 template <typename T, class Pre, class FFT>
 void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
-  hipLaunchKernelGGL((k_rows_fwd<T, FFT, Pre>), grid, dim3(threads), lds, P->stream, pre, fft,
+  OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre>), grid, dim3(threads), lds, P, pre, fft,
                      reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
                      reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
 }
@@ -1269,7 +1288,7 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const int PB = kRowPairsC;
   const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   dim3 grid((unsigned)((npairs + PB - 1) / PB));
-  ProfScope ps(P, PC_ROWS_FWD, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
+  ProfScope ps(P, Pre::kProf, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
   switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
     case NN: go_rows_fwd<T>(P, pre, RowsFFT<NN>{}, F, grid, kRowsTH, lds); break;
@@ -1289,7 +1308,7 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
 template <typename T, class Post, class FFT>
 void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const cplx<T>* src, dim3 grid,
                  int threads, size_t lds, double* part) {
-  hipLaunchKernelGGL((k_rows_inv<T, FFT, Post>), grid, dim3(threads), lds, P->stream, post, fft, src,
+  OFDFT_LAUNCH((k_rows_inv<T, FFT, Post>), grid, dim3(threads), lds, P, post, fft, src,
                      (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
 }
 
@@ -1304,7 +1323,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   dim3 grid((unsigned)nblk);
   double* part = reduce ? P->partials : nullptr;
   {
-    ProfScope ps(P, PC_ROWS_INV, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
+    ProfScope ps(P, Post::kProf, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
     switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
       case NN: go_rows_inv<T>(P, post, RowsFFT<NN>{}, F, src, grid, kRowsTH, lds, part); break;
@@ -1329,7 +1348,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
 template <typename T, class FFT>
 void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid, int threads, size_t lds,
              long long stride, long long ostride, bool inv) {
-  hipLaunchKernelGGL((k_axis_c2c<T, FFT>), grid, dim3(threads), lds, P->stream, fft, spec, P->Nhp, stride, ostride,
+  OFDFT_LAUNCH((k_axis_c2c<T, FFT>), grid, dim3(threads), lds, P, fft, spec, P->Nhp, stride, ostride,
                      P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
@@ -1354,9 +1373,9 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   switch (n) {
 #define OFDFT_CASE(NN)                                                                                       \
     case NN:                                                                                                 \
-      hipLaunchKernelGGL((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
+      OFDFT_LAUNCH((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                             \
                          dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),    \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, spec, P->Nhp, stride, ostride, P->nhp, \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P, spec, P->Nhp, stride, ostride, P->nhp, \
                          reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);               \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
@@ -1376,7 +1395,7 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
 template <typename T, class FFT>
 void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t lds, const GridGeom& g,
                const ConvKinds& ck) {
-  hipLaunchKernelGGL((k_pencil_conv<T, FFT>), grid, dim3(threads), lds, P->stream, fft,
+  OFDFT_LAUNCH((k_pencil_conv<T, FFT>), grid, dim3(threads), lds, P, fft,
                      reinterpret_cast<cplx<T>*>(P->spec), P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,
                      reinterpret_cast<const T*>(P->wtk), P->skip);
 }
@@ -1398,10 +1417,10 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
   switch (P->n[0]) {
 #define OFDFT_CASE(NN)                                                                                   \
     case NN:                                                                                             \
-      hipLaunchKernelGGL((k_pencil_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                 \
+      OFDFT_LAUNCH((k_pencil_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
                          dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1], \
                               (unsigned)F),                                                              \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P->stream, reinterpret_cast<cplx<T>*>(P->spec), \
+                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), \
                          P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                      \
                          reinterpret_cast<const T*>(P->wtk), P->skip);                                   \
       break;
@@ -2103,7 +2122,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   P->skip = nullptr;
   {
     ProfScope ps(P, PC_VEC, (long long)P->rs * 4 * N);
-    hipLaunchKernelGGL((k_tn_init<T>), dim3(nb), dim3(256), 0, P->stream, res, d, r, p, N, P->partials);
+    OFDFT_LAUNCH((k_tn_init<T>), dim3(nb), dim3(256), 0, P, res, d, r, p, N, P->partials);
   }
   LAUNCHCHK();
   hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr,
@@ -2121,7 +2140,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
       if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr))) break;
       {
         ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * N);
-        hipLaunchKernelGGL((k_tn_step<T>), dim3(nb), dim3(256), 0, P->stream, st, (const double*)P->red,
+        OFDFT_LAUNCH((k_tn_step<T>), dim3(nb), dim3(256), 0, P, st, (const double*)P->red,
                            (const T*)p, (const T*)Ap, d, r, N, P->partials);
       }
       hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials,
@@ -2129,7 +2148,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
       hipLaunchKernelGGL(k_tn_ctrl, dim3(1), dim3(1), 0, P->stream, st, (const double*)P->red, (const double*)rr);
       {
         ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
-        hipLaunchKernelGGL((k_tn_apply<T>), dim3(nb), dim3(256), 0, P->stream, (const TnState*)st, d, (const T*)r,
+        OFDFT_LAUNCH((k_tn_apply<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, d, (const T*)r,
                            p, best, N);
       }
       if (hipGetLastError() != hipSuccess) {
@@ -2145,7 +2164,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
       break;
     }
     if (h->done) break;
-    k = std::max(2, k / 2);
+    k = 2;
   }
   P->prof_tag = -1;
   P->skip = nullptr;
@@ -2324,7 +2343,11 @@ int ofdft_plan_get_profile(ofdft_plan P, double* ms, long long* count, double* b
     HIPCHK(hipStreamSynchronize(P->stream));
     for (const auto& r : P->prof_pending) {
       float t = 0.f;
-      HIPCHK(hipEventElapsedTime(&t, P->ev_pool[r.e0], P->ev_pool[r.e1]));
+      // a scope whose launch did not happen leaves its events unrecorded
+      if (hipEventElapsedTime(&t, P->ev_pool[r.e0], P->ev_pool[r.e1]) != hipSuccess) {
+        (void)hipGetLastError();
+        continue;
+      }
       P->prof_ms[r.cls] += t;
       P->prof_count[r.cls] += 1;
       P->prof_bytes[r.cls] += (double)r.bytes;
@@ -2411,10 +2434,10 @@ int ofdft_hvp_prepare(ofdft_plan P, const void* phi, const void* V, const void* 
   {
     ProfScope ps(P, PC_VEC, (long long)P->rs * 5 * P->N);
     if (P->prec == OFDFT_F64)
-      hipLaunchKernelGGL((k_hvp_prepare<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)phi,
+      OFDFT_LAUNCH((k_hvp_prepare<double>), dim3(nb), dim3(256), 0, P, (const double*)phi,
                          (const double*)V, (const double*)aux, mu, (double*)res, (double*)W0, P->N, fp, P->partials);
     else
-      hipLaunchKernelGGL((k_hvp_prepare<float>), dim3(nb), dim3(256), 0, P->stream, (const float*)phi,
+      OFDFT_LAUNCH((k_hvp_prepare<float>), dim3(nb), dim3(256), 0, P, (const float*)phi,
                          (const float*)V, (const float*)aux, (float)mu, (float*)res, (float*)W0, P->N, fp,
                          P->partials);
   }
@@ -2440,13 +2463,13 @@ int ofdft_tn_inner(ofdft_plan P, const void* phi, const void* p, const void* W0,
     if ((rc = hvp_fast_impl<double>(P, (const double*)phi, (const double*)p, (const double*)W0, (double*)Ap, nullptr)))
       return rc;
     ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * P->N);
-    hipLaunchKernelGGL((k_cg_update_dev<double>), dim3(nb), dim3(256), 0, P->stream, P->red, r0, (const double*)p,
+    OFDFT_LAUNCH((k_cg_update_dev<double>), dim3(nb), dim3(256), 0, P, P->red, r0, (const double*)p,
                        (const double*)Ap, (double*)d, (double*)r, P->N, P->partials);
   } else {
     if ((rc = hvp_fast_impl<float>(P, (const float*)phi, (const float*)p, (const float*)W0, (float*)Ap, nullptr)))
       return rc;
     ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * P->N);
-    hipLaunchKernelGGL((k_cg_update_dev<float>), dim3(nb), dim3(256), 0, P->stream, P->red, r0, (const float*)p,
+    OFDFT_LAUNCH((k_cg_update_dev<float>), dim3(nb), dim3(256), 0, P, P->red, r0, (const float*)p,
                        (const float*)Ap, (float*)d, (float*)r, P->N, P->partials);
   }
   LAUNCHCHK();

@@ -87,6 +87,7 @@ class Optimization:
         self.nfeval = 0
         self.nhvp = 0
         self._tn_chunk = 4
+        self._tn_hist = []
         self.verbose = True
 
     # ------------------------------------------------------------------
@@ -136,7 +137,10 @@ class Optimization:
             n, stat, pAp = P.tn_direction(W.phi, W.res, W.W0, T["d"], T["work"],
                                           self.optimization_options["maxfun"], self._tn_chunk)
             self.nhvp += n
-            self._tn_chunk = max(2, n)
+            # queue about as many iterations as the shortest recent loop took;
+            # the device queues 2 more per extra synchronisation after that
+            self._tn_hist = (self._tn_hist + [n])[-3:]
+            self._tn_chunk = max(2, min(self._tn_hist) - 1)
             if stat == L.TN_FAILED:
                 self._print("!WARN : pAp small than zero :iter = ", n - 1, pAp)
             return n

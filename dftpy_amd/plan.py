@@ -241,7 +241,7 @@ class Plan:
         L.check(self.lib.ofdft_convert(self._h, L.ptr(dst), sp, L.ptr(src), dst.numel()))
         return dst
 
-    PROFILE_CLASSES = ("rows_fwd", "axis_c2c", "pencil_conv", "rows_inv", "vector")
+    PROFILE_CLASSES = ("rows_fwd", "rows_fwd_hvp", "axis_c2c", "pencil_conv", "rows_inv", "rows_inv_hvp", "vector")
 
     def set_profiling(self, on=True, classes=None):
         """Time kernel launches with HIP events: all classes, or only the
@@ -258,9 +258,10 @@ class Plan:
 
     def get_profile(self):
         """{class: (ms_total, launches, algorithmic_bytes_total)}"""
-        ms = L.darr(5)
-        cnt = (ctypes.c_longlong * 5)()
-        by = L.darr(5)
+        n = len(self.PROFILE_CLASSES)
+        ms = L.darr(n)
+        cnt = (ctypes.c_longlong * n)()
+        by = L.darr(n)
         L.check(self.lib.ofdft_plan_get_profile(self._h, ms, cnt, by))
         return {k: (ms[i], int(cnt[i]), by[i]) for i, k in enumerate(self.PROFILE_CLASSES)}
 

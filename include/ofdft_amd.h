@@ -96,14 +96,17 @@ long long ofdft_plan_workspace_bytes(ofdft_plan plan);
 /* Per-kernel-class timing with HIP events recorded on the plan stream
  * around every launch (no reference counterpart: it replaces the
  * wall-clock @timer labels of time_data.py:6-122 with device time).
- * Classes: 0 rows_fwd (R2C + pre), 1 axis (strided C2C), 2 pencil
- * (axis-0 C2C + multiplier + inverse), 3 rows_inv (C2R + post),
- * 4 vector kernels.  get_profile synchronises and returns, per class
- * (arrays of 5), total milliseconds, launch count and the algorithmic
+ * Classes: 0 rows_fwd (R2C + evaluation pre-pointwise), 1 rows_fwd_hvp
+ * (R2C + HVP pre-pointwise), 2 axis (strided C2C), 3 pencil (axis-0 C2C +
+ * multiplier + inverse), 4 rows_inv (C2R + evaluation post), 5
+ * rows_inv_hvp (C2R + HVP post), 6 vector kernels.  get_profile
+ * synchronises and returns, per class (arrays of OFDFT_NPROF), total
+ * milliseconds, launch count and the algorithmic
  * bytes (each launch's compulsory reads + writes) summed over launches.
  * set_profiling resets the counters; `mask` selects the classes timed
- * (bit c for class c, 0 = off, 31 = all), so that a caller can time one
+ * (bit c for class c, 0 = off, 127 = all), so that a caller can time one
  * kernel class without adding events around every other launch.        */
+#define OFDFT_NPROF 7
 int ofdft_plan_set_profiling(ofdft_plan plan, int mask);
 int ofdft_plan_get_profile(ofdft_plan plan, double* ms, long long* count, double* bytes);
 
