@@ -267,6 +267,7 @@ __device__ __forceinline__ void fixed_pass(cplx<T>* lds, int ld, int ntr, const 
 template <int N, int NTR, int TH_>
 struct FFTFixed {
   static constexpr int TH = TH_;
+  static constexpr bool kFixed = true;
   __host__ __device__ static constexpr int n() { return N; }
   __device__ static int th() { return TH; }
   template <typename T>
@@ -280,6 +281,7 @@ template <int KB_>
 struct FFTRuntime {
   static constexpr int KB = KB_;
   static constexpr int TH = 1024;
+  static constexpr bool kFixed = false;
   Pass1D pl;
   __host__ __device__ int n() const { return pl.n; }
   __device__ static int th() { return blockDim.x; }

@@ -152,7 +152,10 @@ struct ofdft_plan_s {
 namespace {
 
 constexpr int kMaxSum = 16;
-constexpr int kRowPairsC = 2;   // row pairs per rows_* workgroup
+// row pairs per rows_* workgroup
+template <typename T>
+constexpr int kRowPB = 2;
+constexpr int kRowPBMin = 2;
 constexpr int kColTileC = 16;   // W: contiguous columns per axis-1 / axis-0 tile
 
 // kernel classes reported by ofdft_plan_get_profile (OFDFT_NPROF of them):
@@ -378,7 +381,12 @@ struct PreCopy {
   static constexpr int kProf = PC_ROWS_FWD;
   const T* __restrict__ in;
   __host__ __device__ int nin() const { return 1; }
-  __device__ __forceinline__ void operator()(long long i, T* out) const { out[0] = in[i]; }
+  struct In {
+    T v;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {in[i]}; }
+  __device__ __forceinline__ void apply(const In& x, T* out) const { out[0] = x.v; }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
 };
 
 // rho (mode 0) or phi(t) = c*phi + s*p (mode 1), then the convolution inputs
@@ -393,14 +401,23 @@ struct PreEval {
   T c, s;
   FnParams fp;
   __host__ __device__ int nin() const { return rho ? 1 : (p ? 2 : 1); }
-  __device__ __forceinline__ T density(long long i) const {
-    if (rho) return rho[i];
-    T ph = phi[i];
-    if (p) ph = c * ph + s * p[i];
+  // raw loads only (no arithmetic on them), so that callers can issue the
+  // loads of several points before the first use
+  struct In {
+    T a, b;
+  };
+  __device__ __forceinline__ In load(long long i) const {
+    if (rho) return {rho[i], T(0)};
+    return {phi[i], p ? p[i] : T(0)};
+  }
+  __device__ __forceinline__ T density(const In& x) const {
+    if (rho) return x.a;
+    const T ph = p ? c * x.a + s * x.b : x.a;
     return ph * ph;
   }
-  __device__ __forceinline__ void operator()(long long i, T* out) const {
-    const T r = density(i);
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out) const {
+    const T r = density(x);
     if (fp.fH >= 0) put3(out, fp.fH, r);
     const T s = dev_sqrt(r > T(0) ? r : T(kZero));
     if (fp.fV >= 0) put3(out, fp.fV, s);
@@ -421,8 +438,13 @@ struct PreHvp {
   const T* __restrict__ p;
   FnParams fp;
   __host__ __device__ int nin() const { return 2; }
-  __device__ __forceinline__ void operator()(long long i, T* out) const {
-    const T ph = phi[i], pp = p[i];
+  struct In {
+    T ph, pp;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i]}; }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out) const {
+    const T ph = x.ph, pp = x.pp;
     const T r = ph * ph;
     const T dr = T(2) * ph * pp;
     const T s = dev_sqrt(r);
@@ -445,7 +467,10 @@ struct PostStore {
   static constexpr int kProf = PC_ROWS_INV;
   T* __restrict__ out;
   __host__ __device__ int nio() const { return 1; }
-  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { out[i] = c[0]; }
+  struct In {};
+  __device__ __forceinline__ In load(long long) const { return {}; }
+  __device__ __forceinline__ void apply(long long i, const In&, const T* c, double*) const { out[i] = c[0]; }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, In{}, c, acc); }
 };
 
 template <typename T>
@@ -461,15 +486,21 @@ struct PostEval {
   __host__ __device__ int nio() const {
     return pre.nin() + (pre.fp.pseudo ? 1 : 0) + (V ? 1 : 0) + (aux ? (pre.fp.fV >= 0) + (pre.fp.fW >= 0) : 0);
   }
-  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
+  struct In {
+    typename PreEval<T>::In d;
+    T vl;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {pre.load(i), pre.fp.pseudo ? vloc[i] : T(0)}; }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
     const FnParams& fp = pre.fp;
     T ph = T(0), dph = T(0);
     T r;
     if (pre.rho) {
-      r = pre.rho[i];
+      r = in.d.a;
     } else {
-      const T p0 = pre.phi[i];
-      const T pp = pre.p ? pre.p[i] : T(0);
+      const T p0 = in.d.a;
+      const T pp = in.d.b;
       ph = pre.c * p0 + pre.s * pp;
       dph = pre.c * pp - pre.s * p0;
       r = ph * ph;
@@ -512,7 +543,7 @@ struct PostEval {
       v += vh;
     }
     if (fp.pseudo) {
-      const T vl = vloc[i];
+      const T vl = in.vl;
       acc[OFDFT_E_PP] += (double)(vl * r);
       v += vl;
     }
@@ -537,8 +568,15 @@ struct PostHvp {
   long long N;
   FnParams fp;
   __host__ __device__ int nio() const { return 4 + (fp.fV >= 0) + (fp.fW >= 0); }
-  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
-    const T ph = phi[i], pp = p[i];
+  struct In {
+    T ph, pp, v, a0, a1;
+  };
+  __device__ __forceinline__ In load(long long i) const {
+    return {phi[i], p[i], V[i], fp.fV >= 0 ? aux[i] : T(0), fp.fW >= 0 ? aux[N + i] : T(0)};
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const T ph = in.ph, pp = in.pp;
     const T r = ph * ph;
     const T dr = T(2) * ph * pp;
     const T cb = dev_cbrt(r);
@@ -549,13 +587,13 @@ struct PostHvp {
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
     if (fp.fV >= 0 && r > T(0)) {
       const T ds = dr / (T(2) * s);
-      dv += T(fp.y) * (T(0.5) * pick3(c, fp.fV) / s - aux[i] * ds / s);
+      dv += T(fp.y) * (T(0.5) * pick3(c, fp.fV) / s - in.a0 * ds / s);
     }
     if (fp.fW >= 0 && r >= T(kZero)) {
       const T rb1 = fp.beta56 ? cb / s : dev_pow(r, T(fp.beta) - T(1));
-      dv += T(2 * fp.alpha) * ((T(fp.beta) - T(1)) * rb1 / r * dr * aux[N + i] + rb1 * pick3(c, fp.fW));
+      dv += T(2 * fp.alpha) * ((T(fp.beta) - T(1)) * rb1 / r * dr * in.a1 + rb1 * pick3(c, fp.fW));
     }
-    const T ap = (V[i] - mu) * pp + ph * dv;
+    const T ap = (in.v - mu) * pp + ph * dv;
     Ap[i] = ap;
     acc[0] += (double)(pp * ap);
   }
@@ -574,8 +612,13 @@ struct PostHvpFast {
   T* __restrict__ Ap;
   FnParams fp;
   __host__ __device__ int nio() const { return 4; }
-  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const {
-    const T ph = phi[i], pp = p[i];
+  struct In {
+    T ph, pp, w0;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i], W0[i]}; }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const T ph = in.ph, pp = in.pp;
     const T r = ph * ph;
     T dv = T(0);
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
@@ -587,7 +630,7 @@ struct PostHvpFast {
         dv += T(2 * fp.alpha) * rb1 * pick3(c, fp.fW);
       }
     }
-    const T ap = W0[i] * pp + ph * dv;
+    const T ap = in.w0 * pp + ph * dv;
     Ap[i] = ap;
     acc[0] += (double)(pp * ap);
   }
@@ -630,6 +673,13 @@ __global__ void k_hvp_prepare(const T* __restrict__ phi, const T* __restrict__ V
 // Axis-2 real-to-complex: block handles PB row pairs x F fields.  Rows r and
 // r+1 of field f share one complex sequence z = a + i b; after the FFT,
 // A[k] = (Z[k] + conj Z[-k])/2, B[k] = (Z[k] - conj Z[-k])/(2i).
+// transform length of a compile-time FFT policy (1 for the run-time one)
+template <class FFT>
+constexpr int fixed_len() {
+  if constexpr (FFT::kFixed) return FFT::n();
+  else return 1;
+}
+
 template <typename T, class FFT, class Pre>
 __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
                                                       long long nrows, long long Nh, int nhp, int F,
@@ -638,21 +688,52 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  constexpr int PB = kRowPairsC;
+  constexpr int PB = kRowPB<T>;
   const int n2 = fft.n();
   const int nh = n2 / 2 + 1;
   const int th = fft.th();
   const long long pair0 = (long long)blockIdx.x * PB;
   const int ld = n2;
-  for (int t = threadIdx.x; t < PB * n2; t += th) {
-    const int lp = t / n2, j = t - lp * n2;
-    const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
-    T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-    if (r0 < nrows) pre(r0 * n2 + j, a);
-    if (r1 < nrows) pre(r1 * n2 + j, b);
+  if constexpr (FFT::kFixed) {
+    // every load of the thread's points is issued before the first use
+    constexpr int NN = FFT::n();
+    constexpr int IT = (PB * NN + FFT::TH - 1) / FFT::TH;
+    typename Pre::In xa[IT], xb[IT];
 #pragma unroll
-    for (int f = 0; f < 3; ++f)
-      if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
+    for (int u = 0; u < IT; ++u) {
+      const int t = threadIdx.x + u * FFT::TH;
+      const int lp = t / NN, j = t - lp * NN;
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      xa[u] = {};
+      xb[u] = {};
+      if (t < PB * NN && r0 < nrows) xa[u] = pre.load(r0 * NN + j);
+      if (t < PB * NN && r1 < nrows) xb[u] = pre.load(r1 * NN + j);
+    }
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int t = threadIdx.x + u * FFT::TH;
+      if (t < PB * NN) {
+        const int lp = t / NN, j = t - lp * NN;
+        const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+        T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+        if (r0 < nrows) pre.apply(xa[u], a);
+        if (r1 < nrows) pre.apply(xb[u], b);
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+          if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
+      }
+    }
+  } else {
+    for (int t = threadIdx.x; t < PB * n2; t += th) {
+      const int lp = t / n2, j = t - lp * n2;
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
+      if (r0 < nrows) pre(r0 * n2 + j, a);
+      if (r1 < nrows) pre(r1 * n2 + j, b);
+#pragma unroll
+      for (int f = 0; f < 3; ++f)
+        if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
+    }
   }
   cplx<T>* ltw = lds + PB * F * ld;
   for (int t = threadIdx.x; t < n2; t += th) ltw[t] = tw[t];
@@ -672,7 +753,9 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
 // Axis-2 complex-to-real fused with the post functor.  Two half spectra A, B
 // are combined into Z = A + iB on the full axis (Hermitian extension, DC and
 // Nyquist imaginary parts dropped as irfft does); after the inverse FFT the
-// real part is row r, the imaginary part row r+1.
+// real part is row r, the imaginary part row r+1.  For compile-time lengths
+// the post functor's own inputs are loaded first, so that their latency
+// overlaps the spectrum loads and the transform.
 template <typename T, class FFT, class Post>
 __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
                                                       long long nrows, long long Nh, int nhp, int F,
@@ -681,13 +764,28 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const 
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
-  constexpr int PB = kRowPairsC;
+  constexpr int PB = kRowPB<T>;
   const int n2 = fft.n();
   const int nh = n2 / 2 + 1;
   const int th = fft.th();
   const long long pair0 = (long long)blockIdx.x * PB;
   const int ld = n2;
   const bool even = (n2 % 2) == 0;
+  constexpr int NN = fixed_len<FFT>();
+  constexpr int IT = FFT::kFixed ? (PB * NN + FFT::TH - 1) / FFT::TH : 1;
+  typename Post::In xa[IT], xb[IT];
+  if constexpr (FFT::kFixed) {
+#pragma unroll
+    for (int u = 0; u < IT; ++u) {
+      const int t = threadIdx.x + u * FFT::TH;
+      const int lp = t / NN, j = t - lp * NN;
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      xa[u] = {};
+      xb[u] = {};
+      if (t < PB * NN && r0 < nrows) xa[u] = post.load(r0 * NN + j);
+      if (t < PB * NN && r1 < nrows) xb[u] = post.load(r1 * NN + j);
+    }
+  }
   for (int t = threadIdx.x; t < PB * F * nh; t += th) {
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
@@ -708,20 +806,42 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const 
   double acc[Post::NS];
 #pragma unroll
   for (int s = 0; s < Post::NS; ++s) acc[s] = 0.0;
-  for (int t = threadIdx.x; t < PB * n2; t += th) {
-    const int lp = t / n2, j = t - lp * n2;
-    const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
-    T ca[3] = {T(0), T(0), T(0)}, cb[3] = {T(0), T(0), T(0)};
+  if constexpr (FFT::kFixed) {
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      if (f < F) {
-        const cplx<T> z = lds[(lp * F + f) * ld + j];
-        ca[f] = z.x;
-        cb[f] = z.y;
+    for (int u = 0; u < IT; ++u) {
+      const int t = threadIdx.x + u * FFT::TH;
+      if (t < PB * NN) {
+        const int lp = t / NN, j = t - lp * NN;
+        const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+        T ca[3] = {T(0), T(0), T(0)}, cb[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          if (f < F) {
+            const cplx<T> z = lds[(lp * F + f) * ld + j];
+            ca[f] = z.x;
+            cb[f] = z.y;
+          }
+        }
+        if (r0 < nrows) post.apply(r0 * NN + j, xa[u], ca, acc);
+        if (r1 < nrows) post.apply(r1 * NN + j, xb[u], cb, acc);
       }
     }
-    if (r0 < nrows) post(r0 * n2 + j, ca, acc);
-    if (r1 < nrows) post(r1 * n2 + j, cb, acc);
+  } else {
+    for (int t = threadIdx.x; t < PB * n2; t += th) {
+      const int lp = t / n2, j = t - lp * n2;
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      T ca[3] = {T(0), T(0), T(0)}, cb[3] = {T(0), T(0), T(0)};
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        if (f < F) {
+          const cplx<T> z = lds[(lp * F + f) * ld + j];
+          ca[f] = z.x;
+          cb[f] = z.y;
+        }
+      }
+      if (r0 < nrows) post(r0 * n2 + j, ca, acc);
+      if (r1 < nrows) post(r1 * n2 + j, cb, acc);
+    }
   }
   if (partials) block_sums<Post::NS>(acc, partials + (long long)blockIdx.x * Post::NS);
 }
@@ -1212,7 +1332,6 @@ __global__ void k_convert(Td* __restrict__ dst, const Ts* __restrict__ src, long
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-constexpr int kRowPairs = kRowPairsC;
 constexpr int kColTile = kColTileC;
 
 int vec_blocks(long long N) {
@@ -1269,8 +1388,8 @@ struct RegTile {
 };
 constexpr int kRowsTH = 256;
 constexpr int kColsTH = 256;
-template <int N>
-using RowsFFT = FFTFixed<N, kRowPairsC * 3, kRowsTH>;
+template <typename T, int N>
+using RowsFFT = FFTFixed<N, kRowPB<T> * 3, kRowsTH>;
 template <int N>
 using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
@@ -1285,13 +1404,13 @@ template <typename T, class Pre>
 int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
-  const int PB = kRowPairsC;
+  const int PB = kRowPB<T>;
   const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   dim3 grid((unsigned)((npairs + PB - 1) / PB));
   ProfScope ps(P, Pre::kProf, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
   switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
-    case NN: go_rows_fwd<T>(P, pre, RowsFFT<NN>{}, F, grid, kRowsTH, lds); break;
+    case NN: go_rows_fwd<T>(P, pre, RowsFFT<T, NN>{}, F, grid, kRowsTH, lds); break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
     default: {
@@ -1316,7 +1435,7 @@ template <typename T, class Post>
 int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
-  const int PB = kRowPairsC;
+  const int PB = kRowPB<T>;
   const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   const long long nblk = (npairs + PB - 1) / PB;
   if (reduce && nblk * Post::NS > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
@@ -1326,7 +1445,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
     ProfScope ps(P, Post::kProf, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
     switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
-      case NN: go_rows_inv<T>(P, post, RowsFFT<NN>{}, F, src, grid, kRowsTH, lds, part); break;
+      case NN: go_rows_inv<T>(P, post, RowsFFT<T, NN>{}, F, src, grid, kRowsTH, lds, part); break;
       OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
       default: {
@@ -2218,7 +2337,7 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   }
   // LDS limits (64 KiB per workgroup)
   const size_t cs = 2 * P->rs;
-  if ((size_t)(kRowPairs * 3 + 1) * n2 * cs > 65536 || (size_t)(kColTile * (n0 + 1) + n0) * cs > 65536 ||
+  if ((size_t)((precision == OFDFT_F64 ? kRowPB<double> : kRowPB<float>) * 3 + 1) * n2 * cs > 65536 || (size_t)(kColTile * (n0 + 1) + n0) * cs > 65536 ||
       (size_t)(kColTile * (n1 + 1) + n1) * cs > 65536) {
     delete P;
     return fail(OFDFT_EINVAL, "grid dimension too large for the LDS tile sizes");
@@ -2272,7 +2391,7 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   if (hipMalloc(&P->spec, cs * 3 * P->Nhp) != hipSuccess) return cleanup(OFDFT_ENOMEM, "spectrum workspace");
   if (hipMemset(P->spec, 0, cs * 3 * P->Nhp) != hipSuccess) return cleanup(OFDFT_EHIP, "spectrum workspace clear");
   const long long npairs = ((long long)n0 * n1 + 1) / 2;
-  const long long nblk = (npairs + kRowPairs - 1) / kRowPairs;
+  const long long nblk = (npairs + kRowPBMin - 1) / kRowPBMin;
   P->npartials = std::max<long long>(nblk * OFDFT_NSUM, 2048LL * 3);
   if (hipMalloc((void**)&P->partials, sizeof(double) * P->npartials) != hipSuccess)
     return cleanup(OFDFT_ENOMEM, "partials");
