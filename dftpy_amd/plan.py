@@ -230,6 +230,13 @@ class Plan:
                                          ctypes.byref(out)))
         return out.value
 
+    def orth_sums(self, p, phi, V):
+        """(sum p phi, sum phi^2, sum p^2, sum V phi p, sum V phi phi) in one pass."""
+        out = L.darr(5)
+        self._stream()
+        L.check(self.lib.ofdft_orth_sums(self._h, self._real(p, "p"), self._real(phi, "phi"), self._real(V, "V"), out))
+        return tuple(out[:5])
+
     def convert_from(self, src, dst=None):
         """dst (this plan's precision) = src (any float precision), same size."""
         if dst is None:

@@ -237,6 +237,12 @@ int ofdft_residual(ofdft_plan plan, const void* V, const void* phi, double mu, v
 int ofdft_cg_update(ofdft_plan plan, double a, const void* p, const void* Ap, void* d, void* r, double* r2);
 /* sum(V*phi*p) (optimization.py:371) */
 int ofdft_dot_vphip(ofdft_plan plan, const void* V, const void* phi, const void* p, double* out);
+/* The sums of one outer step in one pass, FP64 accumulation:
+ * out5 = {sum p*phi, sum phi^2, sum p^2, sum V*phi*p, sum V*phi*phi}.
+ * OrthogonalNormalization (optimization.py:205-226) and the steepest-
+ * descent test / derivative at theta = 0 (:370-380, :270-277) follow from
+ * them for p' = s (p - f phi) without another pass over the fields.      */
+int ofdft_orth_sums(ofdft_plan plan, const void* p, const void* phi, const void* V, double* out5);
 /* precision conversion between plans: dst (dst precision) = src (src precision), N values */
 int ofdft_convert(ofdft_plan dst_plan, void* dst, int src_precision, const void* src, long long n);
 

@@ -270,3 +270,20 @@ def test_tn_direction_matches_host_loop(mu, maxfun, chunk):
     n2, st2, _ = P.tn_direction(phi, res, W0, d, work, maxfun, chunk)
     assert (n2, st2) == (n, st)
     assert np.abs(d.cpu().numpy() - dd).max() <= 1e-12 * np.abs(dd).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_orth_sums_match_numpy(prec):
+    """The five FP64-accumulated sums behind OrthogonalNormalization and the
+    theta = 0 derivative (optimization.py:205-226, :370-380)."""
+    nr = (20, 18, 15)
+    P = _plan(nr, np.eye(3) * 7.0, prec)
+    rng = np.random.default_rng(11)
+    p, phi, V = (rng.standard_normal(nr) for _ in range(3))
+    t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
+    got = P.orth_sums(t(p), t(phi), t(V))
+    # the sums of the values the kernel actually sees (rounded to the plan precision)
+    p, phi, V = (t(a).double().cpu().numpy() for a in (p, phi, V))
+    ref = (np.sum(p * phi), np.sum(phi * phi), np.sum(p * p), np.sum(V * phi * p), np.sum(V * phi * phi))
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= 1e-12 * max(abs(r), 1.0)
