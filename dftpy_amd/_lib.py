@@ -65,6 +65,7 @@ _SIGS = {
     "ofdft_hvp": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _vp, _dp]),
     "ofdft_hvp_prepare": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _vp, _dp]),
     "ofdft_hvp_fast": (_i, [_vp, _vp, _vp, _vp, _vp, _dp]),
+    "ofdft_potential_jvp": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "ofdft_tn_inner": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _dp]),
     "ofdft_tn_direction": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _dp]),
     "ofdft_fft_r2c": (_i, [_vp, _vp, _vp]),

@@ -567,6 +567,7 @@ struct PostHvp {
   T mu;
   long long N;
   FnParams fp;
+  int raw = 0;  // 1: store dV (the potential's response to drho = 2 phi p) instead of Ap
   __host__ __device__ int nio() const { return 4 + (fp.fV >= 0) + (fp.fW >= 0); }
   struct In {
     T ph, pp, v, a0, a1;
@@ -594,7 +595,7 @@ struct PostHvp {
       dv += T(2 * fp.alpha) * ((T(fp.beta) - T(1)) * rb1 / r * dr * in.a1 + rb1 * pick3(c, fp.fW));
     }
     const T ap = (in.v - mu) * pp + ph * dv;
-    Ap[i] = ap;
+    Ap[i] = raw ? dv : ap;
     acc[0] += (double)(pp * ap);
   }
 };
@@ -1635,7 +1636,8 @@ int eval_impl(ofdft_plan_s* P, const T* rho, const T* phi, const T* p, double c,
 }
 
 template <typename T>
-int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux, double mu, T* Ap, double* pAp) {
+int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux, double mu, T* Ap, double* pAp,
+             int raw = 0) {
   if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
   FnParams fp = make_fn(P->fn);
   const int F = fp.F;
@@ -1652,7 +1654,7 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
     if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
     if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
   }
-  PostHvp<T> post{phi, p, V, aux, Ap, T(mu), P->N, fp};
+  PostHvp<T> post{phi, p, V, aux, Ap, T(mu), P->N, fp, raw};
   if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true))) return rc;
   if (pAp) {
     double t;
@@ -2568,6 +2570,16 @@ int ofdft_hvp(ofdft_plan P, const void* phi, const void* p, const void* V, const
                             (double*)Ap, pAp);
   return hvp_impl<float>(P, (const float*)phi, (const float*)p, (const float*)V, (const float*)aux, mu, (float*)Ap,
                          pAp);
+}
+
+int ofdft_potential_jvp(ofdft_plan P, const void* phi, const void* p, const void* V, const void* aux, void* dV) {
+  if (!valid(P) || !phi || !p || !V || !dV) return fail(OFDFT_EINVAL, "null argument");
+  if ((P->fn.vw || P->fn.wt) && !aux) return fail(OFDFT_EINVAL, "aux fields required for vW/WT");
+  if (P->prec == OFDFT_F64)
+    return hvp_impl<double>(P, (const double*)phi, (const double*)p, (const double*)V, (const double*)aux, 0.0,
+                            (double*)dV, nullptr, 1);
+  return hvp_impl<float>(P, (const float*)phi, (const float*)p, (const float*)V, (const float*)aux, 0.0, (float*)dV,
+                         nullptr, 1);
 }
 
 int ofdft_hvp_prepare(ofdft_plan P, const void* phi, const void* V, const void* aux, double mu, void* res,

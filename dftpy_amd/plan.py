@@ -160,6 +160,14 @@ class Plan:
                                    float(mu), self._real(Ap, "Ap"), ctypes.byref(out)))
         return out.value
 
+    def potential_jvp(self, phi, p, V, aux, dV):
+        """dV = (d V / d rho)[2 phi p] at rho = phi^2."""
+        self._stream()
+        auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
+        L.check(self.lib.ofdft_potential_jvp(self._h, self._real(phi, "phi"), self._real(p, "p"), self._real(V, "V"),
+                                             auxp, self._real(dV, "dV")))
+        return dV
+
     def hvp_prepare(self, phi, V, aux, mu, res, W0):
         out = ctypes.c_double()
         self._stream()

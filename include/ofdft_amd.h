@@ -149,6 +149,15 @@ int ofdft_hvp(ofdft_plan plan, const void* phi, const void* p, const void* V, co
  * it equals ofdft_hvp up to rounding.                                    */
 int ofdft_hvp_prepare(ofdft_plan plan, const void* phi, const void* V, const void* aux, double mu, void* res,
                       void* W0, double* res2);
+
+/* Response of the total potential to a density change drho = 2 phi p at
+ * rho = phi^2 (the second functional derivative applied to drho, the same
+ * terms as ofdft_hvp): dV_out = dV/drho [drho].  With phi = sqrt(rho) and
+ * p = u / (2 phi) this is the Hessian of E(rho) applied to u, which the
+ * Python autograd layer uses as the backward of V(rho); it replaces the
+ * forward-difference product of get_direction_TN (optimization.py:106-114).
+ * The WT kernel is held at its cached rho0 (wt.py:64-72).               */
+int ofdft_potential_jvp(ofdft_plan plan, const void* phi, const void* p, const void* V, const void* aux, void* dV_out);
 int ofdft_hvp_fast(ofdft_plan plan, const void* phi, const void* p, const void* W0, void* Ap, double* pAp_out);
 
 /* One inner step of get_direction_TN (optimization.py:105-145) with the
