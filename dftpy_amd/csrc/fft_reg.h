@@ -86,7 +86,9 @@ struct Split;
 template <> struct Split<32> { static constexpr int N1 = 4, N2 = 8; };
 template <> struct Split<48> { static constexpr int N1 = 4, N2 = 12; };
 template <> struct Split<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct Split<90> { static constexpr int N1 = 9, N2 = 10; };
 template <> struct Split<96> { static constexpr int N1 = 8, N2 = 12; };
+template <> struct Split<108> { static constexpr int N1 = 9, N2 = 12; };
 template <> struct Split<128> { static constexpr int N1 = 8, N2 = 16; };
 template <> struct Split<160> { static constexpr int N1 = 10, N2 = 16; };
 template <> struct Split<192> { static constexpr int N1 = 12, N2 = 16; };

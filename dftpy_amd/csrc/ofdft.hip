@@ -1404,7 +1404,7 @@ GridGeom geom(const ofdft_plan_s* P) {
 }
 
 // Compile-time FFT lengths; any other 2^a 3^b 5^c length takes the runtime path.
-#define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(160) X(192)
+#define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(90) X(96) X(108) X(128) X(160) X(192)
 
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // columns per workgroup of the register column kernels

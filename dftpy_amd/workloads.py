@@ -70,6 +70,46 @@ def initial_density(nr, nelec, volume, noise=0.01, seed=0):
     return rho
 
 
+def even_fft_size(n):
+    """Smallest even 2^a 3^b 5^c >= n."""
+    m = max(2, int(np.ceil(n)))
+    while True:
+        if m % 2 == 0:
+            k = m
+            for f in (2, 3, 5):
+                while k % f == 0:
+                    k //= f
+            if k == 1:
+                return m
+        m += 1
+
+
+def batch_cells(n=64, seed=11, spacing=0.2, sigma=0.05, mg_max=0.3):
+    """Seeded batch of independent cubic fcc Al/Mg cells: atom counts drawn
+    log-uniform in [32, 500] and snapped (in log space) to 4 m^3, m = 2..5;
+    Mg fraction U[0, mg_max]; grid = smallest even 2^a 3^b 5^c >= L/spacing
+    (spacing in Angstrom: 48, 64, 90, 108 points for m = 2..5 at 0.2 A);
+    displacements sigma (Angstrom).  Cell i uses seeds seed*1000 + i."""
+    rng = np.random.default_rng(seed)
+    ms = np.array([2, 3, 4, 5])
+    out = []
+    for i in range(n):
+        target = np.exp(rng.uniform(np.log(32.0), np.log(500.0)))
+        m = int(ms[np.argmin(np.abs(np.log(4.0 * ms ** 3) - np.log(target)))])
+        mg = float(rng.uniform(0.0, mg_max))
+        symbols, pos, cell = fcc_supercell(m, sigma=sigma, mg_fraction=mg, seed=seed * 1000 + i)
+        ng = even_fft_size(m * A_AL / spacing)
+        nr = (ng,) * 3
+        tables = {s: empty_core_table(*EMPTY_CORE[s]) for s in sorted(set(symbols))}
+        zval = {s: EMPTY_CORE[s][0] for s in tables}
+        nelec = sum(zval[s] for s in symbols)
+        volume = abs(np.linalg.det(cell))
+        out.append(dict(name="batch%02d" % i, symbols=symbols, positions=pos, cell=cell, nr=nr, kedf="WT",
+                        kedf_options=dict(x=1.0, y=1.0), tables=tables, zval=zval, nelec=nelec,
+                        rho_ini=initial_density(nr, nelec, volume, seed=seed * 1000 + i)))
+    return out
+
+
 def build(name):
     """Host description of a workload: dict with symbols, positions, cell,
     nr, kedf options, PP tables, zval, nelec, rho_ini."""

@@ -16,7 +16,8 @@ from tests.conftest import DATA
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(32, 32, 32), (20, 30, 20), (36, 36, 72), (48, 40, 30), (96, 96, 96), (24, 50, 18), (15, 9, 20), (20, 18, 15), (160, 24, 192)]
+SHAPES = [(32, 32, 32), (20, 30, 20), (36, 36, 72), (48, 40, 30), (96, 96, 96), (24, 50, 18), (15, 9, 20), (20, 18, 15),
+          (160, 24, 192), (108, 90, 48), (90, 64, 108)]
 
 
 def _plan(nr, lattice, prec="fp64", full=False):
