@@ -82,6 +82,7 @@ _SIGS = {
     "ofdft_vloc_build": (_i, [_vp, _i, ctypes.POINTER(VLine), _i, _dp, ctypes.POINTER(ctypes.c_int), _i, _i, _vp]),
     "ofdft_local_forces": (_i, [_vp, _vp, _dp]),
     "ofdft_ewald": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
+    "ofdft_ewald_forces": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp]),
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
     "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),

@@ -2040,7 +2040,8 @@ __global__ void k_force_pme_mult(cplx<double>* __restrict__ out, const cplx<doub
   }
 }
 
-// PME force gather for the atoms of species sel, one workgroup per atom:
+// PME force gather for the atoms of species sel (every atom if sel < 0), one
+// workgroup per atom:
 // F_i = -sum_stencil (dQ^T inv(L)) rhoPB[l] (pseudo/__init__.py:546-570).
 __global__ void k_force_pme_gather(const double* __restrict__ rhopb, const double* __restrict__ frac,
                                    const int* __restrict__ species, int sel, int n0, int n1, int n2, int order,
@@ -2049,7 +2050,7 @@ __global__ void k_force_pme_gather(const double* __restrict__ rhopb, const doubl
   __shared__ double M2[3][kMaxOrder + 1];
   __shared__ int base[3];
   const int i = blockIdx.x;
-  if (species[i] != sel) return;
+  if (sel >= 0 && species[i] != sel) return;  // sel < 0: every atom
   if (threadIdx.x < 3) {
     const int d = threadIdx.x;
     const int n = d == 0 ? n0 : (d == 1 ? n1 : n2);
@@ -2097,6 +2098,103 @@ __global__ void k_ewald_real(const double* __restrict__ pos, const double* __res
     acc[0] += q[i] * q[j] * s;
   }
   block_sums<1>(acc, partials + blockIdx.x);
+}
+
+// Ewald real-space forces (ewald.py:541-578), one workgroup per atom i:
+// sum over atoms j and image shifts T of q_j (R_i - R_j + T) w(x),
+// w = erfc(x)/x^3 + 2/sqrt(pi) exp(-x^2)/x^2, x = sqrt(eta) |R_i - R_j + T|;
+// the caller multiplies by q_i eta^(3/2).
+__global__ void k_ewald_force_real(const double* __restrict__ pos, const double* __restrict__ q, int nat,
+                                   const double* __restrict__ shifts, int nshift, double rmax, double eta_sqrt,
+                                   double* __restrict__ partials) {
+  const int i = blockIdx.x;
+  const double Ri0 = pos[3 * i], Ri1 = pos[3 * i + 1], Ri2 = pos[3 * i + 2];
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long long n = (long long)nat * nshift;
+  for (long long t = threadIdx.x; t < n; t += blockDim.x) {
+    const int j = (int)(t % nat), m = (int)(t / nat);
+    const double dx = Ri0 - pos[3 * j] + shifts[3 * m];
+    const double dy = Ri1 - pos[3 * j + 1] + shifts[3 * m + 1];
+    const double dz = Ri2 - pos[3 * j + 2] + shifts[3 * m + 2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    if (d < rmax && d > 0.001) {
+      const double x = eta_sqrt * d;
+      const double w = q[j] * (erfc(x) / (x * x * x) + 1.1283791670955125739 * exp(-x * x) / (x * x));
+      acc[0] += dx * w;
+      acc[1] += dy * w;
+      acc[2] += dz * w;
+    }
+  }
+  block_sums<3>(acc, partials + 3 * i);
+}
+
+// Exact ionic structure factor S(G) = sum_j q_j exp(-i G.R_j) on the half
+// spectrum (ions.py:121-124 summed with the charges, ewald.py:588-591).
+__global__ void k_ewald_strf(cplx<double>* __restrict__ S, GridGeom geo, const double* __restrict__ pos,
+                             const double* __restrict__ q, int nat) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    cplx<double> z = {0.0, 0.0};
+    for (int j = 0; j < nat; ++j) {
+      double sn, cs;
+      sincos(G[0] * pos[3 * j] + G[1] * pos[3 * j + 1] + G[2] * pos[3 * j + 2], &sn, &cs);
+      z.x += q[j] * cs;
+      z.y -= q[j] * sn;
+    }
+    S[r * geo.nhp + c] = z;
+  }
+}
+
+// Exact reciprocal Ewald forces (ewald.py:580-607), one workgroup per atom:
+// sum over the mask of G (cos(G.R_i) Im S + sin(G.R_i) Re S) exp(-G^2/(4 eta))/G^2;
+// the caller multiplies by q_i 8 pi / Omega.
+__global__ void k_ewald_force_rec(const cplx<double>* __restrict__ S, GridGeom geo, const double* __restrict__ pos,
+                                  double eta, double* __restrict__ partials) {
+  const int i = blockIdx.x;
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const double R0 = pos[3 * i], R1 = pos[3 * i + 1], R2 = pos[3 * i + 2];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long long idx = threadIdx.x; idx < Nh; idx += blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    if (!in_mask(geo, k0, k1, c)) continue;
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    const double g2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    if (!(g2 > 0.0)) continue;
+    double sn, cs;
+    sincos(G[0] * R0 + G[1] * R1 + G[2] * R2, &sn, &cs);
+    const cplx<double> z = S[r * geo.nhp + c];
+    const double v = (cs * z.y + sn * z.x) * exp(-g2 / (4.0 * eta)) / g2;
+    acc[0] += G[0] * v;
+    acc[1] += G[1] * v;
+    acc[2] += G[2] * v;
+  }
+  block_sums<3>(acc, partials + 3 * i);
+}
+
+// PME Ewald force multiplier (ewald.py:810-829): spec *= |B|^2 exp(-G^2/(4 eta))/G^2, G = 0 -> 0.
+__global__ void k_ewald_pme_mult(cplx<double>* __restrict__ spec, GridGeom geo, const double* __restrict__ bm,
+                                 double eta) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const double g2 = gsq(geo, k0, k1, c);
+    const cplx<double> b = bmod(bm, geo.n0, geo.n1, k0, k1, c);
+    const double m = g2 > 0.0 ? (b.x * b.x + b.y * b.y) * exp(-g2 / (4.0 * eta)) / g2 : 0.0;
+    const long long ph = r * geo.nhp + c;
+    spec[ph] = cscale(spec[ph], m);
+  }
 }
 
 // Ewald reciprocal sum over the half-space mask: sum |S(G)|^2 exp(-G^2/(4 eta))/G^2
@@ -2326,6 +2424,76 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   *status = h->status;
   *pAp_last = h->pAp;
   return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// Device copies of the ion data an Ewald sum needs, and the image shifts
+// T = (ix, iy, iz) . L with |i| <= ceil(rmax / |a_i|) (ewald.py:287-300).
+struct EwaldDev {
+  double *pos = nullptr, *q = nullptr, *sh = nullptr, *frac = nullptr;
+  int nshift = 0;
+  ~EwaldDev() {
+    if (pos) (void)hipFree(pos);
+    if (q) (void)hipFree(q);
+    if (sh) (void)hipFree(sh);
+    if (frac) (void)hipFree(frac);
+  }
+};
+
+int ewald_upload(ofdft_plan_s* P, int nat, const double* positions, const double* charges, double rmax,
+                 EwaldDev& E) {
+  int Nimg[3];
+  for (int d = 0; d < 3; ++d) {
+    const double len = std::sqrt(P->lat[3 * d] * P->lat[3 * d] + P->lat[3 * d + 1] * P->lat[3 * d + 1] +
+                                 P->lat[3 * d + 2] * P->lat[3 * d + 2]);
+    Nimg[d] = (int)std::ceil(rmax / len);
+  }
+  std::vector<double> sh;
+  for (int a = -Nimg[0]; a <= Nimg[0]; ++a)
+    for (int b = -Nimg[1]; b <= Nimg[1]; ++b)
+      for (int c = -Nimg[2]; c <= Nimg[2]; ++c)
+        for (int d = 0; d < 3; ++d) sh.push_back(a * P->lat[d] + b * P->lat[3 + d] + c * P->lat[6 + d]);
+  E.nshift = (int)(sh.size() / 3);
+  if (hipMalloc((void**)&E.pos, sizeof(double) * 3 * nat) != hipSuccess ||
+      hipMalloc((void**)&E.q, sizeof(double) * nat) != hipSuccess ||
+      hipMalloc((void**)&E.sh, sizeof(double) * sh.size()) != hipSuccess ||
+      hipMalloc((void**)&E.frac, sizeof(double) * 3 * nat) != hipSuccess)
+    return fail(OFDFT_ENOMEM, "Ewald buffers");
+  std::vector<double> frac(3 * (size_t)nat);
+  for (int i = 0; i < nat; ++i)
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int e = 0; e < 3; ++e) f += positions[3 * i + e] * P->bg[d * 3 + e] / (2.0 * kPi);
+      frac[3 * i + d] = pymod1(pymod1(f));
+    }
+  if (hipMemcpy(E.pos, positions, sizeof(double) * 3 * nat, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(E.q, charges, sizeof(double) * nat, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(E.sh, sh.data(), sizeof(double) * sh.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(E.frac, frac.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(OFDFT_EHIP, "Ewald upload");
+  return 0;
+}
+
+int ewald_check(ofdft_plan_s* P, int nat, double eta, double rmax, int pme, int order) {
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "Ewald sums run on an FP64 plan");
+  if (nat < 1 || !(eta > 0) || !(rmax > 0)) return fail(OFDFT_EINVAL, "bad Ewald parameters");
+  if (pme && (order < 3 || order > kMaxOrder)) return fail(OFDFT_EINVAL, "B-spline order must be in [3, 12]");
+  return 0;
+}
+
+// The charge Q-array of the ions, spread with B-splines of the given order,
+// transformed into spectrum slot 0 (ewald.py:137-194, :786-790).
+int ewald_qhat(ofdft_plan_s* P, const EwaldDev& E, int nat, int order) {
+  int rc;
+  if ((rc = ensure_work(P)) || (rc = ensure_bm(P, order))) return rc;
+  if (hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream) != hipSuccess) return fail(OFDFT_EHIP, "memset");
+  hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(1024), 0, P->stream, P->d_work, E.frac, (const int*)nullptr, -1,
+                     E.q, P->n[0], P->n[1], P->n[2], order);
+  if (hipGetLastError() != hipSuccess) return fail(OFDFT_EHIP, "spread launch");
+  return r2c_slot(P, P->d_work, 0);
 }
 
 }  // namespace
@@ -2793,78 +2961,81 @@ int ofdft_local_forces(ofdft_plan P, const void* rho, double* forces_out) {
 int ofdft_ewald(ofdft_plan P, int nat, const double* positions, const double* charges, double eta, double rmax,
                 int pme, int order, double* e_real, double* e_rec) {
   if (!valid(P) || !positions || !charges || !e_real || !e_rec) return fail(OFDFT_EINVAL, "null argument");
-  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "Ewald sums run on an FP64 plan");
-  if (nat < 1 || !(eta > 0) || !(rmax > 0)) return fail(OFDFT_EINVAL, "bad Ewald parameters");
-  if (pme && (order < 3 || order > kMaxOrder)) return fail(OFDFT_EINVAL, "B-spline order must be in [3, 12]");
   int rc;
-  // image shifts T = (ix, iy, iz) . L with |i| <= ceil(rmax / |a_i|)
-  int Nimg[3];
-  for (int d = 0; d < 3; ++d) {
-    const double len = std::sqrt(P->lat[3 * d] * P->lat[3 * d] + P->lat[3 * d + 1] * P->lat[3 * d + 1] +
-                                 P->lat[3 * d + 2] * P->lat[3 * d + 2]);
-    Nimg[d] = (int)std::ceil(rmax / len);
-  }
-  std::vector<double> sh;
-  for (int a = -Nimg[0]; a <= Nimg[0]; ++a)
-    for (int b = -Nimg[1]; b <= Nimg[1]; ++b)
-      for (int c = -Nimg[2]; c <= Nimg[2]; ++c)
-        for (int d = 0; d < 3; ++d) sh.push_back(a * P->lat[d] + b * P->lat[3 + d] + c * P->lat[6 + d]);
-  const int nshift = (int)(sh.size() / 3);
-  double *d_pos = nullptr, *d_q = nullptr, *d_sh = nullptr, *d_frac = nullptr;
-  auto release = [&]() {
-    if (d_pos) (void)hipFree(d_pos);
-    if (d_q) (void)hipFree(d_q);
-    if (d_sh) (void)hipFree(d_sh);
-    if (d_frac) (void)hipFree(d_frac);
-  };
-  if (hipMalloc((void**)&d_pos, sizeof(double) * 3 * nat) != hipSuccess ||
-      hipMalloc((void**)&d_q, sizeof(double) * nat) != hipSuccess ||
-      hipMalloc((void**)&d_sh, sizeof(double) * sh.size()) != hipSuccess ||
-      hipMalloc((void**)&d_frac, sizeof(double) * 3 * nat) != hipSuccess) {
-    release();
-    return fail(OFDFT_ENOMEM, "Ewald buffers");
-  }
-  std::vector<double> frac(3 * (size_t)nat);
-  for (int i = 0; i < nat; ++i)
-    for (int d = 0; d < 3; ++d) {
-      double f = 0.0;
-      for (int e = 0; e < 3; ++e) f += positions[3 * i + e] * P->bg[d * 3 + e] / (2.0 * kPi);
-      frac[3 * i + d] = pymod1(pymod1(f));
-    }
-  hipError_t e1 = hipMemcpy(d_pos, positions, sizeof(double) * 3 * nat, hipMemcpyHostToDevice);
-  hipError_t e2 = hipMemcpy(d_q, charges, sizeof(double) * nat, hipMemcpyHostToDevice);
-  hipError_t e3 = hipMemcpy(d_sh, sh.data(), sizeof(double) * sh.size(), hipMemcpyHostToDevice);
-  hipError_t e4 = hipMemcpy(d_frac, frac.data(), sizeof(double) * 3 * nat, hipMemcpyHostToDevice);
-  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
-    release();
-    return fail(OFDFT_EHIP, "Ewald upload");
-  }
+  if ((rc = ewald_check(P, nat, eta, rmax, pme, order))) return rc;
+  EwaldDev E;
+  if ((rc = ewald_upload(P, nat, positions, charges, rmax, E))) return rc;
   const int nb = vec_blocks((long long)nat * nat);
-  hipLaunchKernelGGL(k_ewald_real, dim3(nb), dim3(256), 0, P->stream, d_pos, d_q, nat, d_sh, nshift, rmax,
+  hipLaunchKernelGGL(k_ewald_real, dim3(nb), dim3(256), 0, P->stream, E.pos, E.q, nat, E.sh, E.nshift, rmax,
                      std::sqrt(eta), P->partials);
-  if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "Ewald real launch"); }
+  if (hipGetLastError() != hipSuccess) return fail(OFDFT_EHIP, "Ewald real launch");
   double er;
-  if ((rc = reduce_to_host(P, nb, 1, &er))) { release(); return rc; }
+  if ((rc = reduce_to_host(P, nb, 1, &er))) return rc;
   *e_real = er / 2.0;
   GridGeom g = geom(P);
   const cplx<double>* qhat = nullptr;
   if (pme) {
-    if ((rc = ensure_work(P)) || (rc = ensure_bm(P, order))) { release(); return rc; }
-    if (hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream) != hipSuccess) { release(); return fail(OFDFT_EHIP, "memset"); }
-    hipLaunchKernelGGL(k_pme_spread, dim3(nat), dim3(1024), 0, P->stream, P->d_work, d_frac, (const int*)nullptr, -1,
-                       d_q, P->n[0], P->n[1], P->n[2], order);
-    if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "spread launch"); }
-    if ((rc = r2c_slot(P, P->d_work, 0))) { release(); return rc; }
+    if ((rc = ewald_qhat(P, E, nat, order))) return rc;
     qhat = reinterpret_cast<const cplx<double>*>(P->spec);
   }
   const int nb2 = vec_blocks(P->Nh);
-  hipLaunchKernelGGL(k_ewald_rec, dim3(nb2), dim3(256), 0, P->stream, qhat, P->d_bm, g, d_pos, d_q, nat, eta,
+  hipLaunchKernelGGL(k_ewald_rec, dim3(nb2), dim3(256), 0, P->stream, qhat, P->d_bm, g, E.pos, E.q, nat, eta,
                      P->partials);
-  if (hipGetLastError() != hipSuccess) { release(); return fail(OFDFT_EHIP, "Ewald rec launch"); }
+  if (hipGetLastError() != hipSuccess) return fail(OFDFT_EHIP, "Ewald rec launch");
   double ek;
-  if ((rc = reduce_to_host(P, nb2, 1, &ek))) { release(); return rc; }
+  if ((rc = reduce_to_host(P, nb2, 1, &ek))) return rc;
   *e_rec = 4.0 * kPi * ek / P->volume;
-  release();
+  return 0;
+}
+
+int ofdft_ewald_forces(ofdft_plan P, int nat, const double* positions, const double* charges, double eta,
+                       double rmax, int pme, int order, double* forces_out) {
+  if (!valid(P) || !positions || !charges || !forces_out) return fail(OFDFT_EINVAL, "null argument");
+  int rc;
+  if ((rc = ewald_check(P, nat, eta, rmax, pme, order))) return rc;
+  if ((long long)3 * nat > P->npartials) return fail(OFDFT_EINVAL, "too many atoms for the partials buffer");
+  EwaldDev E;
+  if ((rc = ewald_upload(P, nat, positions, charges, rmax, E))) return rc;
+  std::vector<double> h(3 * (size_t)nat);
+  // real space
+  hipLaunchKernelGGL(k_ewald_force_real, dim3(nat), dim3(256), 0, P->stream, E.pos, E.q, nat, E.sh, E.nshift, rmax,
+                     std::sqrt(eta), P->partials);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(hipStreamSynchronize(P->stream));
+  const double es3 = eta * std::sqrt(eta);
+  for (int i = 0; i < nat; ++i)
+    for (int d = 0; d < 3; ++d) forces_out[3 * i + d] = h[3 * i + d] * charges[i] * es3;
+  // reciprocal space
+  GridGeom g = geom(P);
+  auto* spec0 = reinterpret_cast<cplx<double>*>(P->spec);
+  if (!pme) {
+    hipLaunchKernelGGL(k_ewald_strf, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec0, g, E.pos, E.q, nat);
+    LAUNCHCHK();
+    hipLaunchKernelGGL(k_ewald_force_rec, dim3(nat), dim3(256), 0, P->stream, spec0, g, E.pos, eta, P->partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (int i = 0; i < nat; ++i)
+      for (int d = 0; d < 3; ++d) forces_out[3 * i + d] += h[3 * i + d] * charges[i] * 8.0 * kPi / P->volume;
+  } else {
+    if ((rc = ewald_qhat(P, E, nat, order))) return rc;
+    hipLaunchKernelGGL(k_ewald_pme_mult, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec0, g, P->d_bm, eta);
+    LAUNCHCHK();
+    if ((rc = c2r_slot0(P, P->d_work))) return rc;
+    double linv[9];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) linv[a * 3 + b] = P->bg[b * 3 + a] / (2.0 * kPi);  // inv(L)
+    double* d_linv = P->red;  // 9 doubles of scratch (kMaxSum >= 9)
+    HIPCHK(hipMemcpyAsync(d_linv, linv, sizeof(linv), hipMemcpyHostToDevice, P->stream));
+    hipLaunchKernelGGL(k_force_pme_gather, dim3(nat), dim3(1024), 0, P->stream, P->d_work, E.frac, (const int*)nullptr,
+                       -1, P->n[0], P->n[1], P->n[2], order, d_linv, P->partials);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    for (int i = 0; i < nat; ++i)
+      for (int d = 0; d < 3; ++d) forces_out[3 * i + d] += h[3 * i + d] * charges[i] * 4.0 * kPi / P->dV;
+  }
   return 0;
 }
 

@@ -427,20 +427,29 @@ class TotalFunctional(AbstractFunctional):
         return out
 
     def get_forces(self, rho, ions=None, split=False, **kwargs):
-        """Local-PP forces plus Ewald forces when available
-        (total_functional.py:148-174); Ha/bohr, shape (nat, 3)."""
-        pseudo = None
-        for f in self.funcDict.values():
-            if f.type == "PSEUDO":
-                pseudo = f
-        if pseudo is None:
-            raise ValueError("Please provide a PSEUDO functional for forces")
-        forces = {"PSEUDO": pseudo.forces(rho)}
-        total = forces["PSEUDO"].copy()
-        if split:
-            forces["TOTAL"] = total
-            return forces
-        return total
+        """Forces of every term that has them plus the ion-ion (Ewald)
+        forces (total_functional.py:148-174); Ha/bohr, shape (nat, 3).  On
+        this path the local PP is the only density term with forces."""
+        ew = self.ewald
+        if ions is None:
+            if ew:
+                ions = ew.ions
+            else:
+                raise ValueError("Please provide ions for forces")
+        forces = {"TOTAL": np.zeros((ions.nat, 3))}
+        for key, f in self.funcDict.items():
+            if hasattr(f, "forces"):
+                fr = f.forces(rho)
+                if fr is not None:
+                    if split:
+                        forces[key] = fr
+                    forces["TOTAL"] = forces["TOTAL"] + fr
+        if ew:
+            fr = ew.forces
+            if split:
+                forces["II"] = fr
+            forces["TOTAL"] = forces["TOTAL"] + fr
+        return forces if split else forces["TOTAL"]
 
 
 def Functional(type=None, optional_kwargs=None, **kwargs):

@@ -232,6 +232,7 @@ class Ewald:
         gmax = math.sqrt(grid.get_reciprocal().g2max)
         self.eta = self.Get_Best_eta(precision, gmax, ions)
         self._energy = None
+        self._forces = None
 
     def Get_Best_eta(self, precision, gmax, ions):
         q2 = float(np.sum(ions.charges * ions.charges))
@@ -247,12 +248,15 @@ class Ewald:
         gz = -4.0 * np.pi * (1.0 / (4.0 * self.eta * self.grid.volume) / 2.0) * self.ions.get_ncharges() ** 2
         return dc + gz
 
-    def _sums(self):
+    def _fp64_plan(self):
         if self._plan is None:
             from .plan import Plan
             g = self.grid
             self._plan = Plan(tuple(int(x) for x in g.nrR), g.lattice, "fp64", full=g.full, device=g.get_device())
-        P = self._plan
+        return self._plan
+
+    def _sums(self):
+        P = self._fp64_plan()
         rmax = special.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
         pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
         q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
@@ -269,3 +273,21 @@ class Ewald:
             er, ek = self._sums()
             self._energy = self.Energy_corr() + er + ek
         return self._energy
+
+    @property
+    def forces(self):
+        """Ion-ion forces (ewald.py:509-519): Forces_real plus Forces_rec_PME
+        or Forces_rec, Ha/bohr, shape (nat, 3)."""
+        if self._forces is None:
+            P = self._fp64_plan()
+            rmax = special.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+            pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
+            q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
+            out = np.zeros((self.ions.nat, 3))
+            P._stream()
+            dp = ctypes.POINTER(ctypes.c_double)
+            L.check(P.lib.ofdft_ewald_forces(P._h, self.ions.nat, pos.ctypes.data_as(dp), q.ctypes.data_as(dp),
+                                             float(self.eta), float(rmax), 1 if self.PME else 0, int(self.order),
+                                             out.ctypes.data_as(dp)))
+            self._forces = out
+        return self._forces

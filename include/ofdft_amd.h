@@ -229,7 +229,12 @@ int ofdft_local_forces(ofdft_plan plan, const void* rho, double* forces_out);
  * mask with the PME structure factor (pme=1, Energy_rec_PME :785-807) or
  * the exact one (pme=0, Energy_rec :448-465).  e_real already halved;
  * e_rec includes 4 pi/Omega.  The G=0 / self terms (Energy_corr) are host
- * arithmetic on the charges.                                             */
+ * arithmetic on the charges.
+ * ofdft_ewald_forces: the ion-ion forces (Ha/bohr, nat x 3) of the same
+ * sums: Forces_real (:541-578) plus Forces_rec_PME (:810-864, pme=1) or
+ * Forces_rec (:580-607, pme=0); `forces` property :509-519.               */
+int ofdft_ewald_forces(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta,
+                       double rmax, int pme, int order, double* forces_out);
 int ofdft_ewald(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta, double rmax,
                 int pme, int order, double* e_real, double* e_rec);
 

@@ -29,6 +29,9 @@ optimize_rho (TN)           notebook density_optimization.ipynb:174-183 log
                             (E0 -8.090977710718, final -8.2811389962163 Ha,
                             mu 0.30115196532053357); test_density_opt.py:46
                             (GaAs -43.3046 Ha)
+Ewald forces (real, rec,    FD of the Ewald energy on a displaced cell; zero
+rec PME)                    forces of the perfect fcc fixture and exact vs PME
+                            agreement (examples/test/test_ewald.py:37-50)
 optimize_rho (LBFGS)        converges to the TN energy (parity unpinned
                             beyond that: no reference test runs LBFGS)
 ==========================  ==================================================
@@ -743,6 +746,80 @@ class Ewald:
             rec = self.energy_rec_pme() if self.PME else self.energy_rec()
             self._energy = self.energy_corr() + self.energy_real() + rec
         return self._energy
+
+    def forces_real(self):
+        """ewald.py:541-578: F_i = q_i eta^(3/2) sum_(j,T) q_j (R_i - R_j + T)
+        [erfc(x)/x^3 + 2/sqrt(pi) exp(-x^2)/x^2], x = sqrt(eta) |R_i - R_j + T|,
+        over the images within rmax (pairs closer than 0.001 skipped)."""
+        lat = self.grid.lattice
+        L = np.sqrt((lat ** 2).sum(axis=1))
+        rmax = sp.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+        N = np.ceil(rmax / L).astype(int)
+        T = np.mgrid[-N[0]:N[0] + 1, -N[1]:N[1] + 1, -N[2]:N[2] + 1].reshape(3, -1).T.astype(np.float64)
+        shifts = T @ lat
+        pos = self.ions.positions
+        q = self.ions.charges
+        es = np.sqrt(self.eta)
+        img = (pos[None, :, :] - shifts[:, None, :]).reshape(-1, 3)
+        qq = np.tile(q, len(shifts))
+        F = np.zeros((self.ions.nat, 3))
+        for i in range(self.ions.nat):
+            dvec = pos[i] - img
+            d = np.sqrt((dvec ** 2).sum(axis=1))
+            m = (d < rmax) & (d > 0.001)
+            x = es * d[m]
+            w = qq[m] * (sp.erfc(x) / x ** 3 + 2.0 / np.sqrt(np.pi) * np.exp(-x ** 2) / x ** 2)
+            F[i] = q[i] * (dvec[m] * w[:, None]).sum(axis=0)
+        return F * es ** 3
+
+    def forces_rec(self):
+        """ewald.py:580-607 (exact structure factor)."""
+        g = self.grid
+        q = self.ions.charges
+        S = np.zeros(tuple(g.nrG), dtype=np.complex128)
+        for i in range(self.ions.nat):
+            S += np.exp(-1j * np.einsum("lijk,l->ijk", g.g, self.ions.positions[i])) * q[i]
+        m = g.mask
+        kern = np.exp(-g.gg[m] / (4.0 * self.eta)) * g.invgg[m]
+        gm = g.g[:, m]
+        F = np.empty((self.ions.nat, 3))
+        for i in range(self.ions.nat):
+            Si = np.exp(-1j * np.einsum("lijk,l->ijk", g.g, self.ions.positions[i]))[m] * q[i]
+            F[i] = gm @ ((Si.real * S[m].imag - Si.imag * S[m].real) * kern)
+        return F * 8.0 * np.pi / g.volume
+
+    def forces_rec_pme(self):
+        """ewald.py:810-864: the charge Q-array through |B|^2 exp(-G^2/(4 eta))/G^2,
+        gathered with the B-spline derivatives."""
+        g = self.grid
+        o = self.order
+        bs = CBspline(g, o)
+        sc = self.ions.get_scaled_positions()
+        Q = bs.spread(sc, self.ions.charges)
+        B = bs.Barray
+        S = fft(Q, g) * (B * np.conjugate(B)) * np.exp(-g.gg / (4.0 * self.eta)) * g.invgg
+        rpb = ifft(S, g)
+        nr = g.nr
+        cell_inv = np.linalg.inv(self.ions.cell)
+        ix = np.mgrid[:o, :o, :o].reshape((3, -1))
+        F = np.zeros((self.ions.nat, 3))
+        for i in range(self.ions.nat):
+            Up = sc[i] * nr
+            fr = Up - np.floor(Up)
+            Mn = [cbspline_Mn(fr[j], o) for j in range(3)]
+            M2 = [_cbspline_Mn_partial(fr[j], o, o - 1) for j in range(3)]
+            dQ = np.zeros((3, o ** 3))
+            dQ[0] = nr[0] * np.einsum("i,j,k->ijk", M2[0][1:] - M2[0][:-1], Mn[1][1:], Mn[2][1:]).ravel()
+            dQ[1] = nr[1] * np.einsum("i,j,k->ijk", Mn[0][1:], M2[1][1:] - M2[1][:-1], Mn[2][1:]).ravel()
+            dQ[2] = nr[2] * np.einsum("i,j,k->ijk", Mn[0][1:], Mn[1][1:], M2[2][1:] - M2[2][:-1]).ravel()
+            l = np.mod(1 + np.floor(Up).astype(np.int32).reshape((3, 1)) - ix, nr.reshape((3, 1)))
+            F[i] = -np.sum(np.matmul(dQ.T, cell_inv) * rpb[l[0], l[1], l[2]][:, None], axis=0) * self.ions.charges[i]
+        return F * 4.0 * np.pi / g.dV
+
+    @property
+    def forces(self):
+        """ewald.py:509-519."""
+        return self.forces_real() + (self.forces_rec_pme() if self.PME else self.forces_rec())
 
 
 # ---------------------------------------------------------------------------

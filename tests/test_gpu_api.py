@@ -115,3 +115,51 @@ def test_field_fft_roundtrip_and_conventions():
         assert float((g - f).abs().max()) < 1e-12
         ref = np.fft.fftn(f.asnumpy()) * grid.dV if full else np.fft.rfftn(f.asnumpy()) * grid.dV
         assert np.abs(F.asnumpy() - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def _displaced_cell():
+    cell = np.eye(3) * 7.65339
+    frac = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
+    pos = frac @ cell + np.random.default_rng(2).normal(0.0, 0.15, size=(4, 3))
+    return ["Al", "Mg", "Al", "Al"], pos, cell, np.array([3.0, 2.0, 3.0, 3.0])
+
+
+@pytest.mark.parametrize("pme", [False, True])
+def test_ewald_forces_match_oracle(pme):
+    """Ion-ion forces (ewald.py:509-519) against the oracle on the fixture
+    (zero by symmetry, test_ewald.py:37-40) and on a displaced Al/Mg cell."""
+    print("case ewald forces pme=%s" % pme, flush=True)
+    oi, rho, og, grid, ions = _fixture()
+    F = D.Ewald(ions=ions, grid=grid, PME=pme).forces
+    assert np.abs(F).max() < 1e-5
+    syms, pos, cell, q = _displaced_cell()
+    for nr in ((24, 24, 24), (27, 30, 32)):
+        grid = D.DirectGrid(cell, nr=nr, full=False)
+        ions = D.Ions(syms, pos, cell, charges=q)
+        F = D.Ewald(ions=ions, grid=grid, PME=pme).forces
+        og = O.Grid(cell, nr, full=False)
+        Fo = O.Ewald(og, O.Ions(syms, pos, cell, charges=q), PME=pme).forces
+        print(nr, np.abs(F - Fo).max(), np.abs(Fo).max())
+        assert np.abs(F - Fo).max() < 1e-10 * max(1.0, np.abs(Fo).max())
+
+
+def test_total_forces_split_and_ewald_exact_vs_pme():
+    """TotalFunctional.get_forces (total_functional.py:148-174): TOTAL =
+    PSEUDO + II; the ion-ion forces of the 256-atom Al/Mg cell at 128^3 agree
+    between the exact and PME reciprocal sums."""
+    wl = W.build("almg256_128")
+    grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
+    ions = D.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    pp = D.LocalPseudo(grid=grid, ions=ions, PP_list={s: D.empty_core(*W.EMPTY_CORE[s]) for s in wl["tables"]})
+    tot = D.TotalFunctional(KEDF=D.KEDF("WT"), XC=D.XC("LDA"), HARTREE=D.Hartree(), PSEUDO=pp)
+    rho = D.DirectField(grid=grid, data=wl["rho_ini"])
+    f = tot.get_forces(rho, split=True)
+    assert set(f) == {"TOTAL", "PSEUDO", "II"}
+    assert np.abs(f["TOTAL"] - f["PSEUDO"] - f["II"]).max() < 1e-12
+    assert np.abs(tot.get_forces(rho) - f["TOTAL"]).max() == 0.0
+    Fx = D.Ewald(ions=ions, grid=grid, PME=False).forces
+    print("Ewald exact vs PME at 128^3: %.3e (max |F| %.3e)" % (np.abs(Fx - f["II"]).max(), np.abs(Fx).max()))
+    assert np.abs(Fx - f["II"]).max() < 1e-4
+    # momentum conservation: exact for the exact sums, to the PME accuracy for PME
+    assert np.abs(Fx.sum(axis=0)).max() < 1e-9 * np.abs(Fx).max() * len(Fx)
+    assert np.abs(f["II"].sum(axis=0)).max() < 1e-5

@@ -67,6 +67,48 @@ def test_ewald_known_answers(fixture_rho):
     assert abs(O.Ewald(grid, ions, PME=True).energy_rec_pme() - 7.606943756474895) < 1e-5
 
 
+def test_ewald_forces_fixture_known_answers(fixture_rho):
+    """examples/test/test_ewald.py:37-50: zero forces on the perfect fcc
+    cell, exact and PME forms agree."""
+    ions, rho, grid = fixture_rho
+    ew = O.Ewald(grid, ions, PME=False)
+    ewp = O.Ewald(grid, ions, PME=True)
+    fr, fk, fp = ew.forces_real(), ew.forces_rec(), ewp.forces_rec_pme()
+    assert abs(fr[0, 0]) < 1e-5 and abs(fk[0, 0]) < 1e-5 and abs(fp[0, 0]) < 1e-5
+    assert np.allclose(ew.forces, ewp.forces, atol=1e-5)
+
+
+@pytest.mark.parametrize("pme", [False, True])
+def test_ewald_forces_are_minus_energy_gradient(pme):
+    """Central difference of the Ewald energy (real + reciprocal; the
+    correction term does not depend on positions) on a displaced Al/Mg cell."""
+    cell = np.eye(3) * 7.65339
+    frac = np.array([[0.0, 0.0, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5], [0.5, 0.5, 0.0]])
+    pos = frac @ cell + np.random.default_rng(2).normal(0.0, 0.15, size=(4, 3))
+    syms = ["Al", "Mg", "Al", "Al"]
+    q = np.array([3.0, 2.0, 3.0, 3.0])
+    grid = O.Grid(cell, (24, 24, 24), full=False)
+
+    def ew(p):
+        e = O.Ewald(grid, O.Ions(syms, p, cell, charges=q), PME=pme)
+        return e
+
+    e0 = ew(pos)
+    F = e0.forces
+    h = 1e-4
+    for a in range(4):
+        for d in range(3):
+            pp = pos.copy()
+            pm = pos.copy()
+            pp[a, d] += h
+            pm[a, d] -= h
+            ep, em = ew(pp), ew(pm)
+            def en(e):
+                return e.energy_real() + (e.energy_rec_pme() if pme else e.energy_rec())
+            fd = -(en(ep) - en(em)) / (2 * h)
+            assert abs(F[a, d] - fd) < 1e-6 * max(1.0, abs(fd)), (a, d, F[a, d], fd)
+
+
 def test_ewald_real_space_brute_force():
     """Real-space Ewald sum equals a direct image loop on a tiny cell."""
     from scipy.special import erfc
