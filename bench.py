@@ -107,9 +107,9 @@ def cpu_baseline(wl, nevals, niter_evals):
     dt = time.perf_counter() - t0
     evals_per_s = nevals / dt
     return dict(value=evals_per_s / niter_evals, unit="TN iterations/s", cores=1, kind="port",
-                sample="%d FP64 E+V evaluations of the oracle (numpy, 1 thread) on the same 192^3 WT workload in "
+                sample="%d FP64 E+V evaluations of the oracle (numpy, 1 thread) on the same %s workload (%d^3, %s) in "
                        "%.1f s = %.4f evals/s, divided by %.2f evaluations per TN iteration (GPU FP64 solve log)"
-                       % (nevals, dt, evals_per_s, niter_evals))
+                       % (nevals, wl["name"], wl["nr"][0], wl["kedf"], dt, evals_per_s, niter_evals))
 
 
 def batch_main(args):
@@ -223,8 +223,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic: seeded fcc Al 6x6x6 supercell, sigma=0.05 A displacements, uniform+1% noise density",
-        "config": {"workload": args.workload, "grid": list(wl["nr"]), "atoms": nat, "kedf": "WT+TF+vW",
-                   "xc": "LDA(PZ)", "pp": "empty-core Al (Z=3, rc=1.12 bohr), PME order 10",
+        "config": {"workload": args.workload, "grid": list(wl["nr"]), "atoms": nat,
+                   "kedf": "WT+TF+vW" if wl["kedf"] == "WT" else "TF+vW(y=%.4g)" % wl["kedf_options"]["y"],
+                   "xc": "LDA(PZ)",
+                   "pp": ", ".join("empty-core %s (Z=%g, rc=%g bohr)" % ((k,) + W.EMPTY_CORE[k]) for k in wl["tables"])
+                         + ", PME order 10",
                    "optimizer": "TN, exact HVP", "econv_Ha": args.econv, "precision": ("mixed: FP32 outer then FP64 outer, FP32 inner CG" if args.phase1 else "mixed: FP64 outer, FP32 inner CG"),
                    "tn_iterations_per_solve": iters / args.steps},
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (cls_d, prec_d), "achieved": achieved,

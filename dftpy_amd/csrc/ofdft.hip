@@ -121,6 +121,7 @@ struct ofdft_plan_s {
   long long npartials;
   double* red;      // device, kMaxSum doubles
   double* hred;     // pinned host mirror
+  double* dhred;    // device address of hred (reduction kernels write final sums there)
   void* tnst;       // device TnState of ofdft_tn_direction
   void* htnst;      // pinned host mirror of it
   const int* skip;  // while set, FFT and reduction kernels return at once when *skip != 0
@@ -1096,13 +1097,15 @@ __global__ void k_wt_table(T* __restrict__ out, GridGeom geo, double tkf, double
 
 template <int NS>
 __global__ void k_reduce_partials(const double* __restrict__ partials, long long nblk, double* __restrict__ out,
-                                  const int* __restrict__ skip) {
+                                  double* __restrict__ hout, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   const int s = blockIdx.x;
   double acc = 0.0;
   for (long long b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[b * NS + s];
   double tmp[1] = {acc};
   block_sums<1>(tmp, out + s);
+  // thread 0 wrote out[s]; the mapped host copy saves a device-to-host copy
+  if (hout && threadIdx.x == 0) hout[s] = out[s];
 }
 
 // --------------------------- vector kernels -------------------------------
@@ -1457,8 +1460,10 @@ void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const
                      (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
 }
 
+// reduce: the post functor's sums end in P->red, and also in the mapped host
+// buffer when to_host (for a fetch_sums that follows).
 template <typename T, class Post>
-int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce) {
+int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce, bool to_host = true) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
   const int PB = kRowPB<T>;
@@ -1484,7 +1489,8 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   }
   LAUNCHCHK();
   if (reduce) {
-    hipLaunchKernelGGL((k_reduce_partials<Post::NS>), dim3(Post::NS), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
+    hipLaunchKernelGGL((k_reduce_partials<Post::NS>), dim3(Post::NS), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red,
+                       to_host ? P->dhred : nullptr, P->skip);
     LAUNCHCHK();
   }
   return 0;
@@ -1582,8 +1588,8 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
   return 0;
 }
 
+// the final reduction kernel also wrote its sums into the mapped host buffer
 int fetch_sums(ofdft_plan_s* P, int ns, double* out) {
-  HIPCHK(hipMemcpyAsync(P->hred, P->red, sizeof(double) * ns, hipMemcpyDeviceToHost, P->stream));
   HIPCHK(hipStreamSynchronize(P->stream));
   for (int i = 0; i < ns; ++i) out[i] = P->hred[i];
   return 0;
@@ -1683,7 +1689,7 @@ int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap,
     if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
   }
   PostHvpFast<T> post{phi, p, W0, Ap, fp};
-  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true))) return rc;
+  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, pAp != nullptr))) return rc;
   if (pAp) {
     double t;
     if ((rc = fetch_sums(P, 1, &t))) return rc;
@@ -1745,9 +1751,9 @@ int conv_impl(ofdft_plan_s* P, int kind, const T* in, T* out) {
 int reduce_blocks(ofdft_plan_s* P, int nblk, int ns, double* out) {
   int rc;
   switch (ns) {
-    case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
-    case 3: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
-    case 5: hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->skip); break;
+    case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
+    case 3: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
+    case 5: hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
     default: return fail(OFDFT_EINVAL, "unsupported number of sums");
   }
   LAUNCHCHK();
@@ -2337,9 +2343,9 @@ int ensure_bm(ofdft_plan_s* P, int order) {
 int reduce_to_host(ofdft_plan_s* P, long long nblk, int ns, double* out) {
   // partial layout [blk][ns]; reuse k_reduce_partials
   if (ns == 1)
-    hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
+    hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->dhred, P->skip);
   else
-    hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->skip);
+    hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red, P->dhred, P->skip);
   LAUNCHCHK();
   return fetch_sums(P, ns, out);
 }
@@ -2371,7 +2377,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
     OFDFT_LAUNCH((k_tn_init<T>), dim3(nb), dim3(256), 0, P, res, d, r, p, N, P->partials);
   }
   LAUNCHCHK();
-  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr,
+  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr, (double*)nullptr,
                      (const int*)nullptr);
   LAUNCHCHK();
   hipLaunchKernelGGL(k_tn_start, dim3(1), dim3(1), 0, P->stream, st, (const double*)rr, maxfun);
@@ -2390,7 +2396,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
                            (const T*)p, (const T*)Ap, d, r, N, P->partials);
       }
       hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials,
-                         (long long)nb, rr, P->skip);
+                         (long long)nb, rr, (double*)nullptr, P->skip);
       hipLaunchKernelGGL(k_tn_ctrl, dim3(1), dim3(1), 0, P->stream, st, (const double*)P->red, (const double*)rr);
       {
         ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
@@ -2595,6 +2601,8 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   if (hipMalloc((void**)&P->red, sizeof(double) * kMaxSum) != hipSuccess) return cleanup(OFDFT_ENOMEM, "red");
   if (hipHostMalloc((void**)&P->hred, sizeof(double) * kMaxSum, hipHostMallocDefault) != hipSuccess)
     return cleanup(OFDFT_ENOMEM, "pinned host buffer");
+  if (hipHostGetDevicePointer((void**)&P->dhred, P->hred, 0) != hipSuccess)
+    return cleanup(OFDFT_EHIP, "pinned host buffer mapping");
   if (hipMalloc(&P->tnst, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_ENOMEM, "TN state");
   if (hipMemset(P->tnst, 0, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_EHIP, "TN state clear");
   if (hipHostMalloc(&P->htnst, sizeof(TnState), hipHostMallocDefault) != hipSuccess)
@@ -2799,7 +2807,7 @@ int ofdft_tn_inner(ofdft_plan P, const void* phi, const void* p, const void* W0,
                        (const float*)Ap, (float*)d, (float*)r, P->N, P->partials);
   }
   LAUNCHCHK();
-  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, red2, P->skip);
+  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, red2, (double*)nullptr, P->skip);
   LAUNCHCHK();
   HIPCHK(hipMemcpyAsync(P->hred, P->red, sizeof(double) * 9, hipMemcpyDeviceToHost, P->stream));
   HIPCHK(hipStreamSynchronize(P->stream));
