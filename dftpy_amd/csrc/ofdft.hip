@@ -203,29 +203,48 @@ template <typename T>
 __device__ __forceinline__ T dev_cbrt(T x);
 template <>
 __device__ __forceinline__ double dev_cbrt<double>(double x) { return cbrt(x); }
+// FP32 transcendentals use the hardware's 1-ulp approximations (v_log_f32 =
+// log2, v_exp_f32 = 2^x, v_sqrt_f32, v_rcp_f32): the FP32 path accumulates
+// every energy in FP64 and is refined in FP64, and the row transforms that
+// evaluate these functions are bound by instruction issue.  FP64 keeps the
+// correctly rounded library functions.
 template <>
-__device__ __forceinline__ float dev_cbrt<float>(float x) { return cbrtf(x); }
+__device__ __forceinline__ float dev_cbrt<float>(float x) {
+  return x > 0.f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.f / 3.f)) : cbrtf(x);
+}
 
 template <typename T>
 __device__ __forceinline__ T dev_pow(T x, T y);
 template <>
 __device__ __forceinline__ double dev_pow<double>(double x, double y) { return pow(x, y); }
 template <>
-__device__ __forceinline__ float dev_pow<float>(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ float dev_pow<float>(float x, float y) {
+  return x > 0.f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)) : powf(x, y);
+}
 
 template <typename T>
 __device__ __forceinline__ T dev_log(T x);
 template <>
 __device__ __forceinline__ double dev_log<double>(double x) { return log(x); }
 template <>
-__device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
+__device__ __forceinline__ float dev_log<float>(float x) {
+  return x > 0.f ? __builtin_amdgcn_logf(x) * 0.693147180559945309f : logf(x);
+}
 
 template <typename T>
 __device__ __forceinline__ T dev_sqrt(T x);
 template <>
 __device__ __forceinline__ double dev_sqrt<double>(double x) { return sqrt(x); }
 template <>
-__device__ __forceinline__ float dev_sqrt<float>(float x) { return sqrtf(x); }
+__device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// a / b: IEEE division in FP64, a * rcp(b) in FP32
+template <typename T>
+__device__ __forceinline__ T dev_div(T a, T b);
+template <>
+__device__ __forceinline__ double dev_div<double>(double a, double b) { return a / b; }
+template <>
+__device__ __forceinline__ float dev_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -285,7 +304,7 @@ __device__ __forceinline__ void lda_point(T rho, T rcbrt, T& ex, T& vxc) {
   const T g = T(-0.1423), b1 = T(1.0529), b2 = T(0.3334);
   T rc = rcbrt;
   if (rc < T(1e-30)) rc = T(1e-30);
-  const T rs = T(0.62035049089940001667) / rc;  // cbrt(3/(4 pi)) / cbrt(rho)
+  const T rs = dev_div(T(0.62035049089940001667), rc);  // cbrt(3/(4 pi)) / cbrt(rho)
   ex = T(-0.73855876638202240588) * rc;         // -3/4 cbrt(3/pi)
   vxc = T(-0.98474502184269654118) * rc;        // cbrt(-3/pi)
   if (rs < T(1)) {
@@ -295,8 +314,8 @@ __device__ __forceinline__ void lda_point(T rho, T rcbrt, T& ex, T& vxc) {
   } else {
     const T sr = dev_sqrt(rs);
     const T den = T(1) + b1 * sr + b2 * rs;
-    ex += g / den;
-    vxc += (g + (T(7.0 / 6.0) * g * b1) * sr + (T(4.0 / 3.0) * g * b2 * rs)) / (den * den);
+    ex += dev_div(g, den);
+    vxc += dev_div(g + (T(7.0 / 6.0) * g * b1) * sr + (T(4.0 / 3.0) * g * b2 * rs), den * den);
   }
 }
 
@@ -450,9 +469,9 @@ struct PreHvp {
     const T dr = T(2) * ph * pp;
     const T s = dev_sqrt(r);
     if (fp.fH >= 0) put3(out, fp.fH, dr);
-    if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dr / (T(2) * s) : T(0));
+    if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dev_div(dr, T(2) * s) : T(0));
     if (fp.fW >= 0) {
-      const T rb1 = fp.beta56 ? dev_cbrt(r) / s : dev_pow(r, T(fp.beta) - T(1));  // rho^(beta-1)
+      const T rb1 = fp.beta56 ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));  // rho^(beta-1)
       put3(out, fp.fW, r < T(kZero) ? T(0) : T(fp.beta) * rb1 * dr);
     }
   }
@@ -518,7 +537,7 @@ struct PostEval {
     }
     if (fp.fV >= 0) {
       // vW: 0.5 * IFFT[G^2 FFT(s)] / s, energy over rho > 0 (vw.py:29-53, :93)
-      const T vraw = T(0.5) * pick3(c, fp.fV) / s;
+      const T vraw = dev_div(T(0.5) * pick3(c, fp.fV), s);
       if (r > T(0)) acc[OFDFT_E_VW] += (double)(r * vraw);
       v += vraw * T(fp.y);
       if (aux) aux[i] = vraw;
@@ -526,7 +545,7 @@ struct PostEval {
     if (fp.fW >= 0) {
       // WT: 2 alpha rho^beta / rho * conv, energy density V rho/(2 alpha) (wt.py:12-28, :125)
       const T rz = r < T(kZero) ? T(kZero) : r;
-      const T rbm1 = (fp.beta56 && r >= T(kZero)) ? cb / s : dev_pow(rz, T(fp.beta)) / rz;
+      const T rbm1 = (fp.beta56 && r >= T(kZero)) ? dev_div(cb, s) : dev_div(dev_pow(rz, T(fp.beta)), rz);
       const T vw = T(2 * fp.alpha) * rbm1 * pick3(c, fp.fW);
       acc[OFDFT_E_NL] += (double)(vw * r / T(2 * fp.alpha));
       v += vw;
@@ -626,9 +645,9 @@ struct PostHvpFast {
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
     if (fp.fV >= 0 || fp.fW >= 0) {
       const T s = dev_sqrt(r);
-      if (fp.fV >= 0 && r > T(0)) dv += T(fp.y) * T(0.5) * pick3(c, fp.fV) / s;
+      if (fp.fV >= 0 && r > T(0)) dv += dev_div(T(fp.y) * T(0.5) * pick3(c, fp.fV), s);
       if (fp.fW >= 0 && r >= T(kZero)) {
-        const T rb1 = fp.beta56 ? dev_cbrt(r) / s : dev_pow(r, T(fp.beta) - T(1));
+        const T rb1 = fp.beta56 ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));
         dv += T(2 * fp.alpha) * rb1 * pick3(c, fp.fW);
       }
     }
@@ -682,12 +701,15 @@ constexpr int fixed_len() {
   else return 1;
 }
 
-template <typename T, class FFT, class Pre>
+// FC > 0: the field count F fixed at compile time (index arithmetic by F
+// becomes constant); FC = 0: F taken from Frt.
+template <typename T, class FFT, class Pre, int FC>
 __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int nhp, int F,
+                                                      long long nrows, long long Nh, int nhp, int Frt,
                                                       const cplx<T>* __restrict__ tw,
                                                       const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  const int F = FC > 0 ? FC : Frt;
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPB<T>;
@@ -758,12 +780,13 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
 // real part is row r, the imaginary part row r+1.  For compile-time lengths
 // the post functor's own inputs are loaded first, so that their latency
 // overlaps the spectrum loads and the transform.
-template <typename T, class FFT, class Post>
+template <typename T, class FFT, class Post, int FC>
 __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int nhp, int F,
+                                                      long long nrows, long long Nh, int nhp, int Frt,
                                                       const cplx<T>* __restrict__ tw,
                                                       double* __restrict__ partials, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  const int F = FC > 0 ? FC : Frt;
   extern __shared__ unsigned char smem_raw[];
   cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPB<T>;
@@ -1424,7 +1447,13 @@ using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
 template <typename T, class Pre, class FFT>
 void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
-  OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre>), grid, dim3(threads), lds, P, pre, fft,
+  if (F == 3) {
+    OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 3>), grid, dim3(threads), lds, P, pre, fft,
+                 reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
+                 reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
+    return;
+  }
+  OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 0>), grid, dim3(threads), lds, P, pre, fft,
                      reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
                      reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
 }
@@ -1456,7 +1485,13 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
 template <typename T, class Post, class FFT>
 void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const cplx<T>* src, dim3 grid,
                  int threads, size_t lds, double* part) {
-  OFDFT_LAUNCH((k_rows_inv<T, FFT, Post>), grid, dim3(threads), lds, P, post, fft, src,
+  if (F == 3) {
+    OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 3>), grid, dim3(threads), lds, P, post, fft, src,
+                 (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part,
+                 P->skip);
+    return;
+  }
+  OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 0>), grid, dim3(threads), lds, P, post, fft, src,
                      (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
 }
 
