@@ -1075,10 +1075,37 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
     dftreg<T, N2>(v, false);
     const T scale = T(ck.scale);
+    if constexpr (sizeof(T) == 4) {
+      if (live && !wt && !geo.full && (kind == OFDFT_CONV_HARTREE || kind == OFDFT_CONV_GG)) {
+        // FP32 Hartree / vW multipliers along the pencil: the (i1, c) part of
+        // G is fixed per thread, only the axis-0 frequency changes
+        const float m1 = (float)fftfreq_i(i1, geo.n1), m2 = (float)c;
+        const float px = m1 * geo.bgf[3] + m2 * geo.bgf[6];
+        const float py = m1 * geo.bgf[4] + m2 * geo.bgf[7];
+        const float pz = m1 * geo.bgf[5] + m2 * geo.bgf[8];
+        const bool hartree = kind == OFDFT_CONV_HARTREE;
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) {
-      const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
-      v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+        for (int k2 = 0; k2 < N2; ++k2) {
+          const float m0 = (float)fftfreq_i(q + N1 * k2, N1 * N2);
+          const float gx = m0 * geo.bgf[0] + px, gy = m0 * geo.bgf[1] + py, gz = m0 * geo.bgf[2] + pz;
+          const float g2 = gx * gx + gy * gy + gz * gz;
+          float m = hartree ? (g2 > 0.f ? float(4.0 * kPi) * __builtin_amdgcn_rcpf(g2) : 0.f) : g2;
+          m *= scale;
+          v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+        }
+      } else {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+          const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+          v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+        }
+      }
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+        v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+      }
     }
     dftreg<T, N2>(v, true);
 #pragma unroll
