@@ -26,6 +26,12 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM traffic per launch of a kernel class, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on the named workload and scaled by what the same
+# counters read for a plain device copy (FETCH x 2.00, WRITE x 1.00); the
+# summary and commands are in profiles/axis_c2c_fp32_al864_192.md.
+MEASURED_TRAFFIC = {("al864_192", "fp32", "axis_c2c"): 198.3e6}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -231,7 +237,8 @@ def main():
                    "optimizer": "TN, exact HVP", "econv_Ha": args.econv, "precision": ("mixed: FP32 outer then FP64 outer, FP32 inner CG" if args.phase1 else "mixed: FP64 outer, FP32 inner CG"),
                    "tn_iterations_per_solve": iters / args.steps},
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (cls_d, prec_d), "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, prec_d, cls_d)),
                      "launches": cnt_d, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "all_kernels_GBps": total_bytes / (total_dev_ms / 1e3) / 1e9 if total_dev_ms else None},
         # one warm-up solve, every class timed: [total ms, launches, us per launch, GB/s]
