@@ -1229,8 +1229,12 @@ __global__ void k_cg_update_dev(const double* __restrict__ red, double r0, const
 // ---------------------------------------------------------------------------
 // Device-resident TN direction loop: get_direction_TN (optimization.py:92-148)
 // with the exact HVP.  The host queues CG iterations without waiting; a
-// one-thread control kernel applies the reference's tests to the reductions
-// and sets `done`, after which every queued kernel returns at once.
+// one-block kernel reduces r^2, applies the reference's tests and sets
+// `done`, after which every queued kernel returns at once.  The updates the
+// tests decide are carried by the next iteration's own kernels: p = r +
+// beta p by the forward row pass of the next product, Best = direction by
+// the next step kernel; the terminal ones (restore Best, the negative-
+// curvature scaling) by one kernel after the loop.
 // ---------------------------------------------------------------------------
 struct TnState {
   double r0, rconv, rmin, r1, pAp, beta, scale, pad;
@@ -1238,10 +1242,10 @@ struct TnState {
   int it;      // index of the CG iteration being run
   int niter;   // iterations taken (the reference's `number`)
   int status;  // OFDFT_TN_*
-  int act;     // kAct* bits for k_tn_apply of this iteration
+  int act;     // kActKeep / kActNext for the next iteration's kernels
   int maxit;
   int alias;   // `Best` still aliases `direction` (no copy taken yet)
-  int pad2;
+  int fin;     // kActRestore / kActScale for k_tn_finish
 };
 constexpr int kActKeep = 1;     // Best = direction.copy()
 constexpr int kActRestore = 2;  // direction = Best
@@ -1279,23 +1283,28 @@ __global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit
   st->act = 0;
   st->maxit = maxit;
   st->alias = 1;
+  st->fin = 0;
   st->done = maxit > 0 ? 0 : 1;
 }
 
-// alpha = r0 / pAp; d += alpha p; r -= alpha Ap; partial sums of r^2.
-// Nothing happens when the loop is done or pAp < 0 (the reference leaves
-// the loop before the update).
+// Best = direction when the previous iteration asked for a copy (direction
+// has not changed since); then alpha = r0 / pAp; d += alpha p;
+// r -= alpha Ap; partial sums of r^2.  Nothing happens when the loop is
+// done or pAp < 0 (the reference leaves the loop before the update).
 template <typename T>
 __global__ void k_tn_step(const TnState* __restrict__ st, const double* __restrict__ red, const T* __restrict__ p,
-                          const T* __restrict__ Ap, T* __restrict__ d, T* __restrict__ r, long long N,
-                          double* __restrict__ partials) {
+                          const T* __restrict__ Ap, T* __restrict__ d, T* __restrict__ r, T* __restrict__ best,
+                          long long N, double* __restrict__ partials) {
   if (st->done) return;
   const double pAp = red[0];
   if (pAp < 0.0) return;
   const T a = T(st->r0 / pAp);
+  const bool keep = (st->act & kActKeep) != 0;
   double acc[1] = {0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-    d[i] += a * p[i];
+    const T di = d[i];
+    if (keep) best[i] = di;
+    d[i] = di + a * p[i];
     const T rr = r[i] - a * Ap[i];
     r[i] = rr;
     acc[0] += (double)rr * (double)rr;
@@ -1303,27 +1312,21 @@ __global__ void k_tn_step(const TnState* __restrict__ st, const double* __restri
   block_sums<1>(acc, partials + blockIdx.x);
 }
 
-// The branch tests of optimization.py:116-145 on pAp = red[0], r1 = rr[0].
-__global__ void k_tn_ctrl(TnState* st, const double* __restrict__ red, const double* __restrict__ rr) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (st->done) {
-    st->act = 0;
-    return;
-  }
-  const double pAp = red[0];
+// The branch tests of optimization.py:116-145 on pAp and r1.
+__device__ void tn_ctrl(TnState* st, double pAp, double r1) {
   st->pAp = pAp;
-  int act = 0;
+  int act = 0, fin = 0;
   if (pAp < 0.0) {
     if (st->it == 0) {
       st->scale = st->r0 / pAp;
-      act = kActScale;
+      fin = kActScale;
       st->status = OFDFT_TN_WARN;
     } else {
       st->status = OFDFT_TN_FAILED;
     }
     st->done = 1;
   } else {
-    const double r1 = rr[0], r0 = st->r0, rmin = st->rmin;  // rmin = min(rLists[:-1])
+    const double r0 = st->r0, rmin = st->rmin;  // rmin = min(rLists[:-1])
     st->r1 = r1;
     if (r1 < rmin) {
       act |= kActKeep;
@@ -1333,11 +1336,11 @@ __global__ void k_tn_ctrl(TnState* st, const double* __restrict__ red, const dou
       st->status = OFDFT_TN_CONV;
       st->done = 1;
     } else if (r1 > 1000.0 * rmin) {
-      act |= kActRestore;
+      fin = kActRestore;
       st->status = OFDFT_TN_NOTREDUCE;
       st->done = 1;
     } else if (st->it > 10 && fabs(r0 - r1) < 0.1 * r0) {
-      act |= kActRestore;
+      fin = kActRestore;
       st->status = OFDFT_TN_SMALL;
       st->done = 1;
     } else {
@@ -1345,9 +1348,9 @@ __global__ void k_tn_ctrl(TnState* st, const double* __restrict__ red, const dou
       st->r0 = r1;
       act |= kActNext;
     }
-    // restoring Best while it still aliases direction leaves direction as is;
-    // a fresh copy followed by a restore is a no-op too
-    if ((act & kActRestore) && (st->alias || (act & kActKeep))) act &= ~kActRestore;
+    // restoring Best while it still aliases direction leaves direction as
+    // is; a copy taken in the same iteration equals direction too
+    if (st->alias || (act & kActKeep)) fin &= ~kActRestore;
     if (act & kActKeep) st->alias = 0;
   }
   if (st->done) {
@@ -1357,23 +1360,70 @@ __global__ void k_tn_ctrl(TnState* st, const double* __restrict__ red, const dou
     st->niter = st->maxit;
     st->status = OFDFT_TN_MAXFUN;
   }
+  // once done, no later kernel reads `act`: the pending copy of Best is dropped
   st->it += 1;
-  st->act = act;
+  st->act = st->done ? 0 : act;
+  st->fin = fin;
 }
 
-template <typename T>
-__global__ void k_tn_apply(const TnState* __restrict__ st, T* __restrict__ d, const T* __restrict__ r,
-                           T* __restrict__ p, T* __restrict__ best, long long N) {
-  const int act = st->act;
-  if (!act) return;
-  const T beta = T(st->beta), scale = T(st->scale);
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
-    if (act & kActKeep) best[i] = d[i];
-    if (act & kActRestore) d[i] = best[i];
-    if (act & kActScale) d[i] = scale * p[i];
-    if (act & kActNext) p[i] = r[i] + beta * p[i];
-  }
+// r1 = sum of the step kernel's partials, then the tests; pAp = red[0].
+__global__ void k_tn_reduce_ctrl(TnState* st, const double* __restrict__ partials, long long nblk,
+                                 const double* __restrict__ red, double* __restrict__ rr) {
+  if (st->done) return;
+  double acc = 0.0;
+  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[b];
+  double tmp[1] = {acc};
+  block_sums<1>(tmp, rr);
+  if (threadIdx.x == 0) tn_ctrl(st, red[0], rr[0]);
 }
+
+// p = r + beta p when the control asked for it: used only where no forward
+// row pass precedes the next product (no convolution fields).
+template <typename T>
+__global__ void k_tn_next(const TnState* __restrict__ st, const T* __restrict__ r, T* __restrict__ p, long long N) {
+  if (st->done || !(st->act & kActNext)) return;
+  const T beta = T(st->beta);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
+    p[i] = r[i] + beta * p[i];
+}
+
+// The terminal update of the loop: direction = Best, or r0 / pAp * p.
+template <typename T>
+__global__ void k_tn_finish(const TnState* __restrict__ st, T* __restrict__ d, const T* __restrict__ p,
+                            const T* __restrict__ best, long long N) {
+  const int fin = st->fin;
+  if (!fin) return;
+  const T scale = T(st->scale);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
+    d[i] = (fin & kActScale) ? scale * p[i] : best[i];
+}
+
+// The HVP's forward row functor with the CG update of the previous
+// iteration folded in: p = r + beta p (optimization.py:144-145) when the
+// control asked for it, written back for the step kernel and the next
+// update; each point is loaded by exactly one thread of the row pass.
+template <typename T>
+struct PreHvpCG {
+  static constexpr int kProf = PC_ROWS_FWD_HVP;
+  PreHvp<T> h;  // its transform of (phi, p); its own pointers are not read
+  const T* __restrict__ phi;
+  T* __restrict__ p;  // the direction, updated in place
+  const T* __restrict__ r;
+  const TnState* __restrict__ st;
+  __host__ __device__ int nin() const { return 4; }
+  using In = typename PreHvp<T>::In;
+  __device__ __forceinline__ In load(long long i) const {
+    const T ph = phi[i];
+    T pp = p[i];
+    if (st->act & kActNext) {
+      pp = r[i] + T(st->beta) * pp;
+      p[i] = pp;
+    }
+    return {ph, pp};
+  }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { h.apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out) const { h.apply(x, out); }
+};
 
 // The five sums one outer iteration needs after the direction is found:
 // sum p phi, sum phi^2, sum p^2 (OrthogonalNormalization), sum V phi p and
@@ -1732,8 +1782,12 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
   return 0;
 }
 
+// The exact HVP; with `cg`, the forward row pass first applies the pending
+// CG update p = r + beta p of the device-side TN loop (p is then updated in
+// place; with no convolution fields the caller does that update itself).
 template <typename T>
-int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap, double* pAp) {
+int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap, double* pAp,
+                  const T* cg_r = nullptr, const TnState* cg_st = nullptr) {
   if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
   FnParams fp = make_fn(P->fn);
   const int F = fp.F;
@@ -1741,7 +1795,12 @@ int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap,
   int rc;
   if (F > 0) {
     PreHvp<T> pre{phi, p, fp};
-    if ((rc = launch_rows_fwd<T>(P, pre, F))) return rc;
+    if (cg_st) {
+      PreHvpCG<T> pcg{pre, phi, const_cast<T*>(p), cg_r, cg_st};
+      if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
+    } else if ((rc = launch_rows_fwd<T>(P, pre, F))) {
+      return rc;
+    }
     if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
     int kinds[3];
     if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
@@ -2447,24 +2506,23 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   auto* h = reinterpret_cast<TnState*>(P->htnst);
   const size_t prof_mark = P->prof_pending.size();
   P->skip = &st->done;
+  const bool fused = make_fn(P->fn).F > 0;  // a forward row pass carries the update of p
   int rc = 0, queued = 0, k = std::max(1, chunk);
   while (queued < maxfun) {
     for (int j = 0; j < k && queued < maxfun; ++j, ++queued) {
       P->prof_tag = queued;
-      if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr))) break;
+      if (!fused) {
+        ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
+        OFDFT_LAUNCH((k_tn_next<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, (const T*)r, p, N);
+      }
+      if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr, (const T*)r, (const TnState*)st))) break;
       {
         ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * N);
         OFDFT_LAUNCH((k_tn_step<T>), dim3(nb), dim3(256), 0, P, st, (const double*)P->red,
-                           (const T*)p, (const T*)Ap, d, r, N, P->partials);
+                     (const T*)p, (const T*)Ap, d, r, best, N, P->partials);
       }
-      hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials,
-                         (long long)nb, rr, (double*)nullptr, P->skip);
-      hipLaunchKernelGGL(k_tn_ctrl, dim3(1), dim3(1), 0, P->stream, st, (const double*)P->red, (const double*)rr);
-      {
-        ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
-        OFDFT_LAUNCH((k_tn_apply<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, d, (const T*)r,
-                           p, best, N);
-      }
+      hipLaunchKernelGGL(k_tn_reduce_ctrl, dim3(1), dim3(kRedThreads), 0, P->stream, st,
+                         (const double*)P->partials, (long long)nb, (const double*)P->red, rr);
       if (hipGetLastError() != hipSuccess) {
         rc = fail(OFDFT_EHIP, "kernel launch failed in the TN loop");
         break;
@@ -2484,6 +2542,12 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   P->skip = nullptr;
   if (rc) return rc;
   if (!h->done) return fail(OFDFT_ESTATE, "TN loop did not finish");
+  if (h->fin) {
+    ProfScope ps(P, PC_VEC, (long long)P->rs * 2 * N);
+    OFDFT_LAUNCH((k_tn_finish<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, d, (const T*)p,
+                 (const T*)best, N);
+    LAUNCHCHK();
+  }
   // launches queued after the loop finished did no work: drop their timings
   P->prof_pending.erase(std::remove_if(P->prof_pending.begin() + prof_mark, P->prof_pending.end(),
                                        [&](const ofdft_plan_s::ProfRec& x) { return x.tag >= h->niter; }),

@@ -231,18 +231,16 @@ def _tn_host(hvp, res0, maxfun):
     return direction, it + 1, stat, pAp
 
 
-@pytest.mark.parametrize("mu,maxfun", [(0.31, 50), (0.31, 2), (0.31, 1), (40.0, 50)])
-@pytest.mark.parametrize("chunk", [1, 3, 16])
-def test_tn_direction_matches_host_loop(mu, maxfun, chunk):
-    """The device-resident CG loop takes the branches of the reference loop
-    (convergence, maxfun, negative curvature at the first iteration) and
-    returns its direction and iteration count."""
+def _tn_case(mu, maxfun, chunk, local=False):
+    """Device TN loop against the host restatement on the 4-atom cell; with
+    `local` only TF + LDA + PP (no convolution fields, so no forward row pass
+    carries the CG update)."""
     from dftpy_amd import _lib as L
-    print("case tn_direction", mu, maxfun, chunk, flush=True)
+    print("case tn_direction", mu, maxfun, chunk, local, flush=True)
     wl, grid, ions, pp = _al4()
     rho = wl["rho_ini"]
     P = _plan(wl["nr"], wl["cell"], "fp64")
-    P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+    P.set_functional(tf=True, vw=not local, wt=not local, lda=True, hartree=not local, pseudo=True, rho0=rho.mean())
     P.set_vloc(torch.tensor(pp.vreal, dtype=P.dtype, device=P.device))
     t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
     phi = t(np.sqrt(rho))
@@ -271,6 +269,30 @@ def test_tn_direction_matches_host_loop(mu, maxfun, chunk):
     n2, st2, _ = P.tn_direction(phi, res, W0, d, work, maxfun, chunk)
     assert (n2, st2) == (n, st)
     assert np.abs(d.cpu().numpy() - dd).max() <= 1e-12 * np.abs(dd).max()
+    return n, L.TN_STATUS[st]
+
+
+# (mu, maxfun, branch the loop ends in); mu inside the spectrum of the
+# Hessian makes the CG system indefinite and reaches the other exits
+TN_CASES = [(0.31, 50, "CONV"), (0.31, 2, "NOTCONV"), (0.31, 1, "NOTCONV"), (40.0, 50, "WARN"),
+            (0.0, 50, "WARN : Change too small"), (0.9, 50, "WARN : Not reduce"), (0.75, 50, "FAILED")]
+
+
+@pytest.mark.parametrize("mu,maxfun,branch", TN_CASES)
+@pytest.mark.parametrize("chunk", [1, 3, 16])
+def test_tn_direction_matches_host_loop(mu, maxfun, branch, chunk):
+    """The device-resident CG loop takes the branches of the reference loop
+    (convergence, maxfun, negative curvature at the first and at a later
+    iteration, `Best` restored after a stall or a growing residual) and
+    returns its direction and iteration count."""
+    n, status = _tn_case(mu, maxfun, chunk)
+    assert status == branch
+
+
+@pytest.mark.parametrize("mu,branch", [(-0.3, "CONV"), (0.0, "FAILED"), (0.15, "WARN : Not reduce"), (40.0, "WARN")])
+def test_tn_direction_local_functional(mu, branch):
+    n, status = _tn_case(mu, 50, 3, local=True)
+    assert status == branch
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
