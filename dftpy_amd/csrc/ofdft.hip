@@ -55,7 +55,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kZero = 1e-30;  // constants.py:85
-constexpr int kRedThreads = 256;
+constexpr int kRedThreads = 1024;
 
 // ---------------------------------------------------------------------------
 // host-side pass schedule and twiddles
@@ -246,6 +246,14 @@ __device__ __forceinline__ double dev_div<double>(double a, double b) { return a
 template <>
 __device__ __forceinline__ float dev_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// sign(x) |x|^(2/3) in FP32 for x != 0: phi rho^(beta-1) at beta = 5/6 with
+// rho = phi^2, two transcendental instructions instead of the five of
+// sqrt, cbrt and the reciprocals.
+__device__ __forceinline__ float sgn_pow23(float x) {
+  const float a = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(__builtin_fabsf(x)) * (2.f / 3.f));
+  return x < 0.f ? -a : a;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -407,6 +415,7 @@ struct PreCopy {
   __device__ __forceinline__ In load(long long i) const { return {in[i]}; }
   __device__ __forceinline__ void apply(const In& x, T* out) const { out[0] = x.v; }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
 };
 
 // rho (mode 0) or phi(t) = c*phi + s*p (mode 1), then the convolution inputs
@@ -436,6 +445,7 @@ struct PreEval {
     return ph * ph;
   }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
   __device__ __forceinline__ void apply(const In& x, T* out) const {
     const T r = density(x);
     if (fp.fH >= 0) put3(out, fp.fH, r);
@@ -463,10 +473,22 @@ struct PreHvp {
   };
   __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i]}; }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
   __device__ __forceinline__ void apply(const In& x, T* out) const {
     const T ph = x.ph, pp = x.pp;
     const T r = ph * ph;
     const T dr = T(2) * ph * pp;
+    if constexpr (sizeof(T) == 4) {
+      // the same fields with sqrt(rho) = |phi| cancelled: drho / (2 sqrt(rho))
+      // = sign(phi) p, beta rho^(beta-1) drho = 2 beta sign(phi) |phi|^(2/3) p
+      if (fp.fH >= 0) put3(out, fp.fH, dr);
+      if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? (ph > T(0) ? pp : -pp) : T(0));
+      if (fp.fW >= 0) {
+        const T w = fp.beta56 ? T(2 * fp.beta) * sgn_pow23(ph) * pp : T(fp.beta) * dev_pow(r, T(fp.beta) - T(1)) * dr;
+        put3(out, fp.fW, r < T(kZero) ? T(0) : w);
+      }
+      return;
+    }
     const T s = dev_sqrt(r);
     if (fp.fH >= 0) put3(out, fp.fH, dr);
     if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dev_div(dr, T(2) * s) : T(0));
@@ -641,6 +663,23 @@ struct PostHvpFast {
   __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
     const T ph = in.ph, pp = in.pp;
     const T r = ph * ph;
+    if constexpr (sizeof(T) == 4) {
+      // phi * dv with sqrt(rho) = |phi| cancelled: phi (y/2) c_vW / sqrt(rho)
+      // = (y/2) sign(phi) c_vW, phi 2 alpha rho^(beta-1) = 2 alpha sign(phi) |phi|^(2/3)
+      T ap = in.w0 * pp;
+      if (fp.fH >= 0) ap += ph * pick3(c, fp.fH);
+      if (fp.fV >= 0 && r > T(0)) {
+        const T cv = T(0.5 * fp.y) * pick3(c, fp.fV);
+        ap += ph > T(0) ? cv : -cv;
+      }
+      if (fp.fW >= 0 && r >= T(kZero)) {
+        const T g = fp.beta56 ? sgn_pow23(ph) : ph * dev_pow(r, T(fp.beta) - T(1));
+        ap += T(2 * fp.alpha) * g * pick3(c, fp.fW);
+      }
+      Ap[i] = ap;
+      acc[0] += (double)(pp * ap);
+      return;
+    }
     T dv = T(0);
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
     if (fp.fV >= 0 || fp.fW >= 0) {
@@ -740,8 +779,8 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
         const int lp = t / NN, j = t - lp * NN;
         const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
         T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-        if (r0 < nrows) pre.apply(xa[u], a);
-        if (r1 < nrows) pre.apply(xb[u], b);
+        if (r0 < nrows) pre.apply(xa[u], a, r0 * NN + j);
+        if (r1 < nrows) pre.apply(xb[u], b, r1 * NN + j);
 #pragma unroll
         for (int f = 0; f < 3; ++f)
           if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
@@ -1145,14 +1184,30 @@ __global__ void k_wt_table(T* __restrict__ out, GridGeom geo, double tkf, double
   }
 }
 
+// This thread's share of sum_b partials[b * NS + s]: four independent
+// accumulators keep four loads in flight (the partials were written by
+// blocks on every XCD, so most loads miss this XCD's L2).
+template <int NS>
+__device__ __forceinline__ double thread_partials(const double* __restrict__ partials, long long nblk, int s) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const long long st = blockDim.x;
+  long long b = threadIdx.x;
+  for (; b + 3 * st < nblk; b += 4 * st) {
+    a0 += partials[b * NS + s];
+    a1 += partials[(b + st) * NS + s];
+    a2 += partials[(b + 2 * st) * NS + s];
+    a3 += partials[(b + 3 * st) * NS + s];
+  }
+  for (; b < nblk; b += st) a0 += partials[b * NS + s];
+  return (a0 + a1) + (a2 + a3);
+}
+
 template <int NS>
 __global__ void k_reduce_partials(const double* __restrict__ partials, long long nblk, double* __restrict__ out,
                                   double* __restrict__ hout, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   const int s = blockIdx.x;
-  double acc = 0.0;
-  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[b * NS + s];
-  double tmp[1] = {acc};
+  double tmp[1] = {thread_partials<NS>(partials, nblk, s)};
   block_sums<1>(tmp, out + s);
   // thread 0 wrote out[s]; the mapped host copy saves a device-to-host copy
   if (hout && threadIdx.x == 0) hout[s] = out[s];
@@ -1370,9 +1425,7 @@ __device__ void tn_ctrl(TnState* st, double pAp, double r1) {
 __global__ void k_tn_reduce_ctrl(TnState* st, const double* __restrict__ partials, long long nblk,
                                  const double* __restrict__ red, double* __restrict__ rr) {
   if (st->done) return;
-  double acc = 0.0;
-  for (long long b = threadIdx.x; b < nblk; b += blockDim.x) acc += partials[b];
-  double tmp[1] = {acc};
+  double tmp[1] = {thread_partials<1>(partials, nblk, 0)};
   block_sums<1>(tmp, rr);
   if (threadIdx.x == 0) tn_ctrl(st, red[0], rr[0]);
 }
@@ -1411,18 +1464,21 @@ struct PreHvpCG {
   const T* __restrict__ r;
   const TnState* __restrict__ st;
   __host__ __device__ int nin() const { return 4; }
-  using In = typename PreHvp<T>::In;
-  __device__ __forceinline__ In load(long long i) const {
-    const T ph = phi[i];
-    T pp = p[i];
+  // raw loads only, so that the row kernel keeps every load of its points in
+  // flight at once; the update and its store happen in apply
+  struct In {
+    T ph, pp, rr;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i], r[i]}; }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long i) const {
+    T pp = x.pp;
     if (st->act & kActNext) {
-      pp = r[i] + T(st->beta) * pp;
+      pp = x.rr + T(st->beta) * pp;
       p[i] = pp;
     }
-    return {ph, pp};
+    h.apply(typename PreHvp<T>::In{x.ph, pp}, out);
   }
-  __device__ __forceinline__ void operator()(long long i, T* out) const { h.apply(load(i), out); }
-  __device__ __forceinline__ void apply(const In& x, T* out) const { h.apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out, i); }
 };
 
 // The five sums one outer iteration needs after the direction is found:

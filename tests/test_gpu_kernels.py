@@ -185,6 +185,40 @@ def test_hvp_fast_equals_hvp(prec):
     assert abs(r2 - np.sum(ref ** 2)) <= 1e-4 * r2
 
 
+def test_hvp_fast_fp32_matches_fp64_with_signed_phi():
+    """The FP32 product (sqrt(rho) = |phi| cancelled in the functors) against
+    the FP64 one on a phi with negative and zero entries, as a line search
+    can produce: both are the reference's formula on rho = phi^2."""
+    wl, grid, ions, pp = _al4()
+    rho = wl["rho_ini"]
+    rng = np.random.default_rng(8)
+    phi = np.sqrt(rho) * np.where(rng.random(rho.shape) < 0.1, -1.0, 1.0)
+    phi.flat[rng.integers(0, phi.size, 20)] = 0.0
+    p = rng.standard_normal(rho.shape) * np.abs(phi).mean() * 0.1
+    out = {}
+    for prec in ("fp64", "fp32"):
+        P = _plan(wl["nr"], wl["cell"], prec)
+        P.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, pseudo=True, rho0=rho.mean())
+        P.set_vloc(torch.tensor(pp.vreal, dtype=P.dtype, device=P.device))
+        t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
+        V = P.empty()
+        aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+        P.eval_phi(t(np.abs(phi)), None, 1.0, 0.0, V, aux)
+        res, W0, A = P.empty(), P.empty(), P.empty()
+        P.hvp_prepare(t(phi), V, aux, 0.31, res, W0)
+        q = P.hvp_fast(t(phi), t(p), W0, A)
+        A1 = P.empty()
+        P.hvp(t(phi), t(p), V, aux, 0.31, A1)
+        out[prec] = (A.cpu().double().numpy(), q, A1.cpu().double().numpy())
+    a64, q64, h64 = out["fp64"]
+    a32, q32, h32 = out["fp32"]
+    assert np.abs(a64 - h64).max() <= 1e-12 * np.abs(a64).max()
+    err = np.abs(a32 - a64).max() / np.abs(a64).max()
+    print("FP32 hvp_fast relative error %.3e" % err)
+    assert err < 2e-5
+    assert abs(q32 - q64) <= 2e-5 * abs(q64)
+
+
 def _tn_host(hvp, res0, maxfun):
     """get_direction_TN (optimization.py:92-148) on the host, step for step,
     with Ap from the given product; `Best` aliases `direction` until the
