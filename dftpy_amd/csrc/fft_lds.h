@@ -211,19 +211,37 @@ struct Sched {
   int r[kMaxPass];
 };
 
-constexpr Sched make_sched(int n) {
+// Radix counts as 8s, then one 4 or 2, then 3s and 5s.  `asc`: the passes
+// run in ascending radix order (the first pass stores at a stride of R
+// points, and a small R there keeps its LDS bank conflicts low); otherwise
+// 8s, 4, 2, 3s, 5s.  FP64 keeps the second order: its rounding, and so the
+// forward-difference HVP trajectories built on it, stay as they are.
+constexpr Sched make_sched(int n, bool asc) {
   Sched s{0, {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}};
-  int m = n;
-  while (m % 8 == 0) { s.r[s.np++] = 8; m /= 8; }
-  while (m % 4 == 0) { s.r[s.np++] = 4; m /= 4; }
-  while (m % 2 == 0) { s.r[s.np++] = 2; m /= 2; }
-  while (m % 3 == 0) { s.r[s.np++] = 3; m /= 3; }
-  while (m % 5 == 0) { s.r[s.np++] = 5; m /= 5; }
+  int m = n, c8 = 0, c4 = 0, c2 = 0, c3 = 0, c5 = 0;
+  while (m % 8 == 0) { ++c8; m /= 8; }
+  while (m % 4 == 0) { ++c4; m /= 4; }
+  while (m % 2 == 0) { ++c2; m /= 2; }
+  while (m % 3 == 0) { ++c3; m /= 3; }
+  while (m % 5 == 0) { ++c5; m /= 5; }
+  if (asc) {
+    for (int i = 0; i < c3; ++i) s.r[s.np++] = 3;
+    for (int i = 0; i < c5; ++i) s.r[s.np++] = 5;
+    for (int i = 0; i < c2; ++i) s.r[s.np++] = 2;
+    for (int i = 0; i < c4; ++i) s.r[s.np++] = 4;
+    for (int i = 0; i < c8; ++i) s.r[s.np++] = 8;
+  } else {
+    for (int i = 0; i < c8; ++i) s.r[s.np++] = 8;
+    for (int i = 0; i < c4; ++i) s.r[s.np++] = 4;
+    for (int i = 0; i < c2; ++i) s.r[s.np++] = 2;
+    for (int i = 0; i < c3; ++i) s.r[s.np++] = 3;
+    for (int i = 0; i < c5; ++i) s.r[s.np++] = 5;
+  }
   return s;
 }
 
 constexpr int sched_maxnb(int n) {
-  Sched s = make_sched(n);
+  Sched s = make_sched(n, false);
   int mx = 0;
   for (int i = 0; i < s.np; ++i) mx = (n / s.r[i] > mx) ? n / s.r[i] : mx;
   return mx;
@@ -232,7 +250,7 @@ constexpr int sched_maxnb(int n) {
 template <typename T, int N, int P, int NS, int NTR, int TH>
 __device__ __forceinline__ void fixed_pass(cplx<T>* lds, int ld, int ntr, const cplx<T>* __restrict__ tw,
                                            bool inv) {
-  constexpr Sched S = make_sched(N);
+  constexpr Sched S = make_sched(N, sizeof(T) == 4);
   if constexpr (P < S.np) {
     constexpr int R = S.r[P];
     constexpr int nb = N / R;

@@ -60,7 +60,8 @@ constexpr int kRedThreads = 1024;
 // ---------------------------------------------------------------------------
 // host-side pass schedule and twiddles
 // ---------------------------------------------------------------------------
-bool make_pass(int n, Pass1D* pl) {
+// `asc`: passes in ascending radix order (FP32 plans), as make_sched.
+bool make_pass(int n, Pass1D* pl, bool asc) {
   pl->n = n;
   pl->npass = 0;
   int m = n;
@@ -69,11 +70,18 @@ bool make_pass(int n, Pass1D* pl) {
     pl->radix[pl->npass++] = r;
     return true;
   };
-  while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
-  while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
-  while (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
-  while (m % 3 == 0) { if (!push(3)) return false; m /= 3; }
-  while (m % 5 == 0) { if (!push(5)) return false; m /= 5; }
+  int c8 = 0, c4 = 0, c2 = 0, c3 = 0, c5 = 0;
+  while (m % 8 == 0) { ++c8; m /= 8; }
+  while (m % 4 == 0) { ++c4; m /= 4; }
+  while (m % 2 == 0) { ++c2; m /= 2; }
+  while (m % 3 == 0) { ++c3; m /= 3; }
+  while (m % 5 == 0) { ++c5; m /= 5; }
+  const int order[5] = {asc ? 3 : 8, asc ? 5 : 4, 2, asc ? 4 : 3, asc ? 8 : 5};
+  for (int r : order) {
+    const int c = r == 8 ? c8 : r == 4 ? c4 : r == 2 ? c2 : r == 3 ? c3 : c5;
+    for (int i = 0; i < c; ++i)
+      if (!push(r)) return false;
+  }
   return m == 1;
 }
 
@@ -2715,7 +2723,7 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   P->rs = precision == OFDFT_F64 ? sizeof(double) : sizeof(float);
   P->stream = reinterpret_cast<hipStream_t>(hip_stream);
   for (int i = 0; i < 3; ++i) {
-    if (!make_pass(P->n[i], &P->pl[i])) {
+    if (!make_pass(P->n[i], &P->pl[i], P->prec == OFDFT_F32)) {
       delete P;
       return fail(OFDFT_EINVAL, "grid sizes must be products of 2, 3 and 5 (at most 12 passes)");
     }
