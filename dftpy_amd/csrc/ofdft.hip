@@ -1579,10 +1579,16 @@ template <typename T>
 struct RegTile {
   static constexpr int W = 16;
 };
-constexpr int kRowsTH = 256;
+// threads of a rows_* workgroup for a compile-time length N: for N = 96
+// and 192 (2 x 3 row pairs), 192 threads keep the load, radix-3 and
+// point passes fully occupied where 256 would leave a quarter idle
+template <int N>
+constexpr int rows_th() {
+  return (N == 96 || N == 192) ? 192 : 256;
+}
 constexpr int kColsTH = 256;
 template <typename T, int N>
-using RowsFFT = FFTFixed<N, kRowPB<T> * 3, kRowsTH>;
+using RowsFFT = FFTFixed<N, kRowPB<T> * 3, rows_th<N>()>;
 template <int N>
 using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
@@ -1609,7 +1615,7 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   ProfScope ps(P, Pre::kProf, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
   switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
-    case NN: go_rows_fwd<T>(P, pre, RowsFFT<T, NN>{}, F, grid, kRowsTH, lds); break;
+    case NN: go_rows_fwd<T>(P, pre, RowsFFT<T, NN>{}, F, grid, rows_th<NN>(), lds); break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
     default: {
@@ -1652,7 +1658,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
     ProfScope ps(P, Post::kProf, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
     switch (P->n[2]) {
 #define OFDFT_CASE(NN) \
-      case NN: go_rows_inv<T>(P, post, RowsFFT<T, NN>{}, F, src, grid, kRowsTH, lds, part); break;
+      case NN: go_rows_inv<T>(P, post, RowsFFT<T, NN>{}, F, src, grid, rows_th<NN>(), lds, part); break;
       OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
       default: {
