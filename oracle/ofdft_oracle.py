@@ -34,6 +34,11 @@ rec PME)                    forces of the perfect fcc fixture and exact vs PME
                             agreement (examples/test/test_ewald.py:37-50)
 optimize_rho (LBFGS)        converges to the TN energy (parity unpinned
                             beyond that: no reference test runs LBFGS)
+stress: tf, vw, wt (with    (1/Omega) dE/d(strain) of the oracle's own energy
+lindhard_derivative), lda,  under a homogeneous strain, central difference,
+hartree, LocalPseudo        every term; test_pme.py:35-36 and
+(exact, PME), Ewald (real,  test_ewald.py:51-53 exact-vs-PME agreement on
+rec, rec PME)               the Al_fde_rho.pp fixture
 ==========================  ==================================================
 
 Unit constants: ase's CODATA-2014 values Bohr = 0.5291772105638411 Angstrom,
@@ -55,7 +60,8 @@ __all__ = [
     "lda_functional", "lda_v2rho2", "hartree_functional", "PseudoTable", "read_recpot",
     "empty_core_table", "LocalPseudo", "Ewald", "TotalFunctional",
     "Optimization", "read_qepp", "read_poscar", "ecut2nr", "best_fft_size",
-    "cbspline_Mn",
+    "cbspline_Mn", "tf_stress", "vw_stress", "wt_stress", "lda_stress", "hartree_stress",
+    "lindhard_derivative",
 ]
 
 
@@ -116,6 +122,15 @@ class Grid:
         gg = self.gg.copy()
         gg[0, 0, 0] = 1.0
         inv = 1.0 / gg
+        inv[0, 0, 0] = 0.0
+        return inv
+
+    @property
+    def invq(self):
+        """grid.py:543-553 (1/|G|, 0 at G = 0)."""
+        q = self.q.copy()
+        q[0, 0, 0] = 1.0
+        inv = 1.0 / q
         inv[0, 0, 0] = 0.0
         return inv
 
@@ -381,6 +396,96 @@ def hartree_functional(rho, grid, calc=("E", "V")):
     V = ifft(fft(rho, grid) * grid.invgg * 4 * np.pi, grid)
     E = np.einsum("ijk,ijk->", V, rho) * grid.dV / 2.0 if "E" in calc else None
     return E, V
+
+
+# ---------------------------------------------------------------------------
+# stress of the density functionals (3x3, Ha/bohr^3)
+# ---------------------------------------------------------------------------
+_PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def tf_stress(E_tf, grid):
+    """ThomasFermiStress: kedf/tf.py:52-62 (-2/3 E / Omega on the diagonal)."""
+    return np.eye(3) * (-2.0 / 3.0 * E_tf / grid.volume)
+
+
+def vw_stress(rho, grid, y=1.0):
+    """vonWeizsackerStress: kedf/vw.py:97-112; gradients of rho by FFT,
+    -1/4 sum d_i rho d_j rho / rho dV / Omega."""
+    g = grid.g
+    rhoG = fft(rho, grid)
+    d = [ifft(1j * g[i] * rhoG, grid) for i in range(3)]
+    out = np.zeros((3, 3))
+    for i, j in _PAIRS:
+        e = -0.25 / grid.volume * grid.dV * np.einsum("ijk->", d[i] * d[j] / rho)
+        out[i, j] = out[j, i] = e * y
+    return out
+
+
+def lindhard_derivative(eta, mu):
+    """LindhardDerivative: kedf/kernel.py:197-211 (eta dF/deta with the
+    reference's three branches)."""
+    eta = np.asarray(eta, dtype=np.float64)
+    out = np.zeros_like(eta)
+    atol = 1.0e-10
+    c0 = (eta > atol) & (np.abs(eta - 1.0) > atol)
+    c1 = eta < atol
+    c2 = np.abs(eta - 1.0) < atol
+    e = eta[c0]
+    ta = np.log(np.abs((1.0 + e) / (1.0 - e)))
+    out[c0] = (0.5 / e - 0.25 * (e ** 2 + 1.0) / e ** 2 * ta) / (0.5 + 0.25 * (1 - e ** 2) / e * ta) ** 2 \
+        + 6.0 * e * mu
+    out[c1] = -2.0 * eta[c1] * (1.0 / 3.0 - 3.0 * mu)
+    out[c2] = -48
+    return out * eta
+
+
+def wt_stress(rho, grid, E_nl, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0):
+    """WTStress: kedf/wt.py:75-106 (rho0 = mean of rho, mu = y in the
+    Lindhard derivative)."""
+    rho0 = rho.mean()
+    g = grid.g
+    m = grid.mask
+    tkf = float(2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0))
+    A = fft(rho ** alpha, grid) / grid.volume
+    B = np.conjugate(fft(rho ** beta, grid)) / grid.volume
+    dd = lindhard_derivative(grid.q / tkf, y) * A * B
+    ig = grid.invgg
+    out = np.zeros((3, 3))
+    for i, j in _PAIRS:
+        fac = 1.0 / 3.0 if i == j else 0.0
+        den = (g[i][m] * g[j][m] * ig[m] - fac) * dd[m]
+        out[i, j] = out[j, i] = np.einsum("i->", den).real
+    out *= np.pi ** 2 / (alpha * beta * rho0 ** (alpha + beta - 2) * tkf / 2.0)
+    for i in range(3):
+        out[i, i] -= 2.0 / 3.0 * E_nl / grid.volume
+    return out
+
+
+def lda_stress(rho, grid):
+    """XC.stress -> _LDAStress: xc/__init__.py:72-87, xc/semilocal_xc.py:343-346:
+    (E_xc - sum V_xc rho dV) / Omega on the diagonal."""
+    E, V = lda_functional(rho, grid)
+    P = E - np.sum(V * rho) * grid.dV
+    return np.eye(3) * P / grid.volume
+
+
+def hartree_stress(rho, grid, E_h=None):
+    """HartreeFunctionalStress: hartree.py:68-90."""
+    if E_h is None:
+        E_h = hartree_functional(rho, grid, calc=("E",))[0]
+    g = grid.g
+    m = grid.mask
+    ig = grid.invgg
+    rhoG = fft(rho, grid)
+    r2 = rhoG * np.conjugate(rhoG) * ig * ig
+    out = np.zeros((3, 3))
+    for i, j in _PAIRS:
+        e = np.sum(g[i][m] * g[j][m] * r2[m])
+        out[i, j] = out[j, i] = e.real * 8.0 * np.pi / grid.volume ** 2
+        if i == j:
+            out[i, j] -= E_h / grid.volume
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -668,6 +773,65 @@ class LocalPseudo:
         return F
 
 
+    def vline_deriv(self, key):
+        """_PP_Derivative_One: pseudo/__init__.py:454-460 (dv/dq of the
+        cubic spline, 0 beyond the table)."""
+        q = self.grid.q
+        t = self.tables[key]
+        out = np.zeros_like(q)
+        m = q < np.max(t.gp)
+        out[m] = splev(q[m], t.tck, der=1)
+        return out
+
+    def stress(self, rho, energy=None):
+        """LocalPseudo.stress: pseudo/__init__.py:248-258."""
+        if energy is None:
+            energy = self.compute(rho, calc=("E",))[0]
+        return self._stress_pme(rho, energy) if self.PME else self._stress_exact(rho, energy)
+
+    def _stress_exact(self, rho, energy):
+        """_Stress: pseudo/__init__.py:462-503."""
+        g = self.grid
+        rhoG = fft(rho, g)
+        vd = np.zeros(tuple(g.nrG), dtype=np.complex128)
+        for key in sorted(self.tables):
+            d = self.vline_deriv(key)
+            for i, s in enumerate(self.ions.symbols):
+                if s == key:
+                    vd += d * self.strf(i, +1.0)
+        rv = rhoG * vd * g.invq
+        m = g.mask
+        gv = g.g
+        out = np.zeros((3, 3))
+        for i, j in _PAIRS:
+            out[i, j] = out[j, i] = -(np.einsum("i->", gv[i][m] * gv[j][m] * rv[m])).real / g.volume * 2.0
+            if i == j:
+                out[i, j] -= energy
+        return out / g.volume
+
+    def _stress_pme(self, rho, energy):
+        """_StressPME: pseudo/__init__.py:573-602 (one Q-array per species)."""
+        g = self.grid
+        rhoGB = np.conjugate(fft(rho, g)) * self.bspline.Barray
+        m = g.mask
+        gv = g.g
+        iq = g.invq
+        sc = self.ions.get_scaled_positions()
+        out = np.zeros((3, 3))
+        for key in self.ions.symbols_uniq:
+            idx = [i for i, s in enumerate(self.ions.symbols) if s == key]
+            Q = self.bspline.spread(sc[idx], np.ones(len(idx)))
+            rv = rhoGB * self.vline_deriv(key) * fft(Q, g)
+            for i, j in _PAIRS:
+                out[i, j] -= (np.einsum("i->", gv[i][m] * gv[j][m] * rv[m] * iq[m])).real / g.volume ** 2
+        out *= 2.0 * g.nnr
+        for i in range(3):
+            for j in range(i, 3):
+                out[j, i] = out[i, j]
+            out[i, i] -= energy
+        return out / g.volume
+
+
 class Ewald:
     """Ion-ion energy: ewald.py:197-278 (eta), :280-322 (real-space sum),
     :448-482 (reciprocal and correction terms), :785-807 (PME reciprocal)."""
@@ -821,6 +985,71 @@ class Ewald:
         """ewald.py:509-519."""
         return self.forces_real() + (self.forces_rec_pme() if self.PME else self.forces_rec())
 
+    def stress_real(self):
+        """Stress_real: ewald.py:609-662; -1/(2 Omega) sum_(i,j,T) q_i q_j
+        [2 sqrt(eta/pi) exp(-eta d^2) + erfc(sqrt(eta) d)/d] d_a d_b / d^2."""
+        lat = self.grid.lattice
+        L = np.sqrt((lat ** 2).sum(axis=1))
+        rmax = sp.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+        N = np.ceil(rmax / L).astype(int)
+        T = np.mgrid[-N[0]:N[0] + 1, -N[1]:N[1] + 1, -N[2]:N[2] + 1].reshape(3, -1).T.astype(np.float64)
+        shifts = T @ lat
+        pos = self.ions.positions
+        q = self.ions.charges
+        es = np.sqrt(self.eta)
+        img = (pos[None, :, :] - shifts[:, None, :]).reshape(-1, 3)
+        qq = np.tile(q, len(shifts))
+        acc = np.zeros(6)
+        for ia in range(self.ions.nat):
+            dvec = pos[ia] - img
+            d = np.sqrt((dvec ** 2).sum(axis=1))
+            m = (d < rmax) & (d > 0.001)
+            dv, dm = dvec[m], d[m]
+            w = qq[m] * (2 * es / np.sqrt(np.pi) * np.exp(-self.eta * dm ** 2) + sp.erfc(es * dm) / dm)
+            for k, (i, j) in enumerate(_PAIRS):
+                acc[k] += q[ia] * np.sum(w * dv[:, i] * dv[:, j] / dm ** 2)
+        acc *= -0.5 / self.grid.volume
+        out = np.zeros((3, 3))
+        for k, (i, j) in enumerate(_PAIRS):
+            out[i, j] = out[j, i] = acc[k]
+        return out
+
+    def _stress_rec_sum(self, S2, scale):
+        g = self.grid
+        m = g.mask
+        gg = g.gg[m]
+        ig = g.invgg[m]
+        w = S2[m] * np.exp(-gg / (4.0 * self.eta)) * ig
+        out = np.zeros((3, 3))
+        for i, j in _PAIRS:
+            sf = g.g[i][m] * g.g[j][m] * 2.0 * ig * (1 + gg / (4.0 * self.eta))
+            if i == j:
+                sf = sf - 1.0
+            out[i, j] = out[j, i] = np.sum(w * sf).real * scale
+        s_g0 = self.ions.get_ncharges() ** 2 * 4.0 * np.pi * (1.0 / (4.0 * self.eta * g.volume ** 2) / 2.0)
+        return out + np.eye(3) * s_g0
+
+    def stress_rec(self):
+        """Stress_rec: ewald.py:719-759 (exact structure factor)."""
+        g = self.grid
+        S = np.zeros(tuple(g.nrG), dtype=np.complex128)
+        for i in range(self.ions.nat):
+            S += np.exp(-1j * np.einsum("lijk,l->ijk", g.g, self.ions.positions[i])) * self.ions.charges[i]
+        return self._stress_rec_sum(np.conjugate(S) * S, 4.0 * np.pi / g.volume ** 2)
+
+    def stress_rec_pme(self):
+        """Stress_rec_PME: ewald.py:915-958 (2 x the half-space sum)."""
+        g = self.grid
+        bs = CBspline(g, self.order)
+        Q = bs.spread(self.ions.get_scaled_positions(), self.ions.charges)
+        S = fft(Q, g) * bs.Barray
+        return self._stress_rec_sum(np.conjugate(S) * S, 2.0 * 2.0 * np.pi / g.volume ** 2 / g.dV ** 2)
+
+    @property
+    def stress(self):
+        """ewald.py:523-538."""
+        return self.stress_real() + (self.stress_rec_pme() if self.PME else self.stress_rec())
+
 
 # ---------------------------------------------------------------------------
 # total functional
@@ -884,6 +1113,37 @@ class TotalFunctional:
                 out["II"] = self.ewald.energy
             return E, V, out
         return E, V
+
+    def get_stress(self, rho, split=False):
+        """TotalFunctional.get_stress: total_functional.py:176-209, each term
+        with its energy at ``rho``; the KEDF parts keyed KEDF-TF, KEDF-VW,
+        KEDF-NL (kedf/__init__.py:193-235)."""
+        g = self.grid
+        o = self.opts
+        _, _, parts = self.compute(rho, calc=("E",), split=True)
+        st = {}
+        if "TF" in self.knames:
+            st["KEDF-TF"] = tf_stress(parts["KEDF-TF"], g)
+        if "VW" in self.knames:
+            st["KEDF-VW"] = vw_stress(rho, g, y=o["y"])
+        if "WT-NL" in self.knames:
+            st["KEDF-NL"] = wt_stress(rho, g, parts["KEDF-NL"], y=o["y"], alpha=o["alpha"], beta=o["beta"])
+        if self.xc:
+            st["XC"] = lda_stress(rho, g)
+        if self.hartree:
+            st["HARTREE"] = hartree_stress(rho, g, parts["HARTREE"])
+        if self.pseudo is not None:
+            st["PSEUDO"] = self.pseudo.stress(rho, energy=parts["PSEUDO"])
+        if self.ewald is not None:
+            st["II"] = self.ewald.stress
+        tot = np.zeros((3, 3))
+        for v in st.values():
+            tot = tot + v
+        for i in range(1, 3):
+            for j in range(i - 1, -1, -1):
+                tot[i, j] = tot[j, i]
+        st["TOTAL"] = tot
+        return st if split else tot
 
 
 # ---------------------------------------------------------------------------

@@ -309,3 +309,87 @@ def test_lbfgs_reaches_tn_minimum():
         opt.optimize_rho(rho0)
         E[m] = opt.energy
     assert abs(E["TN"] - E["LBFGS"]) < 1e-9
+
+
+def test_stress_pme_vs_exact_fixture(fixture_rho):
+    """examples/test/test_pme.py:35-36 and test_ewald.py:51-53: the exact and
+    PME forms of the local-PP and Ewald stress agree."""
+    ions, rho, grid = fixture_rho
+    tab = O.read_recpot(os.path.join(DATA, "Al_lda.oe01.recpot"))
+    a = O.LocalPseudo(grid, ions, {"Al": tab}, PME=False)
+    b = O.LocalPseudo(grid, ions, {"Al": tab}, PME=True)
+    sa, sb = a.stress(rho), b.stress(rho)
+    assert np.allclose(sa, sb, atol=1e-4)
+    assert np.abs(sa).max() > 1e-4  # not trivially zero
+    ea, eb = O.Ewald(grid, ions, PME=False).stress, O.Ewald(grid, ions, PME=True).stress
+    assert np.allclose(ea, eb, atol=1e-5)
+    assert np.abs(ea).max() > 1e-4
+
+
+def _strained_al4(eps, rho, wl):
+    """The fcc Al cell under the homogeneous strain eps: lattice rows and
+    positions times (1 + eps), density values divided by det(1 + eps) (same
+    fractional grid, same electron count)."""
+    F = np.eye(3) + eps
+    L = wl["cell"] @ F
+    g = O.Grid(L, wl["nr"])
+    ions = O.Ions(wl["symbols"], wl["positions"] @ F, L)
+    return g, ions, rho / np.linalg.det(F)
+
+
+def test_stress_is_strain_derivative_of_energy():
+    """Every term's stress equals (1/Omega) dE/d(eps_ab) of the same oracle's
+    energy under a homogeneous strain (central difference, h = 1e-5), for a
+    diagonal and an off-diagonal component: TF, vW, WT, LDA, Hartree, local
+    PP (exact and PME) and Ewald (exact and PME, eta held fixed)."""
+    from dftpy_amd import workloads as W
+    wl = W.build("al4_32")
+    n = wl["nr"][0]
+    vol = abs(np.linalg.det(wl["cell"]))
+    i = np.arange(n) / n
+    x, y, z = np.meshgrid(i, i, i, indexing="ij")
+    rng = np.random.default_rng(4)
+    f = np.ones((n, n, n))
+    for _ in range(6):
+        k = rng.integers(-3, 4, size=3)
+        f += 0.05 * np.cos(2 * np.pi * (k[0] * x + k[1] * y + k[2] * z) + rng.uniform(0, 2 * np.pi))
+    rho = f * 12.0 / vol
+    tabs = {"Al": O.empty_core_table(3.0, 1.12)}
+    g0, ions0, r0 = _strained_al4(np.zeros((3, 3)), rho, wl)
+    ions0.set_charges({"Al": tabs["Al"].zval})
+    eta = O.Ewald(g0, ions0, PME=True).eta
+
+    def energies(eps):
+        g, ions, r = _strained_al4(eps, rho, wl)
+        out = {"TF": O.tf_functional(r, g)[0], "VW": O.vw_functional(r, g)[0], "NL": O.wt_functional(r, g)[0],
+               "XC": O.lda_functional(r, g)[0], "H": O.hartree_functional(r, g)[0]}
+        for pme in (True, False):
+            out["PP%d" % pme] = O.LocalPseudo(g, ions, tabs, PME=pme).compute(r)[0]
+            ew = O.Ewald(g, ions, PME=pme)
+            ew.eta = eta
+            out["II%d" % pme] = ew.energy
+        return out
+
+    E0 = energies(np.zeros((3, 3)))
+    st = {"TF": O.tf_stress(E0["TF"], g0), "VW": O.vw_stress(r0, g0), "NL": O.wt_stress(r0, g0, E0["NL"]),
+          "XC": O.lda_stress(r0, g0), "H": O.hartree_stress(r0, g0, E0["H"])}
+    for pme in (True, False):
+        st["PP%d" % pme] = O.LocalPseudo(g0, ions0, tabs, PME=pme).stress(r0)
+        ew = O.Ewald(g0, ions0, PME=pme)
+        st["II%d" % pme] = ew.stress
+    h = 1e-5
+    for a, b in ((0, 0), (1, 2)):
+        e = np.zeros((3, 3))
+        e[a, b] += 0.5 * h
+        e[b, a] += 0.5 * h
+        Ep, Em = energies(e), energies(-e)
+        for k, s in st.items():
+            fd = (Ep[k] - Em[k]) / (2 * h) / vol
+            assert abs(s[a, b] - fd) <= 1e-7 * np.abs(s).max() + 1e-13, (k, a, b, s[a, b], fd)
+    # the symmetrised total of get_stress is the sum of the terms
+    ew = O.Ewald(g0, ions0, PME=True)
+    tot = O.TotalFunctional(g0, kedf="WT", pseudo=O.LocalPseudo(g0, ions0, tabs, PME=True), ewald=ew)
+    parts = tot.get_stress(r0, split=True)
+    assert set(parts) == {"KEDF-TF", "KEDF-VW", "KEDF-NL", "XC", "HARTREE", "PSEUDO", "II", "TOTAL"}
+    ref = st["TF"] + st["VW"] + st["NL"] + st["XC"] + st["H"] + st["PP1"] + st["II1"]
+    assert np.abs(parts["TOTAL"] - ref).max() <= 1e-12 * np.abs(ref).max()
