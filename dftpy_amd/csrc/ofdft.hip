@@ -523,6 +523,47 @@ struct PostStore {
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, In{}, c, acc); }
 };
 
+// rho^a (numpy power: rho ** a) as the row transform's input, for the WT
+// stress transforms (wt.py:86-87)
+template <typename T>
+struct PrePow {
+  static constexpr int kProf = PC_ROWS_FWD;
+  const T* __restrict__ rho;
+  T a;
+  __host__ __device__ int nin() const { return 1; }
+  struct In {
+    T v;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {rho[i]}; }
+  __device__ __forceinline__ void apply(const In& x, T* out) const { out[0] = pow(x.v, a); }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+};
+
+// the six products d_a rho d_b rho / rho of the three gradient fields
+// (vonWeizsackerStress, kedf/vw.py:97-112); c[a] = d_a rho at point i
+template <typename T>
+struct PostStressVW {
+  static constexpr int NS = 6;
+  static constexpr int kProf = PC_ROWS_INV;
+  const T* __restrict__ rho;
+  __host__ __device__ int nio() const { return 1; }
+  struct In {
+    T r;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {rho[i]}; }
+  __device__ __forceinline__ void apply(long long, const In& in, const T* c, double* acc) const {
+    const double inv = 1.0 / (double)in.r;
+    acc[0] += (double)c[0] * c[0] * inv;
+    acc[1] += (double)c[0] * c[1] * inv;
+    acc[2] += (double)c[0] * c[2] * inv;
+    acc[3] += (double)c[1] * c[1] * inv;
+    acc[4] += (double)c[1] * c[2] * inv;
+    acc[5] += (double)c[2] * c[2] * inv;
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+};
+
 template <typename T>
 struct PostEval {
   static constexpr int NS = OFDFT_NSUM;
@@ -1945,6 +1986,7 @@ int reduce_blocks(ofdft_plan_s* P, int nblk, int ns, double* out) {
     case 1: hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
     case 3: hipLaunchKernelGGL((k_reduce_partials<3>), dim3(3), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
     case 5: hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
+    case 6: hipLaunchKernelGGL((k_reduce_partials<6>), dim3(6), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nblk, P->red, P->dhred, P->skip); break;
     default: return fail(OFDFT_EINVAL, "unsupported number of sums");
   }
   LAUNCHCHK();
@@ -2297,6 +2339,38 @@ __global__ void k_ewald_real(const double* __restrict__ pos, const double* __res
   block_sums<1>(acc, partials + blockIdx.x);
 }
 
+// Ewald real-space stress sums (Stress_real, ewald.py:609-662): one thread
+// per ordered pair (i, j) over the image shifts,
+// q_i q_j [2 sqrt(eta/pi) exp(-eta d^2) + erfc(sqrt(eta) d)/d] d_a d_b / d^2.
+__global__ void k_ewald_stress_real(const double* __restrict__ pos, const double* __restrict__ q, int nat,
+                                    const double* __restrict__ shifts, int nshift, double rmax, double eta,
+                                    double* __restrict__ partials) {
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double es = sqrt(eta);
+  const long long npair = (long long)nat * nat;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < npair;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t / nat), j = (int)(t % nat);
+    const double qq = q[i] * q[j];
+    for (int m = 0; m < nshift; ++m) {
+      const double dx = pos[3 * i] - pos[3 * j] + shifts[3 * m];
+      const double dy = pos[3 * i + 1] - pos[3 * j + 1] + shifts[3 * m + 1];
+      const double dz = pos[3 * i + 2] - pos[3 * j + 2] + shifts[3 * m + 2];
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const double d = sqrt(d2);
+      if (!(d < rmax && d > 0.001)) continue;
+      const double w = qq * (1.1283791670955125739 * es * exp(-eta * d2) + erfc(es * d) / d) / d2;
+      acc[0] += w * dx * dx;
+      acc[1] += w * dx * dy;
+      acc[2] += w * dx * dz;
+      acc[3] += w * dy * dy;
+      acc[4] += w * dy * dz;
+      acc[5] += w * dz * dz;
+    }
+  }
+  block_sums<6>(acc, partials + 6 * (long long)blockIdx.x);
+}
+
 // Ewald real-space forces (ewald.py:541-578), one workgroup per atom i:
 // sum over atoms j and image shifts T of q_j (R_i - R_j + T) w(x),
 // w = erfc(x)/x^3 + 2/sqrt(pi) exp(-x^2)/x^2, x = sqrt(eta) |R_i - R_j + T|;
@@ -2428,6 +2502,207 @@ __global__ void k_ewald_rec(const cplx<double>* __restrict__ qhat, const double*
   block_sums<1>(acc, partials + blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// stress: reciprocal-space sums over the half-space mask and the vW
+// gradient products (total_functional.py:176-209 and the per-term routines)
+// ---------------------------------------------------------------------------
+// d/dq of the cubic spline (splev der=1: FITPACK's derivative of the
+// B-spline sum), first polynomial piece below the first knot, zero from
+// qmax on (_PP_Derivative_One, pseudo/__init__.py:454-460).
+__device__ double vline_deriv(const SplineDev& sp, double x) {
+  if (!(x < sp.qmax)) return 0.0;
+  const int k = 3;
+  const double* t = sp.t;
+  const int hi = sp.nt - k - 2;
+  int l;
+  if (x >= t[hi + 1]) {
+    l = hi;
+  } else {
+    int a = k, b = hi;
+    while (a < b) {
+      const int m = (a + b + 1) >> 1;
+      if (t[m] <= x) a = m; else b = m - 1;
+    }
+    l = a;
+  }
+  // degree-2 B-splines B_{l-2..l} at x
+  double h[3] = {1.0, 0.0, 0.0};
+  double hh[2];
+#pragma unroll
+  for (int j = 1; j <= k - 1; ++j) {
+#pragma unroll
+    for (int i = 0; i < j; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= j; ++i) {
+      const int li = l + i, lj = li - j;
+      const double f = hh[i - 1] / (t[li] - t[lj]);
+      h[i - 1] += f * (t[li] - x);
+      h[i] = f * (x - t[lj]);
+    }
+  }
+  // s'(x) = sum_j k (c_j - c_(j-1)) / (t_(j+k) - t_j) B_(j,k-1)(x)
+  double d = 0.0;
+#pragma unroll
+  for (int i = 0; i < k; ++i) {
+    const int j = l - 2 + i;
+    d += k * (sp.c[j] - sp.c[j - 1]) / (t[j + k] - t[j]) * h[i];
+  }
+  return d;
+}
+
+// eta dF/deta of the inverse Lindhard function with the reference's three
+// branches (LindhardDerivative, kedf/kernel.py:197-211).
+__device__ double lindhard_deriv_eta(double eta, double mu) {
+  const double atol = 1.0e-10;
+  double d = 0.0;
+  if (eta > atol && fabs(eta - 1.0) > atol) {
+    const double ta = log(fabs((1.0 + eta) / (1.0 - eta)));
+    const double den = 0.5 + 0.25 * (1.0 - eta * eta) / eta * ta;
+    d = (0.5 / eta - 0.25 * (eta * eta + 1.0) / (eta * eta) * ta) / (den * den) + 6.0 * eta * mu;
+  }
+  if (eta < atol) d = -2.0 * eta * (1.0 / 3.0 - 3.0 * mu);
+  if (fabs(eta - 1.0) < atol) d = -48.0;
+  return d * eta;
+}
+
+enum StressMode { kStHartree = 0, kStWT = 1, kStPPExact = 2, kStPPPme = 3, kStEwald = 4 };
+
+struct StressArgs {
+  const cplx<double>* a;  // Hartree, PP: rfftn(rho); WT: rfftn(rho^alpha); Ewald PME: rfftn(Q)
+  const cplx<double>* b;  // WT: rfftn(rho^beta); PP PME: rfftn(Q_s)
+  const double* bm;       // PME b-moduli
+  const SplineDev* spl;
+  int nspecies, s;        // PP: tables; PP PME: the species of b
+  const double* pos;
+  const int* species;
+  const double* q;        // Ewald charges
+  int nat;
+  double tkf, mu, eta;
+};
+
+// acc[k] = sum over the mask of the six pair weights (ab = xx xy xz yy yz
+// zz) of one term:
+//   Hartree   G_a G_b |a|^2 / G^4                      (hartree.py:68-90)
+//   WT        (G_a G_b / G^2 - d_ab/3) eta F'(eta) Re(a conj b), eta = |G|/tkf
+//                                                       (wt.py:75-106)
+//   PP exact  G_a G_b Re(a sum_i v'_s(i)(|G|) e^(i G.R_i)) / |G|
+//                                                       (pseudo/__init__.py:479-503)
+//   PP PME    G_a G_b Re(conj(a) B v'_s(|G|) b) / |G|   (pseudo/__init__.py:573-602)
+//   Ewald     |S|^2 e^(-G^2/(4 eta)) / G^2 (2 G_a G_b / G^2 (1 + G^2/(4 eta)) - d_ab),
+//             S = a B (PME) or sum_i q_i e^(-i G.R_i)  (ewald.py:719-759, :915-958)
+__global__ void k_stress_recip(StressArgs A, GridGeom geo, int mode, double* __restrict__ partials) {
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    if (!in_mask(geo, k0, k1, c)) continue;
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    const double g2 = G[0] * G[0] + G[1] * G[1] + G[2] * G[2];
+    const long long ph = r * geo.nhp + c;
+    double w = 0.0, wd = 0.0, wg = 1.0;  // pair weight: wg * G_a G_b * w - d_ab * wd
+    if (mode == kStHartree) {
+      if (!(g2 > 0.0)) continue;
+      const cplx<double> z = A.a[ph];
+      w = (z.x * z.x + z.y * z.y) / (g2 * g2);
+    } else if (mode == kStWT) {
+      const cplx<double> x = A.a[ph], y = A.b[ph];
+      const double ld = lindhard_deriv_eta(sqrt(g2) / A.tkf, A.mu) * (x.x * y.x + x.y * y.y);
+      w = g2 > 0.0 ? ld / g2 : 0.0;
+      wd = ld / 3.0;
+    } else if (mode == kStPPExact || mode == kStPPPme) {
+      if (!(g2 > 0.0)) continue;
+      const double qv = sqrt(g2);
+      const cplx<double> z = A.a[ph];
+      if (mode == kStPPExact) {
+        double vr = 0.0, vi = 0.0;
+        for (int sp = 0; sp < A.nspecies; ++sp) {
+          const double d = vline_deriv(A.spl[sp], qv);
+          if (d == 0.0) continue;
+          for (int i = 0; i < A.nat; ++i) {
+            if (A.species[i] != sp) continue;
+            double sn, cs;
+            sincos(G[0] * A.pos[3 * i] + G[1] * A.pos[3 * i + 1] + G[2] * A.pos[3 * i + 2], &sn, &cs);
+            vr += d * cs;
+            vi += d * sn;
+          }
+        }
+        w = (z.x * vr - z.y * vi) / qv;
+      } else {
+        const cplx<double> u = cmul(cmul(cplx<double>{z.x, -z.y}, bmod(A.bm, geo.n0, geo.n1, k0, k1, c)), A.b[ph]);
+        w = u.x * vline_deriv(A.spl[A.s], qv) / qv;
+      }
+    } else {
+      if (!(g2 > 0.0)) continue;
+      cplx<double> S;
+      if (A.a) {
+        S = cmul(A.a[ph], bmod(A.bm, geo.n0, geo.n1, k0, k1, c));
+      } else {
+        S = {0.0, 0.0};
+        for (int i = 0; i < A.nat; ++i) {
+          double sn, cs;
+          sincos(G[0] * A.pos[3 * i] + G[1] * A.pos[3 * i + 1] + G[2] * A.pos[3 * i + 2], &sn, &cs);
+          S.x += A.q[i] * cs;
+          S.y -= A.q[i] * sn;
+        }
+      }
+      const double e = (S.x * S.x + S.y * S.y) * exp(-g2 / (4.0 * A.eta)) / g2;
+      w = e;
+      wg = 2.0 / g2 * (1.0 + g2 / (4.0 * A.eta));
+      wd = e;
+    }
+    acc[0] += G[0] * G[0] * wg * w - wd;
+    acc[1] += G[0] * G[1] * wg * w;
+    acc[2] += G[0] * G[2] * wg * w;
+    acc[3] += G[1] * G[1] * wg * w - wd;
+    acc[4] += G[1] * G[2] * wg * w;
+    acc[5] += G[2] * G[2] * wg * w - wd;
+  }
+  block_sums<6>(acc, partials + 6 * (long long)blockIdx.x);
+}
+
+// Gradient spectra for the vW stress: slot a = i G_a X / N, a = 0..2, from
+// X in slot 0 (read before it is overwritten).  With full=1 frequencies
+// G_a is replaced by its Hermitian part (G(k) - G(-k))/2, so that the
+// half-spectrum inverse equals Re(ifftn) of the full array (vw.py:97-112
+// with ReciprocalField.ifft(force_real=True)).
+__global__ void k_grad_spec(cplx<double>* __restrict__ spec, long long Nhp, GridGeom geo, double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const long long ph = r * geo.nhp + c;
+    const cplx<double> z = spec[ph];
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    if (geo.full) {
+      double Gp[3];
+      gvec(geo, k0, k1, c, 1, Gp);
+      for (int a = 0; a < 3; ++a) G[a] = 0.5 * (G[a] - Gp[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) spec[a * Nhp + ph] = cplx<double>{-G[a] * z.y * scale, G[a] * z.x * scale};
+  }
+}
+
+// sum V_xc rho of the LDA potential (the P of _LDAStress, xc/semilocal_xc.py:343-346)
+__global__ void k_lda_vrho(const double* __restrict__ rho, long long N, double* __restrict__ partials) {
+  double acc[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const double r = rho[i];
+    double ex, vxc;
+    lda_point(r, cbrt(r), ex, vxc);
+    acc[0] += vxc * r;
+  }
+  block_sums<1>(acc, partials + blockIdx.x);
+}
+
 // Python's float modulo x % 1.0, applied twice as ase's wrap does.
 double pymod1(double x) {
   double r = std::fmod(x, 1.0);
@@ -2466,8 +2741,8 @@ std::vector<double> make_bm(const ofdft_plan_s* P, int order) {
 }
 
 // forward r2c of a real FP64 field into spectrum slot `slot` of the workspace
-int r2c_slot(ofdft_plan_s* P, const double* in, int slot) {
-  PreCopy<double> pre{in};
+template <class Pre>
+int r2c_pre_slot(ofdft_plan_s* P, const Pre& pre, int slot) {
   int rc;
   auto* spec = reinterpret_cast<cplx<double>*>(P->spec) + (long long)slot * P->Nhp;
   // rows kernel writes slot 0; move it if another slot is requested
@@ -2477,6 +2752,17 @@ int r2c_slot(ofdft_plan_s* P, const double* in, int slot) {
   if ((rc = launch_axis<double>(P, s0, 0, 1, false))) return rc;
   if (slot != 0) HIPCHK(hipMemcpyAsync(spec, s0, sizeof(cplx<double>) * P->Nhp, hipMemcpyDeviceToDevice, P->stream));
   return 0;
+}
+
+int r2c_slot(ofdft_plan_s* P, const double* in, int slot) { return r2c_pre_slot(P, PreCopy<double>{in}, slot); }
+
+// six stress sums of k_stress_recip over the half spectrum into out[0..6)
+int stress_recip(ofdft_plan_s* P, const StressArgs& A, int mode, double* out) {
+  const int nb = vec_blocks(P->Nh);
+  if ((long long)nb * 6 > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
+  hipLaunchKernelGGL(k_stress_recip, dim3(nb), dim3(256), 0, P->stream, A, geom(P), mode, P->partials);
+  if (hipGetLastError() != hipSuccess) return fail(OFDFT_EHIP, "stress kernel launch");
+  return reduce_blocks(P, nb, 6, out);
 }
 
 // c2r of spectrum slot 0 (consumed) into a real FP64 field, 1/N normalised.
@@ -3242,6 +3528,118 @@ int ofdft_ewald_forces(ofdft_plan P, int nat, const double* positions, const dou
   }
   return 0;
 }
+
+int ofdft_functional_stress(ofdft_plan P, const void* rho, double tkf, double* out) {
+  if (!valid(P) || !rho || !out) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "stress is computed on an FP64 plan");
+  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
+  const ofdft_functional& f = P->fn;
+  if (f.wt && !(tkf > 0.0)) return fail(OFDFT_EINVAL, "tkf must be positive");
+  for (int i = 0; i < OFDFT_NSTRESS; ++i) out[i] = 0.0;
+  const double* r = (const double*)rho;
+  auto* spec = reinterpret_cast<cplx<double>*>(P->spec);
+  StressArgs A{};
+  int rc;
+  if (f.hartree) {
+    if ((rc = r2c_slot(P, r, 0))) return rc;
+    A.a = spec;
+    if ((rc = stress_recip(P, A, kStHartree, out))) return rc;
+  }
+  if (f.vw) {
+    if ((rc = r2c_slot(P, r, 0))) return rc;
+    hipLaunchKernelGGL(k_grad_spec, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nhp, geom(P),
+                       1.0 / (double)P->N);
+    LAUNCHCHK();
+    if ((rc = launch_axis<double>(P, spec, 0, 3, true))) return rc;
+    if ((rc = launch_axis<double>(P, spec, 1, 3, true))) return rc;
+    PostStressVW<double> post{r};
+    if ((rc = launch_rows_inv<double>(P, post, 3, spec, true))) return rc;
+    if ((rc = fetch_sums(P, 6, out + 6))) return rc;
+  }
+  if (f.wt) {
+    if ((rc = r2c_pre_slot(P, PrePow<double>{r, f.alpha}, 0))) return rc;
+    const bool same = f.beta == f.alpha;
+    if (!same && (rc = r2c_pre_slot(P, PrePow<double>{r, f.beta}, 1))) return rc;
+    A.a = spec;
+    A.b = same ? spec : spec + P->Nhp;
+    A.tkf = tkf;
+    A.mu = f.y;
+    if ((rc = stress_recip(P, A, kStWT, out + 12))) return rc;
+  }
+  if (f.lda) {
+    const int nb = vec_blocks(P->N);
+    hipLaunchKernelGGL(k_lda_vrho, dim3(nb), dim3(256), 0, P->stream, r, P->N, P->partials);
+    LAUNCHCHK();
+    if ((rc = reduce_blocks(P, nb, 1, out + 18))) return rc;
+  }
+  return 0;
+}
+
+int ofdft_local_stress(ofdft_plan P, const void* rho, double* out6) {
+  if (!valid(P) || !rho || !out6) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec != OFDFT_F64) return fail(OFDFT_EINVAL, "stress is computed on an FP64 plan");
+  if (!P->d_splines || P->nat < 1) return fail(OFDFT_ESTATE, "call ofdft_vloc_build first");
+  int rc;
+  auto* spec = reinterpret_cast<cplx<double>*>(P->spec);
+  auto* rhog = spec + P->Nhp;  // slot 1 = rfftn(rho)
+  if ((rc = r2c_slot(P, (const double*)rho, 1))) return rc;
+  StressArgs A{};
+  A.a = rhog;
+  A.spl = reinterpret_cast<const SplineDev*>(P->d_splines);
+  A.nspecies = P->nspecies;
+  for (int k = 0; k < 6; ++k) out6[k] = 0.0;
+  if (!P->pme) {
+    A.pos = P->d_pos;
+    A.species = P->d_species;
+    A.nat = P->nat;
+    return stress_recip(P, A, kStPPExact, out6);
+  }
+  if ((rc = ensure_work(P))) return rc;
+  if (!P->d_bm) return fail(OFDFT_ESTATE, "PME tables missing");
+  A.bm = P->d_bm;
+  for (int s = 0; s < P->nspecies; ++s) {
+    // Q_s: the unit-weight B-spline spread of the atoms of species s
+    HIPCHK(hipMemsetAsync(P->d_work, 0, sizeof(double) * P->N, P->stream));
+    hipLaunchKernelGGL(k_pme_spread, dim3(P->nat), dim3(1024), 0, P->stream, P->d_work, P->d_frac, P->d_species, s,
+                       (const double*)nullptr, P->n[0], P->n[1], P->n[2], P->order);
+    LAUNCHCHK();
+    if ((rc = r2c_slot(P, P->d_work, 0))) return rc;
+    A.b = spec;
+    A.s = s;
+    double t[6];
+    if ((rc = stress_recip(P, A, kStPPPme, t))) return rc;
+    for (int k = 0; k < 6; ++k) out6[k] += t[k];
+  }
+  return 0;
+}
+
+int ofdft_ewald_stress(ofdft_plan P, int nat, const double* positions, const double* charges, double eta,
+                       double rmax, int pme, int order, double* real6, double* rec6) {
+  if (!valid(P) || !positions || !charges || !real6 || !rec6) return fail(OFDFT_EINVAL, "null argument");
+  int rc;
+  if ((rc = ewald_check(P, nat, eta, rmax, pme, order))) return rc;
+  EwaldDev E;
+  if ((rc = ewald_upload(P, nat, positions, charges, rmax, E))) return rc;
+  const int nb = vec_blocks((long long)nat * nat);
+  if ((long long)nb * 6 > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
+  hipLaunchKernelGGL(k_ewald_stress_real, dim3(nb), dim3(256), 0, P->stream, E.pos, E.q, nat, E.sh, E.nshift, rmax,
+                     eta, P->partials);
+  if (hipGetLastError() != hipSuccess) return fail(OFDFT_EHIP, "Ewald stress launch");
+  if ((rc = reduce_blocks(P, nb, 6, real6))) return rc;
+  StressArgs A{};
+  A.eta = eta;
+  if (pme) {
+    if ((rc = ewald_qhat(P, E, nat, order))) return rc;
+    A.a = reinterpret_cast<const cplx<double>*>(P->spec);
+    A.bm = P->d_bm;
+  } else {
+    A.pos = E.pos;
+    A.q = E.q;
+    A.nat = nat;
+  }
+  return stress_recip(P, A, kStEwald, rec6);
+}
+
 
 int ofdft_dot3(ofdft_plan P, const void* a, const void* b, double* out3) {
   if (!valid(P) || !a || !b || !out3) return fail(OFDFT_EINVAL, "null argument");

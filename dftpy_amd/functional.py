@@ -13,6 +13,7 @@ launches for up to three convolutions plus the pointwise terms) instead of
 one pass per term.
 """
 import copy
+import ctypes
 from abc import ABC, abstractmethod
 
 import numpy as np
@@ -198,6 +199,18 @@ class KEDF(AbstractFunctional):
         return tot.compute(density, calcType=calcType)
 
 
+    def stress(self, density, name=None, kedf=None, split=False, **kwargs):
+        """KEDF.stress (kedf/__init__.py:193-235): per engine ("TF", "VW",
+        "WT-NL") when split, else their sum; energies at ``density``."""
+        out, _ = _density_stress(self._total(), density)
+        if split:
+            return out
+        tot = np.zeros((3, 3))
+        for v in out.values():
+            tot = tot + v
+        return tot
+
+
 class XC(AbstractFunctional):
     """LDA exchange-correlation (Slater + Perdew-Zunger, the in-house path of
     xc/semilocal_xc.py:283-341; LibXC is not on the device path)."""
@@ -223,6 +236,15 @@ class XC(AbstractFunctional):
         if "E" in calcType:
             self.energy = out.energy
         return out
+
+    def stress(self, density, pseudo=None, energy=None, **kwargs):
+        """XC.stress (xc/__init__.py:72-87) with _LDAStress
+        (xc/semilocal_xc.py:343-346): (E - sum V_xc rho dV) / Omega on the
+        diagonal; E is ``energy`` when given, else the LDA energy at density."""
+        if getattr(self, "_tot", None) is None:
+            self._tot = TotalFunctional(XC=self)
+        out, _ = _density_stress(self._tot, density, energy_xc=energy)
+        return out["XC"]
 
 
 class Hartree(AbstractFunctional):
@@ -253,9 +275,70 @@ class Hartree(AbstractFunctional):
     def energy(self):
         return self._energy
 
+    def stress(self, density, **kwargs):
+        """HartreeFunctionalStress (hartree.py:51-90) with the Hartree energy
+        at ``density``."""
+        if getattr(self, "_tot", None) is None:
+            self._tot = TotalFunctional(HARTREE=self)
+        out, _ = _density_stress(self._tot, density)
+        return out["HARTREE"]
+
 
 _SPLIT_KEYS = [("KEDF-TF", L.E_TF, "tf"), ("KEDF-VW", L.E_VW, "vw"), ("KEDF-NL", L.E_NL, "wt"),
                ("XC", L.E_XC, "lda"), ("HARTREE", L.E_H, "hartree"), ("PSEUDO", L.E_PP, "pseudo")]
+
+
+def sym6(v):
+    """3x3 symmetric matrix from the six components xx xy xz yy yz zz."""
+    a = np.empty((3, 3))
+    a[0, 0], a[0, 1], a[0, 2], a[1, 1], a[1, 2], a[2, 2] = v
+    a[1, 0], a[2, 0], a[2, 1] = a[0, 1], a[0, 2], a[1, 2]
+    return a
+
+
+def _density_stress(total, rho, energy_xc=None):
+    """Stress of the density terms of ``total`` at rho, keyed like the
+    reference's split output (KEDF parts by their engine names "TF", "VW",
+    "WT-NL"; "XC"; "HARTREE"), with each term's energy at rho.  One FP64
+    energy evaluation (which also builds the WT kernel) and one
+    ofdft_functional_stress call; the reference's factors:
+    TF kedf/tf.py:52-62, vW kedf/vw.py:97-112, WT kedf/wt.py:75-106,
+    LDA xc/semilocal_xc.py:343-346 (xc/__init__.py:72-87), Hartree
+    hartree.py:68-90."""
+    if not isinstance(rho, DirectField):
+        raise TypeError("rho must be DirectField")
+    grid = rho.grid
+    ev = total.evaluator(grid, torch.float64)
+    t = ev.terms
+    with torch._C.DisableTorchFunctionSubclass():
+        r = rho.to(torch.float64).contiguous()
+        sums = ev.eval_rho(r, None)
+        rho0 = float(r.mean())
+    parts = ev.energies(sums)
+    tkf = 2.0 * (3.0 * rho0 * np.pi ** 2) ** (1.0 / 3.0)
+    raw = np.zeros(L.NSTRESS)
+    ev.P._stream()
+    with torch._C.DisableTorchFunctionSubclass():
+        L.check(ev.P.lib.ofdft_functional_stress(ev.P._h, L.ptr(r), float(tkf),
+                                                 raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    vol, dV = grid.volume, grid.dV
+    eye = np.eye(3)
+    out = {}
+    if t["tf"]:
+        out["TF"] = eye * (-2.0 / 3.0 * parts["KEDF-TF"] / vol)
+    if t["vw"]:
+        out["VW"] = sym6(raw[6:12]) * (-0.25 / vol * dV * t["y"])
+    if t["wt"]:
+        a, b = t["alpha"], t["beta"]
+        s = sym6(raw[12:18]) * (dV * dV / vol ** 2)
+        s = s * (np.pi ** 2 / (a * b * rho0 ** (a + b - 2) * tkf / 2.0))
+        out["WT-NL"] = s - eye * (2.0 / 3.0 * parts["KEDF-NL"] / vol)
+    if t["lda"]:
+        e = parts["XC"] if energy_xc is None else energy_xc
+        out["XC"] = eye * ((e - raw[18] * dV) / vol)
+    if t["hartree"]:
+        out["HARTREE"] = sym6(raw[0:6]) * (dV * dV * 8.0 * np.pi / vol ** 2) - eye * (parts["HARTREE"] / vol)
+    return out, parts
 
 
 class DeviceEvaluator:
@@ -450,6 +533,30 @@ class TotalFunctional(AbstractFunctional):
                 forces["II"] = fr
             forces["TOTAL"] = forces["TOTAL"] + fr
         return forces if split else forces["TOTAL"]
+
+    def get_stress(self, rho, split=False, **kwargs):
+        """Stress tensor (total_functional.py:176-209), Ha/bohr^3: the KEDF
+        parts keyed KEDF-TF, KEDF-VW, KEDF-NL, then XC, HARTREE, PSEUDO and
+        the ion-ion II; every term with its energy at rho."""
+        dens, parts = _density_stress(self, rho)
+        stress = {"TOTAL": np.zeros((3, 3))}
+        for k, v in dens.items():
+            key = "KEDF-" + k.split("-")[-1] if k in ("TF", "VW", "WT-NL") else k
+            stress[key] = v
+            stress["TOTAL"] = stress["TOTAL"] + v
+        for f in self.funcDict.values():
+            if f.type == "PSEUDO":
+                s = f.stress(rho, energy=parts.get("PSEUDO"))
+                stress["PSEUDO"] = s
+                stress["TOTAL"] = stress["TOTAL"] + s
+        ew = self.ewald
+        if ew:
+            stress["II"] = ew.stress
+            stress["TOTAL"] = stress["TOTAL"] + stress["II"]
+        for i in range(1, 3):
+            for j in range(i - 1, -1, -1):
+                stress["TOTAL"][i, j] = stress["TOTAL"][j, i]
+        return stress if split else stress["TOTAL"]
 
 
 def Functional(type=None, optional_kwargs=None, **kwargs):

@@ -62,6 +62,14 @@ def read_recpot(fname):
     return RadialPseudo(r, v, zval)
 
 
+
+def sym6(v):
+    """3x3 symmetric matrix from the six components xx xy xz yy yz zz."""
+    a = np.empty((3, 3))
+    a[0, 0], a[0, 1], a[0, 2], a[1, 1], a[1, 2], a[2, 2] = v
+    a[1, 0], a[2, 0], a[2, 1] = a[0, 1], a[0, 2], a[1, 2]
+    return a
+
 def empty_core(Z, rc, qmax=30.0, npts=2001):
     """Ashcroft empty-core table as a RadialPseudo (zval recovered as
     recpot.py:41 does)."""
@@ -206,6 +214,32 @@ class LocalPseudo(AbstractFunctional):
             L.check(P.lib.ofdft_local_forces(P._h, L.ptr(r), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return out
 
+    def stress(self, rho, energy=None):
+        """Local-PP stress (pseudo/__init__.py:248-258): _StressPME or _Stress,
+        Ha/bohr^3, 3x3."""
+        if rho is None:
+            raise AttributeError("Must specify rho")
+        if not isinstance(rho, DirectField):
+            raise TypeError("rho must be DirectField")
+        self.local_PP()
+        if energy is None:
+            energy = self.compute(rho, calcType=("E",)).energy
+        P = self._fp64_plan()
+        s6 = np.zeros(6)
+        with torch._C.DisableTorchFunctionSubclass():
+            r = rho.to(torch.float64).contiguous()
+            P._stream()
+            L.check(P.lib.ofdft_local_stress(P._h, L.ptr(r), s6.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        g = self.grid
+        dV, vol = g.dV, g.volume
+        if self.PME:
+            # -sum / Omega^2 with fft() = dV rfftn() on rho and Q, times 2 nnr
+            st = -sym6(s6) * dV * dV / vol ** 2 * 2.0 * float(np.prod(g.nrR))
+        else:
+            st = -sym6(s6) * dV / vol * 2.0
+        st = st - np.eye(3) * energy
+        return st / vol
+
     def get_ewald(self, PME=None):
         if self._ewald is None:
             self._ewald = Ewald(ions=self.ions, grid=self.grid, PME=self.PME if PME is None else PME,
@@ -273,6 +307,29 @@ class Ewald:
             er, ek = self._sums()
             self._energy = self.Energy_corr() + er + ek
         return self._energy
+
+    @property
+    def stress(self):
+        """Ion-ion stress (ewald.py:523-538): Stress_real plus Stress_rec_PME
+        or Stress_rec, Ha/bohr^3, 3x3."""
+        if getattr(self, "_stress", None) is None:
+            P = self._fp64_plan()
+            rmax = special.erfcinv(self.precision / 3.0) / np.sqrt(self.eta)
+            pos = np.ascontiguousarray(self.ions.positions, dtype=np.float64)
+            q = np.ascontiguousarray(self.ions.charges, dtype=np.float64)
+            real6, rec6 = np.zeros(6), np.zeros(6)
+            P._stream()
+            dp = ctypes.POINTER(ctypes.c_double)
+            L.check(P.lib.ofdft_ewald_stress(P._h, self.ions.nat, pos.ctypes.data_as(dp), q.ctypes.data_as(dp),
+                                             float(self.eta), float(rmax), 1 if self.PME else 0, int(self.order),
+                                             real6.ctypes.data_as(dp), rec6.ctypes.data_as(dp)))
+            vol = self.grid.volume
+            # Stress_rec_PME doubles the half-space sum and scales by 2 pi/Omega^2
+            # (|fft(Q)|^2 / dV^2 = |rfftn(Q)|^2); Stress_rec scales by 4 pi/Omega^2
+            rec = sym6(rec6) * (4.0 * np.pi / vol ** 2)
+            s_g0 = self.ions.get_ncharges() ** 2 * 4.0 * np.pi * (1.0 / (4.0 * self.eta * vol ** 2) / 2.0)
+            self._stress = sym6(real6) * (-0.5 / vol) + rec + np.eye(3) * s_g0
+        return self._stress
 
     @property
     def forces(self):

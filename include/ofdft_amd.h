@@ -238,6 +238,40 @@ int ofdft_ewald_forces(ofdft_plan plan, int nat, const double* positions, const 
 int ofdft_ewald(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta, double rmax,
                 int pme, int order, double* e_real, double* e_rec);
 
+/* Stress (TotalFunctional.get_stress, total_functional.py:176-209).  The
+ * six components of each symmetric 3x3 sum are ordered xx xy xz yy yz zz;
+ * the callers apply the reference's factors, volumes and energy terms.
+ *
+ * ofdft_functional_stress: FP64 plan, rho device N doubles, the terms of
+ * the set functional.  out[OFDFT_NSTRESS] (host):
+ *   [0,6)   Hartree  sum_mask G_a G_b |rfftn(rho)|^2 / G^4   (hartree.py:68-90)
+ *   [6,12)  vW       sum_r d_a rho d_b rho / rho, gradients by FFT
+ *                                                            (kedf/vw.py:97-112)
+ *   [12,18) WT       sum_mask (G_a G_b / G^2 - d_ab / 3) eta F'(eta)
+ *                    Re(rfftn(rho^alpha) conj rfftn(rho^beta)), eta = |G| / tkf,
+ *                    mu = y                                  (kedf/wt.py:75-106,
+ *                                                             kernel.py:197-211)
+ *   [18]    LDA      sum_r V_xc rho                          (xc/semilocal_xc.py:343-346)
+ * Terms not in the functional are 0.                                      */
+#define OFDFT_NSTRESS 19
+int ofdft_functional_stress(ofdft_plan plan, const void* rho, double tkf, double* out);
+
+/* Local-PP stress sum for the ions and tables of the last ofdft_vloc_build
+ * (pseudo/__init__.py:248-258): exact form sum_mask G_a G_b
+ * Re(rfftn(rho) sum_i v'_s(i)(|G|) exp(i G.R_i)) / |G| (_Stress :462-503),
+ * or the PME form sum_s sum_mask G_a G_b Re(conj(rfftn(rho)) B v'_s(|G|)
+ * rfftn(Q_s)) / |G| (_StressPME :573-602).  v' is the derivative of the
+ * table's cubic spline (_PP_Derivative_One :454-460).  out6 host.         */
+int ofdft_local_stress(ofdft_plan plan, const void* rho, double* out6);
+
+/* Ewald stress sums (ewald.py:523-538): real6 = sum over ordered pairs and
+ * images of q_i q_j [2 sqrt(eta/pi) exp(-eta d^2) + erfc(sqrt(eta) d)/d]
+ * d_a d_b / d^2 (Stress_real :609-662); rec6 = sum_mask |S|^2
+ * exp(-G^2/(4 eta))/G^2 (2 G_a G_b/G^2 (1 + G^2/(4 eta)) - d_ab) with the
+ * PME (Stress_rec_PME :915-958) or exact (Stress_rec :719-759) S.        */
+int ofdft_ewald_stress(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta,
+                       double rmax, int pme, int order, double* real6, double* rec6);
+
 /* ---- vector kernels used by the optimizer (optimization.py:92-280) ---- */
 /* out[0] = sum(a*b), out[1] = sum(a*a), out[2] = sum(b*b) in FP64 (host).  */
 int ofdft_dot3(ofdft_plan plan, const void* a, const void* b, double* out3);
