@@ -1,17 +1,68 @@
 """Ions: symbols, Cartesian positions (bohr), cell rows (bohr), charges
-(the parts of ions.py:5-135 the hot path uses; the reference builds on
-ase.Atoms, which is not a dependency here)."""
+(the parts of ions.py:5-135 the hot path and the file formats use; the
+reference builds on ase.Atoms, which is not a dependency here).
+
+The first three positional arguments are (symbols, positions, cell); the
+reference's keyword arguments ``numbers``, ``scaled_positions``,
+``charges`` and ``units`` ('au', or 'ase' for Angstrom input converted to
+bohr as Ions.convert_units does, ions.py:62-78) are accepted by name."""
 import numpy as np
+
+BOHR = 0.5291772105638411  # Angstrom per bohr (ase.units.Bohr, CODATA 2014)
+
+# chemical symbols by atomic number (0 = unknown, 'X')
+CHEMICAL_SYMBOLS = (
+    "X H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
+    "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb "
+    "Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf Es Fm Md No Lr Rf "
+    "Db Sg Bh Hs Mt Ds Rg Cn Nh Fl Mc Lv Ts Og").split()
+ATOMIC_NUMBERS = {s: z for z, s in enumerate(CHEMICAL_SYMBOLS)}
+
+
+def symbol_of(label):
+    """A chemical symbol from a symbol or an atomic number (int or digits)."""
+    if isinstance(label, (int, np.integer)):
+        return CHEMICAL_SYMBOLS[int(label)]
+    label = str(label)
+    return CHEMICAL_SYMBOLS[int(label)] if label.isdigit() else label
 
 
 class Ions:
-    def __init__(self, symbols, positions=None, cell=None, scaled_positions=None, charges=None):
-        self.symbols = list(symbols)
-        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+    def __init__(self, symbols=None, positions=None, cell=None, scaled_positions=None, charges=None,
+                 numbers=None, units="au"):
+        if symbols is None:
+            if numbers is None:
+                raise AttributeError("Ions need symbols or numbers")
+            symbols = [symbol_of(int(z)) for z in np.asarray(numbers).ravel()]
+        self.symbols = [symbol_of(s) for s in symbols]
+        self.cell = np.asarray(cell, dtype=np.float64).reshape(3, 3).copy()
+        if units not in ("au", "ase"):
+            raise AttributeError("units must be 'au' or 'ase'")
+        if units == "ase":
+            self.cell /= BOHR
         if positions is None:
-            positions = np.asarray(scaled_positions, dtype=np.float64) @ self.cell
+            positions = np.asarray(scaled_positions, dtype=np.float64).reshape(-1, 3) @ self.cell
+        else:
+            positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+            if units == "ase":
+                positions = positions / BOHR
         self.positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3).copy()
         self._charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+
+    @property
+    def numbers(self):
+        return np.array([ATOMIC_NUMBERS.get(s, 0) for s in self.symbols], dtype=np.int64)
+
+    def get_charges(self):
+        """Atomic charges, zeros when none are set (ase's initial charges)."""
+        return np.zeros(self.nat) if self._charges is None else self._charges.copy()
+
+    @property
+    def zval(self):
+        """{symbol: charge} of the set charges (ions.py:107-115)."""
+        if self._charges is None:
+            return {}
+        return {s: float(c) for s, c in zip(self.symbols, self._charges)}
 
     @property
     def nat(self):
