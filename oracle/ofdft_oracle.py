@@ -34,6 +34,13 @@ rec PME)                    forces of the perfect fcc fixture and exact vs PME
                             agreement (examples/test/test_ewald.py:37-50)
 optimize_rho (LBFGS)        converges to the TN energy (parity unpinned
                             beyond that: no reference test runs LBFGS)
+gga_functional and the 22   LC94 energy on Al_fde_rho.pp
+GGA enhancement factors     (test_functional.py:18, 1.6821337114254904);
+(GGA_KEDFS)                 dFds2 against the central difference of F and V
+                            against dE/drho for every factor except PW91,
+                            PW91K, LC94, PW91O, VT84F, whose reference dFds2
+                            is not the derivative of its F (kept as is):
+                            their potentials are parity unpinned
 stress: tf, vw, wt (with    (1/Omega) dE/d(strain) of the oracle's own energy
 lindhard_derivative), lda,  under a homogeneous strain, central difference,
 hartree, LocalPseudo        every term; test_pme.py:35-36 and
@@ -61,7 +68,7 @@ __all__ = [
     "empty_core_table", "LocalPseudo", "Ewald", "TotalFunctional",
     "Optimization", "read_qepp", "read_poscar", "ecut2nr", "best_fft_size",
     "cbspline_Mn", "tf_stress", "vw_stress", "wt_stress", "lda_stress", "hartree_stress",
-    "lindhard_derivative",
+    "lindhard_derivative", "GGA_KEDFS", "gga_functional", "gga_enhancement", "gga_params",
 ]
 
 
@@ -395,6 +402,287 @@ def hartree_functional(rho, grid, calc=("E", "V")):
     """hartree.py:24-45."""
     V = ifft(fft(rho, grid) * grid.invgg * 4 * np.pi, grid)
     E = np.einsum("ijk,ijk->", V, rho) * grid.dV / 2.0 if "E" in calc else None
+    return E, V
+
+
+# ---------------------------------------------------------------------------
+# GGA kinetic functionals: kedf/gga.py:18-652 (enhancement factors F(s) and
+# dFds2 = (1/s) dF/ds) and :1212-1326 (the energy and potential)
+# ---------------------------------------------------------------------------
+C_TF = 2.87123400018819181594  # constants.py:82
+TKF0 = 6.18733545256027186194  # constants.py:83
+CBRT_TWO = 1.25992104989487316477  # constants.py:84
+
+
+def _lkt_mvw(ss, p):
+    e1, e2 = np.exp(-p[0] * ss), np.exp(-2.0 * p[0] * ss)
+    F = 2.0 * e1 / (1.0 + e2)
+    return F, -p[0] * (1.0 - e2) / (1.0 + e2) * F / ss
+
+
+def _lkt(ss, p):
+    s2 = ss * ss
+    F, d = np.empty_like(ss), np.empty_like(ss)
+    m1, m2 = ss > 100.0, ss < 1e-5
+    m = ~(m1 | m2)
+    F[m] = 1.0 / np.cosh(p[0] * ss[m]) + 5.0 / 3.0 * s2[m] * p[1]
+    F[m1] = 5.0 / 3.0 * s2[m1] * p[1]
+    F[m2] = 1.0 + (5.0 / 3.0 * p[1] - 0.5 * p[0] ** 2) * s2[m2] + 5.0 / 24.0 * p[0] ** 4 * s2[m2] ** 2
+    d[m] = 10.0 / 3.0 * p[1] - p[0] * np.sinh(p[0] * ss[m]) / np.cosh(p[0] * ss[m]) ** 2 / ss[m]
+    d[m1] = 10.0 / 3.0 * p[1]
+    d[m2] = 10.0 / 3.0 * p[1] - p[0] ** 2 + 5.0 / 6.0 * p[0] ** 4 * s2[m2] - 61.0 / 120.0 * p[0] ** 6 * s2[m2] ** 2
+    return F, d
+
+
+def _dk(s, p):
+    x = s * s / (72 * C_TF)
+    Fa = 9.0 * p[5] * x ** 4 + p[2] * x ** 3 + p[1] * x ** 2 + p[0] * x + 1.0
+    Fb = p[5] * x ** 3 + p[4] * x ** 2 + p[3] * x + 1.0
+    d = (36.0 * p[5] * x ** 3 + 3 * p[2] * x ** 2 + 2 * p[1] * x + p[0]) / Fb \
+        - Fa / (Fb * Fb) * (3.0 * p[5] * x ** 2 + 2.0 * p[4] * x + p[3])
+    return Fa / Fb, d / (36.0 * C_TF)
+
+
+def _llp(s, p):
+    bs = CBRT_TWO * s
+    Fa = p[0] * bs * bs
+    Fb = 1.0 + p[1] * bs * np.arcsinh(bs)
+    d = 2.0 * p[0] / Fb - (p[0] * p[1] * bs * np.arcsinh(bs) + Fa * p[1] / np.sqrt(1.0 + bs * bs)) / (Fb * Fb)
+    return 1.0 + Fa / Fb, d * CBRT_TWO * CBRT_TWO
+
+
+def _ol(s, p):
+    F = 1.0 + s * s / 72.0 / C_TF + p[0] / C_TF * s / (1 + p[1] * 4 * s)
+    d = 1.0 / 36.0 / C_TF * np.ones_like(s)
+    m = s > 1.0e-8
+    d[m] += p[0] / C_TF / (1 + p[1] * 4 * s[m]) ** 2 / s[m]
+    return F, d
+
+
+def _thak(s, p):
+    bs = CBRT_TWO * s
+    bs2 = bs * bs
+    F = 1.0 + p[0] * bs2 / (1.0 + p[1] * bs * np.arcsinh(bs)) - p[2] * bs / (1.0 + 2 ** (5.0 / 3.0) * bs)
+    Fb = (1.0 + p[1] * bs * np.arcsinh(bs)) ** 2
+    d = (-(p[0] * p[1] * bs2) / np.sqrt(1 + bs2) + (p[0] * p[1] * bs * np.arcsinh(bs)) + 2.0 * p[0]) / Fb
+    m = s > 1.0e-8
+    d[m] -= p[2] / (1.0 + 2 ** (5.0 / 3.0) * bs[m]) ** 2 / bs[m]
+    return F, d * CBRT_TWO * CBRT_TWO
+
+
+def _b86a(s, p):
+    bs2 = (CBRT_TWO * s) ** 2
+    Fb = 1.0 + p[1] * bs2
+    return 1.0 + p[0] * bs2 / Fb, 2 * p[0] / (Fb * Fb) * (CBRT_TWO * CBRT_TWO)
+
+
+def _b86b(s, p):
+    bs2 = (CBRT_TWO * s) ** 2
+    Fb = np.power(1.0 + p[1] * bs2, 4.0 / 5.0)
+    d = (2 * p[0] * (p[1] * bs2 + 5.0)) / (5 * (1.0 + p[1] * bs2) * Fb) * (CBRT_TWO * CBRT_TWO)
+    return 1.0 + p[0] * bs2 / Fb, d
+
+
+def _dk87(s, p):
+    bs = CBRT_TWO * s
+    bs2 = bs * bs
+    Fa = p[0] * bs2 * (1 + p[1] * bs)
+    Fb = 1.0 + p[2] * bs2
+    d = p[0] * (2.0 + 3.0 * p[1] * bs + p[1] * p[2] * bs2 * bs) / (Fb * Fb) * (CBRT_TWO * CBRT_TWO)
+    return 1.0 + Fa / Fb, d
+
+
+def _pw86(ss, p):
+    s2 = ss * ss
+    Fa = 1.0 + p[0] * s2 + p[1] * s2 ** 2 + p[2] * s2 ** 3
+    d = 2.0 / 15.0 * (p[0] + 2 * p[1] * s2 + 3 * p[2] * s2 ** 2) / np.power(Fa, 14.0 / 15.0)
+    return np.power(Fa, 1.0 / 15.0), d
+
+
+def _pw91o(ss, p):
+    s2 = ss * ss
+    s4 = s2 * s2
+    ex = np.exp(-p[3] * s2)
+    Fa = (p[1] - p[2] * ex) * s2 - p[5] * s4
+    Fb = 1.0 + p[0] * ss * np.arcsinh(p[4] * ss) + p[5] * s4
+    Fa_s2 = (p[1] - p[2] * ex) - p[5] * s2
+    d = 2.0 * (p[1] + (p[3] * s2 - 1) * p[2] * ex - 4.0 * p[5] * s2) / Fb \
+        - Fa_s2 * ss / (Fb * Fb) * ((p[0] * p[4] * ss) / (p[4] ** 2 * s2 + 1) + p[0] * np.arcsinh(p[4] * ss)
+                                   + 4.0 * p[5] * s2)
+    return 1.0 + Fa / Fb, d
+
+
+def _pw91(ss, p):
+    s2 = ss * ss
+    ex = np.exp(-p[3] * s2)
+    ash = np.arcsinh(p[4] * ss)
+    Fa = 1.0 + p[0] * ss * ash + (p[1] - p[2] * ex) * s2
+    Fb = 1.0 + p[0] * ss * ash + p[5] * s2 * s2
+    sq = np.sqrt(p[4] ** 2 * s2 + 1)
+    Fa_s = p[0] * p[4] * ss / sq + p[0] * ash + 2.0 * ss * (p[1] + 2.0 * p[2] * p[3] * s2 * ex - 2.0 * p[2] * ex)
+    d = Fa_s / Fb - Fa / (Fb * Fb) * (p[0] * p[4] * ss / sq + (p[0] * ash + 4.0 * p[5] * s2 * ss))
+    m = ss > 1.0e-8
+    d[m] /= ss[m]
+    return Fa / Fb, d
+
+
+def _lg94(ss, p):
+    s2 = ss * ss
+    s4, s6 = s2 * s2, s2 * s2 * s2
+    s8, s10, s12 = s4 * s4, s4 * s6, s6 * s6
+    Fa = 1.0 + p[0] * s2 + p[1] * s4 + p[2] * s6 + p[3] * s8 + p[4] * s10 + p[5] * s12
+    Fb = 1.0 + 1e-8 * s2
+    F = np.power(Fa / Fb, p[6])
+    d = -2 * p[6] * F * (1e-8 * Fa - Fb * (p[0] + 2 * p[1] * s2 + 3 * p[2] * s4 + 4 * p[3] * s6 + 5 * p[4] * s8
+                                           + 6 * p[5] * s10)) / (Fa * Fb)
+    return F, d
+
+
+def _p92(ss, p):
+    s2 = ss * ss
+    Fa = p[0] + p[1] * s2 + p[2] * s2 * s2
+    Fb = p[0] + p[3] * s2
+    return Fa / Fb, (2.0 * p[1] + 4.0 * p[2] * s2) / Fb - (2.0 * p[3] * Fa) / (Fb * Fb)
+
+
+def _pbe(ss, p):
+    s2 = ss * ss
+    Fb = 1.0 + p[1] / p[0] * s2
+    Fb2 = Fb * Fb
+    F = 1.0 + p[1] * s2 / Fb
+    d = 2.0 * p[1] / Fb2
+    if len(p) > 2:
+        Fb3 = Fb2 * Fb
+        F = F + p[2] * s2 * s2 / Fb2
+        d = d + 4 * p[2] * s2 / Fb3
+        if len(p) > 3:
+            F = F + p[3] * s2 ** 3 / Fb3
+            d = d + 6 * p[3] * s2 * s2 / (Fb3 * Fb)
+    return F, d
+
+
+def _p82(ss, p):
+    s2 = ss * ss
+    s6 = s2 ** 3
+    Fb = 1 + s6
+    return 1.0 + p[0] * s2 / Fb, p[0] * (2.0 / Fb - 6.0 * s6 / (Fb * Fb))
+
+
+def _vjks00(ss, p):
+    s2 = ss * ss
+    s4, s6 = s2 * s2, s2 * s2 * s2
+    Fa = 1.0 + p[0] * s2 - p[2] * s6
+    Fb = 1.0 + p[1] * s2 + p[2] * s4
+    return Fa / Fb, (2.0 * p[0] - 6.0 * p[2] * s4) / Fb - (2.0 * p[1] + 4.0 * p[2] * s2) * Fa / (Fb * Fb)
+
+
+def _vt84f(ss, p):
+    s2 = ss * ss
+    s2 = np.where(s2 < 1.0e-8, 1.0e-8, s2)
+    s4 = s2 * s2
+    e1, e2 = np.exp(-p[1] * s2), np.exp(-p[1] * s4)
+    F = 1.0 + 5.0 / 3.0 * s2 + p[0] * s2 * e1 / (1 + p[0] * s2) + (1 - e2) * (1.0 / s2 - 1.0)
+    d = (10.0 / 3.0 + 2.0 * p[0] * e1 * (1.0 - p[1] * s2) / (1 + p[0] * s2)
+         - 2.0 * p[0] * s2 * e1 / (1 + p[0] * s2) ** 2 + 4.0 * p[1] * s2 * (1.0 / s2 - 1.0) * e2
+         - 2.0 * (1 - e2) / s4)
+    return F, d
+
+
+def _tfvw(ss, p):
+    return p[0] + 5.0 / 3.0 * p[1] * ss * ss, 10.0 / 3.0 * p[1] * np.ones_like(ss)
+
+
+def _stv(ss, p):
+    s2 = ss * ss
+    Fb = p[3] + p[2] * s2
+    return p[0] + 5.0 / 3.0 * p[1] * s2 / Fb, 5.0 / 3.0 * p[1] * (2.0 / Fb - 2.0 * s2 * p[2] / (Fb * Fb))
+
+
+def _pbe2m(ss, p):
+    s2 = ss * ss
+    Fb = p[0] + p[1] * s2
+    return 1.0 + p[2] * s2 / Fb, 2.0 * p[2] / (Fb * Fb)
+
+
+def _pg(ss, p):
+    s2 = ss * ss
+    Fa = np.exp(-p[0] * s2)
+    return Fa + 5.0 / 3.0 * p[1] * s2, -2.0 * p[0] * Fa + 10.0 / 3.0 * p[1]
+
+
+# name -> (function, default params, takes s / TKF0): kedf/gga.py:577-635
+GGA_KEDFS = {
+    "LKT-VW": (_lkt_mvw, [1.3], True), "LKT": (_lkt, [1.3, 1.0], True),
+    "DK": (_dk, [0.95, 14.28111, -19.5762, -0.05, 9.99802, 2.96085], False),
+    "LLP": (_llp, [0.0044188, 0.0253], False), "LLP91": (_llp, [0.0044188, 0.0253], False),
+    "OL1": (_ol, [0.00677, 0.0], False), "OL": (_ol, [0.00677, 0.0], False), "OL2": (_ol, [0.0887, 1.0], False),
+    "T92": (_thak, [0.0055, 0.0253, 0.072], False), "THAK": (_thak, [0.0055, 0.0253, 0.072], False),
+    "B86A": (_b86a, [0.0039, 0.004], False), "B86": (_b86a, [0.0039, 0.004], False),
+    "B86B": (_b86b, [0.00403, 0.007], False),
+    "DK87": (_dk87, [7.0 / 324.0 / (18.0 * np.pi ** 4) ** (1.0 / 3.0), 0.861504, 0.044286], False),
+    "PW86": (_pw86, [1.296, 14.0, 0.2], True),
+    "PW91O": (_pw91o, [0.093907, 0.26608, 0.0809615, 100.0, 76.320, 0.57767e-4], True),
+    "PW91": (_pw91, [0.19645, 0.2747, 0.1508, 100.0, 7.7956, 0.004], True),
+    "PW91K": (_pw91, [0.19645, 0.2747, 0.1508, 100.0, 7.7956, 0.004], True),
+    "LG94": (_lg94, [(1e-8 + 0.1234) / 0.024974, 29.790, 22.417, 12.119, 1570.1, 55.944, 0.024974], True),
+    "E00": (_p92, [135.0, 28.0, 5.0, 3.0], True), "P92": (_p92, [1.0, 88.3960, 16.3683, 88.2108], True),
+    "PBE2": (_pbe, [2.0309 / 0.2942, 2.0309], True), "PBE3": (_pbe, [-3.7425 / 4.1355, -3.7425, 50.258], True),
+    "PBE4": (_pbe, [-7.2333 / 1.7107, -7.2333, 61.645, -93.683], True),
+    "P82": (_p82, [5.0 / 27.0], True), "TW02": (_pbe, [0.8438, 0.2319], True),
+    "APBE": (_pbe, [0.804, 0.23889], True), "APBEK": (_pbe, [0.804, 0.23889], True),
+    "REVAPBEK": (_pbe, [1.245, 0.23889], True), "REVAPBE": (_pbe, [1.245, 0.23889], True),
+    "RPBE": (_pbe, [1.9632, 0.01979], True), "VJKS00": (_vjks00, [0.8944, 0.6511, 0.0431], True),
+    "LC94": (_pw91, [0.093907, 0.26608, 0.0809615, 100.0, 76.32, 0.000057767], True),
+    "VT84F": (_vt84f, [2.777028126, 2.777028126 - 40.0 / 27.0], True),
+    "TF": (_tfvw, [1.0, 0.0], True), "VW": (_tfvw, [0.0, 1.0], True), "X_TF_Y_VW": (_tfvw, [1.0, 1.0], True),
+    "TFVW": (_tfvw, [1.0, 1.0], True), "STV": (_stv, [1.0, 1.0, 0.01, 1.0], True),
+    "PBE2M": (_pbe2m, [1.0, 0.2942, 2.0309], True), "PG": (_pg, [0.75, 1.0], True),
+}
+
+
+def gga_params(functional, params=None):
+    """GGA's parameter completion (kedf/gga.py:1263-1274)."""
+    fn, p0, _ = GGA_KEDFS[functional.upper()]
+    if params is None:
+        return list(p0)
+    p = [params] if isinstance(params, (float, int)) else list(params)
+    if len(p) > len(p0):
+        p = p[:len(p0)]
+    elif len(p) < len(p0):
+        p = p + list(p0[len(p):])
+    return p
+
+
+def gga_enhancement(s, functional, params=None):
+    """F(s) and dFds2 of one GGA functional at the reduced gradient s."""
+    fn, _, tk = GGA_KEDFS[functional.upper()]
+    p = gga_params(functional, params)
+    if tk:
+        F, d = fn(s / TKF0, p)
+        return F, d / (TKF0 * TKF0)
+    return fn(s, p)
+
+
+def gga_functional(rho, grid, functional="LKT", params=None, calc=("E", "V")):
+    """GGA: kedf/gga.py:1212-1298 (standard FFT gradient, no sigma)."""
+    rhom = np.where(rho < 1e-16, 1e-16, rho)
+    rho23 = np.cbrt(rhom * rhom)
+    rho53 = rho23 * rhom
+    tf = C_TF * rho53
+    rho43 = rho23 * rho23
+    rho83 = rho43 * rho43
+    rhoG = fft(rho, grid)
+    grad = [ifft(1j * grid.g[i] * rhoG, grid) for i in range(3)]
+    s = np.sqrt(grad[0] ** 2 + grad[1] ** 2 + grad[2] ** 2) / rho43
+    s = np.where(s < 1e-16, 1e-16, s)
+    F, d = gga_enhancement(s, functional, params)
+    E = V = None
+    if "E" in calc:
+        E = (tf * F).sum() * grid.dV
+    if "V" in calc:
+        pot1 = 5.0 / 3.0 * C_TF * rho23 * F - 4.0 / 3.0 * tf * d * s * s / rhom
+        p3 = [fft(tf * d * grad[i] / rho83, grid) for i in range(3)]
+        V = pot1 - ifft(1j * (grid.g[0] * p3[0] + grid.g[1] * p3[1] + grid.g[2] * p3[2]), grid)
     return E, V
 
 

@@ -393,3 +393,47 @@ def test_stress_is_strain_derivative_of_energy():
     assert set(parts) == {"KEDF-TF", "KEDF-VW", "KEDF-NL", "XC", "HARTREE", "PSEUDO", "II", "TOTAL"}
     ref = st["TF"] + st["VW"] + st["NL"] + st["XC"] + st["H"] + st["PP1"] + st["II1"]
     assert np.abs(parts["TOTAL"] - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_gga_lc94_known_answer(fixture_rho):
+    """examples/test/test_functional.py:10-18: GGA KEDF 'lc94' on Al_fde_rho.pp."""
+    ions, rho, grid = fixture_rho
+    assert abs(O.gga_functional(rho, grid, "LC94", calc=("E",))[0] - 1.6821337114254904) < 1e-12
+
+
+# The reference's dFds2 of PW91 (and its aliases PW91K, LC94), PW91O and
+# VT84F is not (1/s) dF/ds of its own F (kedf/gga.py:216-277, :407-433: in
+# PW91, for example, two terms of dFa/ds carry a factor 4 where the
+# derivative has 2).  The oracle keeps the reference's formulas, so these
+# potentials are pinned only by the restatement itself; their energies
+# (LC94: test_functional.py:18) are pinned.
+GGA_DERIVATIVE_MISMATCH = {"PW91", "PW91K", "LC94", "PW91O", "VT84F"}
+
+
+@pytest.mark.parametrize("name", sorted(set(O.GGA_KEDFS) - GGA_DERIVATIVE_MISMATCH))
+def test_gga_potential_and_derivative_consistency(name):
+    """dFds2 = (1/s) dF/ds by central difference on s in (0.05, 4) (away from
+    the reference's own branch switches), and V = dE/drho along a smooth
+    direction, for every GGA enhancement factor whose reference formulas
+    are mutually consistent."""
+    s = np.linspace(0.05, 4.0, 400)
+    h = 1e-6
+    Fp, _ = O.gga_enhancement(s + h, name)
+    Fm, _ = O.gga_enhancement(s - h, name)
+    _, d = O.gga_enhancement(s, name)
+    num = (Fp - Fm) / (2 * h) / s
+    assert np.abs(num - d).max() <= 1e-6 * max(np.abs(d).max(), 1.0)
+    cell = np.eye(3) * 7.0
+    grid = O.Grid(cell, (16, 16, 16), full=False)
+    i = np.arange(16) / 16
+    x, y, z = np.meshgrid(i, i, i, indexing="ij")
+    rho = 0.03 * (1 + 0.3 * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * (y + z)) + 0.2 * np.cos(2 * np.pi * 2 * z))
+    u = 0.003 * np.cos(2 * np.pi * x) * np.sin(2 * np.pi * (y + z)) + 0.002 * np.cos(2 * np.pi * 2 * z) \
+        + 0.001 * np.sin(2 * np.pi * (x - y))
+    E, V = O.gga_functional(rho, grid, name)
+    t = 1e-5
+    fd = (O.gga_functional(rho + t * u, grid, name, calc=("E",))[0]
+          - O.gga_functional(rho - t * u, grid, name, calc=("E",))[0]) / (2 * t)
+    an = np.sum(V * u) * grid.dV
+    scale = np.sum(np.abs(V * u)) * grid.dV
+    assert abs(fd - an) <= 1e-6 * scale, (fd, an, scale)
