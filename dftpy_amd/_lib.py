@@ -24,6 +24,7 @@ CONV_NONE, CONV_HARTREE, CONV_GG, CONV_WT = range(4)
 # ofdft_tn_direction outcomes: the `stat` strings of get_direction_TN
 TN_CONV, TN_WARN, TN_FAILED, TN_NOTREDUCE, TN_SMALL, TN_MAXFUN = range(6)
 NSTRESS = 19  # OFDFT_NSTRESS
+GGA_NOUT = 10  # OFDFT_GGA_NOUT
 
 TN_STATUS = ("CONV", "WARN", "FAILED", "WARN : Not reduce", "WARN : Change too small", "NOTCONV")
 
@@ -88,6 +89,7 @@ _SIGS = {
     "ofdft_functional_stress": (_i, [_vp, _vp, _d, _dp]),
     "ofdft_local_stress": (_i, [_vp, _vp, _dp]),
     "ofdft_ewald_stress": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
+    "ofdft_gga": (_i, [_vp, _i, _dp, _i, _vp, _vp, _vp, _d, _d, _vp, _dp]),
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
     "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),

@@ -145,6 +145,7 @@ struct ofdft_plan_s {
   int* d_species;    // nat
   double* d_bm;      // PME b-moduli: complex n0 + n1 + nbm2
   double* d_work;    // N doubles: PME Q array / rhoPB
+  void* d_gga;       // 4 N (plan precision): GGA pot1 and vector field
   // per-kernel-class event timing (ofdft_plan_set_profiling)
   int prof_on;
   std::vector<hipEvent_t> ev_pool;  // reused across profile windows
@@ -560,6 +561,340 @@ struct PostStressVW {
     acc[3] += (double)c[1] * c[1] * inv;
     acc[4] += (double)c[1] * c[2] * inv;
     acc[5] += (double)c[2] * c[2] * inv;
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+};
+
+// ---------------------------------------------------------------------------
+// GGA kinetic functionals (kedf/gga.py): enhancement factor F(s) and
+// dFds2 = (1/s) dF/ds of the reduced gradient s = |grad rho| / rho^(4/3);
+// the factors marked "ss" take ss = s / TKF0 and return dFds2 / TKF0^2
+// (s_with_tkf0, kedf/gga.py:18-40).  FP64 in every precision.
+// ---------------------------------------------------------------------------
+constexpr double kCTF = 2.87123400018819181594;   // constants.py:82
+constexpr double kTKF0 = 6.18733545256027186194;  // constants.py:83
+constexpr double kCbrt2 = 1.25992104989487316477; // constants.py:84
+
+struct GgaSpec {
+  int id;  // OFDFT_GGA_*
+  int np;
+  double p[8];
+};
+
+__device__ void gga_enhance(const GgaSpec& g, double s, double& F, double& d) {
+  const double* p = g.p;
+  bool tk = true;
+  double x = s / kTKF0;  // ss
+  switch (g.id) {
+    case OFDFT_GGA_LKTVW: {  // :42-55
+      const double e1 = exp(-p[0] * x), e2 = exp(-2.0 * p[0] * x);
+      F = 2.0 * e1 / (1.0 + e2);
+      d = -p[0] * (1.0 - e2) / (1.0 + e2) * F / x;
+      break;
+    }
+    case OFDFT_GGA_LKT: {  // :547-575
+      const double s2 = x * x;
+      if (x > 100.0) {
+        F = 5.0 / 3.0 * s2 * p[1];
+        d = 10.0 / 3.0 * p[1];
+      } else if (x < 1e-5) {
+        const double a2 = p[0] * p[0];
+        F = 1.0 + (5.0 / 3.0 * p[1] - 0.5 * a2) * s2 + 5.0 / 24.0 * a2 * a2 * s2 * s2;
+        d = 10.0 / 3.0 * p[1] - a2 + 5.0 / 6.0 * a2 * a2 * s2 - 61.0 / 120.0 * a2 * a2 * a2 * s2 * s2;
+      } else {
+        const double ch = cosh(p[0] * x);
+        F = 1.0 / ch + 5.0 / 3.0 * s2 * p[1];
+        d = 10.0 / 3.0 * p[1] - p[0] * sinh(p[0] * x) / (ch * ch) / x;
+      }
+      break;
+    }
+    case OFDFT_GGA_DK: {  // :58-74 (s)
+      tk = false;
+      const double t = s * s / (72.0 * kCTF);
+      const double Fa = 9.0 * p[5] * t * t * t * t + p[2] * t * t * t + p[1] * t * t + p[0] * t + 1.0;
+      const double Fb = p[5] * t * t * t + p[4] * t * t + p[3] * t + 1.0;
+      F = Fa / Fb;
+      d = ((36.0 * p[5] * t * t * t + 3 * p[2] * t * t + 2 * p[1] * t + p[0]) / Fb -
+           Fa / (Fb * Fb) * (3.0 * p[5] * t * t + 2.0 * p[4] * t + p[3])) / (36.0 * kCTF);
+      break;
+    }
+    case OFDFT_GGA_LLP: {  // :77-95 (s)
+      tk = false;
+      const double bs = kCbrt2 * s, bs2 = bs * bs;
+      const double Fa = p[0] * bs2, Fb = 1.0 + p[1] * bs * asinh(bs);
+      F = 1.0 + Fa / Fb;
+      d = (2.0 * p[0] / Fb - (p[0] * p[1] * bs * asinh(bs) + Fa * p[1] / sqrt(1.0 + bs2)) / (Fb * Fb)) * kCbrt2 * kCbrt2;
+      break;
+    }
+    case OFDFT_GGA_OL: {  // :98-112 (s)
+      tk = false;
+      F = 1.0 + s * s / 72.0 / kCTF + p[0] / kCTF * s / (1 + p[1] * 4 * s);
+      d = 1.0 / 36.0 / kCTF;
+      if (s > 1.0e-8) {
+        const double q = 1 + p[1] * 4 * s;
+        d += p[0] / kCTF / (q * q) / s;
+      }
+      break;
+    }
+    case OFDFT_GGA_THAK: {  // :115-142 (s)
+      tk = false;
+      const double bs = kCbrt2 * s, bs2 = bs * bs;
+      const double c53 = 3.1748021039363989495;  // 2^(5/3)
+      F = 1.0 + p[0] * bs2 / (1.0 + p[1] * bs * asinh(bs)) - p[2] * bs / (1.0 + c53 * bs);
+      const double fb = 1.0 + p[1] * bs * asinh(bs);
+      d = (-(p[0] * p[1] * bs2) / sqrt(1 + bs2) + (p[0] * p[1] * bs * asinh(bs)) + 2.0 * p[0]) / (fb * fb);
+      if (s > 1.0e-8) {
+        const double q = 1.0 + c53 * bs;
+        d -= p[2] / (q * q) / bs;
+      }
+      d *= kCbrt2 * kCbrt2;
+      break;
+    }
+    case OFDFT_GGA_B86A: {  // :145-159 (s)
+      tk = false;
+      const double bs2 = kCbrt2 * s * kCbrt2 * s, Fb = 1.0 + p[1] * bs2;
+      F = 1.0 + p[0] * bs2 / Fb;
+      d = 2 * p[0] / (Fb * Fb) * (kCbrt2 * kCbrt2);
+      break;
+    }
+    case OFDFT_GGA_B86B: {  // :162-176 (s)
+      tk = false;
+      const double bs2 = kCbrt2 * s * kCbrt2 * s, q = 1.0 + p[1] * bs2, Fb = pow(q, 4.0 / 5.0);
+      F = 1.0 + p[0] * bs2 / Fb;
+      d = (2 * p[0] * (p[1] * bs2 + 5.0)) / (5 * q * Fb) * (kCbrt2 * kCbrt2);
+      break;
+    }
+    case OFDFT_GGA_DK87: {  // :179-195 (s)
+      tk = false;
+      const double bs = kCbrt2 * s, bs2 = bs * bs;
+      const double Fa = p[0] * bs2 * (1 + p[1] * bs), Fb = 1.0 + p[2] * bs2;
+      F = 1.0 + Fa / Fb;
+      d = p[0] * (2.0 + 3.0 * p[1] * bs + p[1] * p[2] * bs2 * bs) / (Fb * Fb) * (kCbrt2 * kCbrt2);
+      break;
+    }
+    case OFDFT_GGA_PW86: {  // :198-213 (ss)
+      const double s2 = x * x, Fa = 1.0 + p[0] * s2 + p[1] * s2 * s2 + p[2] * s2 * s2 * s2;
+      F = pow(Fa, 1.0 / 15.0);
+      d = 2.0 / 15.0 * (p[0] + 2 * p[1] * s2 + 3 * p[2] * s2 * s2) / pow(Fa, 14.0 / 15.0);
+      break;
+    }
+    case OFDFT_GGA_PW91O: {  // :216-237 (ss)
+      const double s2 = x * x, s4 = s2 * s2, ex = exp(-p[3] * s2);
+      const double Fa = (p[1] - p[2] * ex) * s2 - p[5] * s4;
+      const double Fb = 1.0 + p[0] * x * asinh(p[4] * x) + p[5] * s4;
+      const double Fa_s2 = (p[1] - p[2] * ex) - p[5] * s2;
+      F = 1.0 + Fa / Fb;
+      d = 2.0 * (p[1] + (p[3] * s2 - 1) * p[2] * ex - 4.0 * p[5] * s2) / Fb -
+          Fa_s2 * x / (Fb * Fb) * ((p[0] * p[4] * x) / (p[4] * p[4] * s2 + 1) + p[0] * asinh(p[4] * x) + 4.0 * p[5] * s2);
+      break;
+    }
+    case OFDFT_GGA_PW91: {  // :240-277 (ss); dFds2 as the reference writes it
+      const double s2 = x * x, ex = exp(-p[3] * s2), ash = asinh(p[4] * x);
+      const double Fa = 1.0 + p[0] * x * ash + (p[1] - p[2] * ex) * s2;
+      const double Fb = 1.0 + p[0] * x * ash + p[5] * s2 * s2;
+      const double sq = sqrt(p[4] * p[4] * s2 + 1);
+      const double Fa_s = p[0] * p[4] * x / sq + p[0] * ash + 2.0 * x * (p[1] + 2.0 * p[2] * p[3] * s2 * ex - 2.0 * p[2] * ex);
+      F = Fa / Fb;
+      d = Fa_s / Fb - Fa / (Fb * Fb) * (p[0] * p[4] * x / sq + (p[0] * ash + 4.0 * p[5] * s2 * x));
+      if (x > 1.0e-8) d /= x;
+      break;
+    }
+    case OFDFT_GGA_LG94: {  // :280-316 (ss)
+      const double s2 = x * x, s4 = s2 * s2, s6 = s4 * s2, s8 = s4 * s4, s10 = s4 * s6, s12 = s6 * s6;
+      const double Fa = 1.0 + p[0] * s2 + p[1] * s4 + p[2] * s6 + p[3] * s8 + p[4] * s10 + p[5] * s12;
+      const double Fb = 1.0 + 1e-8 * s2;
+      F = pow(Fa / Fb, p[6]);
+      d = -2 * p[6] * F *
+          (1e-8 * Fa - Fb * (p[0] + 2 * p[1] * s2 + 3 * p[2] * s4 + 4 * p[3] * s6 + 5 * p[4] * s8 + 6 * p[5] * s10)) /
+          (Fa * Fb);
+      break;
+    }
+    case OFDFT_GGA_P92: {  // :319-335 (ss)
+      const double s2 = x * x, Fa = p[0] + p[1] * s2 + p[2] * s2 * s2, Fb = p[0] + p[3] * s2;
+      F = Fa / Fb;
+      d = (2.0 * p[1] + 4.0 * p[2] * s2) / Fb - (2.0 * p[3] * Fa) / (Fb * Fb);
+      break;
+    }
+    case OFDFT_GGA_PBE: {  // :338-368 (ss), 2-4 parameters
+      const double s2 = x * x, Fb = 1.0 + p[1] / p[0] * s2, Fb2 = Fb * Fb;
+      F = 1.0 + p[1] * s2 / Fb;
+      d = 2.0 * p[1] / Fb2;
+      if (g.np > 2) {
+        const double Fb3 = Fb2 * Fb;
+        F += p[2] * s2 * s2 / Fb2;
+        d += 4 * p[2] * s2 / Fb3;
+        if (g.np > 3) {
+          F += p[3] * s2 * s2 * s2 / Fb3;
+          d += 6 * p[3] * s2 * s2 / (Fb3 * Fb);
+        }
+      }
+      break;
+    }
+    case OFDFT_GGA_P82: {  // :371-385 (ss)
+      const double s2 = x * x, s6 = s2 * s2 * s2, Fb = 1 + s6;
+      F = 1.0 + p[0] * s2 / Fb;
+      d = p[0] * (2.0 / Fb - 6.0 * s6 / (Fb * Fb));
+      break;
+    }
+    case OFDFT_GGA_VJKS00: {  // :388-404 (ss)
+      const double s2 = x * x, s4 = s2 * s2, s6 = s4 * s2;
+      const double Fa = 1.0 + p[0] * s2 - p[2] * s6, Fb = 1.0 + p[1] * s2 + p[2] * s4;
+      F = Fa / Fb;
+      d = (2.0 * p[0] - 6.0 * p[2] * s4) / Fb - (2.0 * p[1] + 4.0 * p[2] * s2) * Fa / (Fb * Fb);
+      break;
+    }
+    case OFDFT_GGA_VT84F: {  // :407-433 (ss), s^2 floored at 1e-8
+      double s2 = x * x;
+      if (s2 < 1.0e-8) s2 = 1.0e-8;
+      const double s4 = s2 * s2, e1 = exp(-p[1] * s2), e2 = exp(-p[1] * s4), q = 1 + p[0] * s2;
+      F = 1.0 + 5.0 / 3.0 * s2 + p[0] * s2 * e1 / q + (1 - e2) * (1.0 / s2 - 1.0);
+      d = 10.0 / 3.0 + 2.0 * p[0] * e1 * (1.0 - p[1] * s2) / q - 2.0 * p[0] * s2 * e1 / (q * q) +
+          4.0 * p[1] * s2 * (1.0 / s2 - 1.0) * e2 - 2.0 * (1 - e2) / s4;
+      break;
+    }
+    case OFDFT_GGA_TFVW: {  // :436-447 (ss)
+      F = p[0] + 5.0 / 3.0 * p[1] * x * x;
+      d = 10.0 / 3.0 * p[1];
+      break;
+    }
+    case OFDFT_GGA_STV: {  // :450-462 (ss)
+      const double s2 = x * x, Fb = p[3] + p[2] * s2;
+      F = p[0] + 5.0 / 3.0 * p[1] * s2 / Fb;
+      d = 5.0 / 3.0 * p[1] * (2.0 / Fb - 2.0 * s2 * p[2] / (Fb * Fb));
+      break;
+    }
+    case OFDFT_GGA_PBE2M: {  // :465-478 (ss)
+      const double s2 = x * x, Fb = p[0] + p[1] * s2;
+      F = 1.0 + p[2] * s2 / Fb;
+      d = 2.0 * p[2] / (Fb * Fb);
+      break;
+    }
+    default: {  // OFDFT_GGA_PG, :481-490 (ss)
+      const double s2 = x * x, Fa = exp(-p[0] * s2);
+      F = Fa + 5.0 / 3.0 * p[1] * s2;
+      d = -2.0 * p[0] * Fa + 10.0 / 3.0 * p[1];
+    }
+  }
+  if (tk) d /= kTKF0 * kTKF0;
+}
+
+// rho (mode rho) or (c phi + s p)^2 as a single-field row-transform input
+template <typename T>
+struct PreRho {
+  static constexpr int kProf = PC_ROWS_FWD;
+  const T* __restrict__ rho;
+  const T* __restrict__ phi;
+  const T* __restrict__ p;
+  T c, s;
+  __host__ __device__ int nin() const { return rho ? 1 : (p ? 2 : 1); }
+  struct In {
+    T a, b;
+  };
+  __device__ __forceinline__ In load(long long i) const {
+    if (rho) return {rho[i], T(0)};
+    return {phi[i], p ? p[i] : T(0)};
+  }
+  __device__ __forceinline__ double dens(const In& x) const {
+    if (rho) return (double)x.a;
+    const double ph = p ? (double)c * x.a + (double)s * x.b : (double)x.a;
+    return ph * ph;
+  }
+  __device__ __forceinline__ void apply(const In& x, T* out) const { out[0] = T(dens(x)); }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+};
+
+// three stored fields as the row-transform inputs (w[f*N + i])
+template <typename T>
+struct PreCopy3 {
+  static constexpr int kProf = PC_ROWS_FWD;
+  const T* __restrict__ w;
+  long long N;
+  __host__ __device__ int nin() const { return 3; }
+  struct In {
+    T a, b, c;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {w[i], w[N + i], w[2 * N + i]}; }
+  __device__ __forceinline__ void apply(const In& x, T* out) const {
+    out[0] = x.a;
+    out[1] = x.b;
+    out[2] = x.c;
+  }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+};
+
+// First GGA pass, on the gradient c[0..2] of rho (GGA, kedf/gga.py:1240-1283,
+// :1299-1311): energy density tf F, pot1, the vector field
+// tf dFds2 grad rho / rho^(8/3) (stored for the divergence) and the stress
+// sums.  acc: [sum tf F, sum pot1 rho, 6 x sum vkin2 d_a rho d_b rho].
+template <typename T>
+struct PostGga1 {
+  static constexpr int NS = 8;
+  static constexpr int kProf = PC_ROWS_INV;
+  PreRho<T> r;
+  GgaSpec g;
+  T* __restrict__ W;  // [pot1 | w_x | w_y | w_z], 4 N
+  long long N;
+  __host__ __device__ int nio() const { return r.nin() + (W ? 4 : 0); }
+  using In = typename PreRho<T>::In;
+  __device__ __forceinline__ In load(long long i) const { return r.load(i); }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const double rho = r.dens(in);
+    const double rhom = rho < 1e-16 ? 1e-16 : rho;
+    const double rho23 = cbrt(rhom * rhom);
+    const double tf = kCTF * rho23 * rhom;
+    const double rho43 = rho23 * rho23, rho83 = rho43 * rho43;
+    const double g0 = c[0], g1 = c[1], g2 = c[2];
+    double s = sqrt(g0 * g0 + g1 * g1 + g2 * g2) / rho43;
+    if (s < 1e-16) s = 1e-16;
+    double F, d;
+    gga_enhance(g, s, F, d);
+    const double pot1 = 5.0 / 3.0 * kCTF * rho23 * F - 4.0 / 3.0 * tf * d * s * s / rhom;
+    const double vk = tf * d / rho83;
+    acc[0] += tf * F;
+    acc[1] += pot1 * rho;
+    acc[2] += vk * g0 * g0;
+    acc[3] += vk * g0 * g1;
+    acc[4] += vk * g0 * g2;
+    acc[5] += vk * g1 * g1;
+    acc[6] += vk * g1 * g2;
+    acc[7] += vk * g2 * g2;
+    if (W) {
+      W[i] = T(pot1);
+      W[N + i] = T(vk * g0);
+      W[2 * N + i] = T(vk * g1);
+      W[3 * N + i] = T(vk * g2);
+    }
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+};
+
+// Second GGA pass: V_gga = pot1 - div (c[0]), added to V; acc: [sum V_gga rho,
+// 2 sum V_gga phi(t) dphi/dt] (the terms the evaluation's mu and line-search
+// derivative sums get from this functional).
+template <typename T>
+struct PostGga2 {
+  static constexpr int NS = 2;
+  static constexpr int kProf = PC_ROWS_INV;
+  PreRho<T> r;
+  const T* __restrict__ pot1;
+  T* __restrict__ V;
+  __host__ __device__ int nio() const { return r.nin() + 3; }
+  struct In {
+    typename PreRho<T>::In x;
+    T p1, v;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {r.load(i), pot1[i], V[i]}; }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const double vg = (double)in.p1 - (double)c[0];
+    V[i] = T((double)in.v + vg);
+    acc[0] += vg * r.dens(in.x);
+    if (!r.rho && r.p) {
+      const double ph = (double)r.c * in.x.a + (double)r.s * in.x.b;
+      const double dph = (double)r.c * in.x.b - (double)r.s * in.x.a;
+      acc[1] += 2.0 * vg * ph * dph;
+    }
   }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
 };
@@ -2670,7 +3005,8 @@ __global__ void k_stress_recip(StressArgs A, GridGeom geo, int mode, double* __r
 // G_a is replaced by its Hermitian part (G(k) - G(-k))/2, so that the
 // half-spectrum inverse equals Re(ifftn) of the full array (vw.py:97-112
 // with ReciprocalField.ifft(force_real=True)).
-__global__ void k_grad_spec(cplx<double>* __restrict__ spec, long long Nhp, GridGeom geo, double scale) {
+template <typename T>
+__global__ void k_grad_spec(cplx<T>* __restrict__ spec, long long Nhp, GridGeom geo, double scale) {
   const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
   for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
        idx += (long long)gridDim.x * blockDim.x) {
@@ -2678,7 +3014,7 @@ __global__ void k_grad_spec(cplx<double>* __restrict__ spec, long long Nhp, Grid
     const long long r = idx / geo.nh;
     const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
     const long long ph = r * geo.nhp + c;
-    const cplx<double> z = spec[ph];
+    const cplx<T> z = spec[ph];
     double G[3];
     gvec(geo, k0, k1, c, 0, G);
     if (geo.full) {
@@ -2687,7 +3023,39 @@ __global__ void k_grad_spec(cplx<double>* __restrict__ spec, long long Nhp, Grid
       for (int a = 0; a < 3; ++a) G[a] = 0.5 * (G[a] - Gp[a]);
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) spec[a * Nhp + ph] = cplx<double>{-G[a] * z.y * scale, G[a] * z.x * scale};
+    for (int a = 0; a < 3; ++a)
+      spec[a * Nhp + ph] = cplx<T>{T(-G[a] * (double)z.y * scale), T(G[a] * (double)z.x * scale)};
+  }
+}
+
+// Divergence spectrum for the GGA potential: slot 0 = i (G . W) / N from the
+// three spectra W_a in slots 0..2 (all read before slot 0 is written); the
+// same Hermitian part of G as k_grad_spec with full=1 frequencies
+// (kedf/gga.py:1284-1290).
+template <typename T>
+__global__ void k_div_spec(cplx<T>* __restrict__ spec, long long Nhp, GridGeom geo, double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const long long ph = r * geo.nhp + c;
+    double G[3];
+    gvec(geo, k0, k1, c, 0, G);
+    if (geo.full) {
+      double Gp[3];
+      gvec(geo, k0, k1, c, 1, Gp);
+      for (int a = 0; a < 3; ++a) G[a] = 0.5 * (G[a] - Gp[a]);
+    }
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const cplx<T> w = spec[a * Nhp + ph];
+      re += G[a] * (double)w.x;
+      im += G[a] * (double)w.y;
+    }
+    spec[ph] = cplx<T>{T(-im * scale), T(re * scale)};
   }
 }
 
@@ -2755,6 +3123,55 @@ int r2c_pre_slot(ofdft_plan_s* P, const Pre& pre, int slot) {
 }
 
 int r2c_slot(ofdft_plan_s* P, const double* in, int slot) { return r2c_pre_slot(P, PreCopy<double>{in}, slot); }
+
+// The GGA kinetic functional at rho (or (c phi + s p)^2): forward transform
+// of rho, three gradient fields, first pass (energy, stress sums, pot1 and
+// the vector field), then -- with V -- the divergence of the vector field
+// and V += pot1 - div (GGA, kedf/gga.py:1212-1311).
+template <typename T>
+int gga_impl(ofdft_plan_s* P, const GgaSpec& g, const T* rho, const T* phi, const T* p, double c, double s, T* V,
+             double* out) {
+  const long long N = P->N;
+  if (V && !P->d_gga) HIPCHK(hipMalloc(&P->d_gga, 4 * (size_t)N * sizeof(T)));
+  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
+  PreRho<T> pr{rho, phi, p, T(c), T(s)};
+  int rc;
+  if ((rc = launch_rows_fwd<T>(P, pr, 1))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 1, 1, false))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 0, 1, false))) return rc;
+  hipLaunchKernelGGL((k_grad_spec<T>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nhp, geom(P),
+                     1.0 / (double)N);
+  LAUNCHCHK();
+  if ((rc = launch_axis<T>(P, spec, 0, 3, true))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 1, 3, true))) return rc;
+  T* W = V ? reinterpret_cast<T*>(P->d_gga) : nullptr;
+  PostGga1<T> post1{pr, g, W, N};
+  if ((rc = launch_rows_inv<T>(P, post1, 3, spec, true))) return rc;
+  double s1[8];
+  if ((rc = fetch_sums(P, 8, s1))) return rc;
+  const double dV = P->dV;
+  for (int k = 0; k < OFDFT_GGA_NOUT; ++k) out[k] = 0.0;
+  out[0] = s1[0] * dV;
+  out[3] = s1[1] * dV;
+  for (int k = 0; k < 6; ++k) out[4 + k] = s1[2 + k] * dV;
+  if (!V) return 0;
+  PreCopy3<T> p3{W + N, N};
+  if ((rc = launch_rows_fwd<T>(P, p3, 3))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 1, 3, false))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 0, 3, false))) return rc;
+  hipLaunchKernelGGL((k_div_spec<T>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nhp, geom(P),
+                     1.0 / (double)N);
+  LAUNCHCHK();
+  if ((rc = launch_axis<T>(P, spec, 0, 1, true))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 1, 1, true))) return rc;
+  PostGga2<T> post2{pr, W, V};
+  if ((rc = launch_rows_inv<T>(P, post2, 1, spec, true))) return rc;
+  double s2[2];
+  if ((rc = fetch_sums(P, 2, s2))) return rc;
+  out[2] = s2[0] * dV;
+  out[1] = s2[1] * dV;
+  return 0;
+}
 
 // six stress sums of k_stress_recip over the half spectrum into out[0..6)
 int stress_recip(ofdft_plan_s* P, const StressArgs& A, int mode, double* out) {
@@ -3109,6 +3526,7 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->d_species) (void)hipFree(P->d_species);
   if (P->d_bm) (void)hipFree(P->d_bm);
   if (P->d_work) (void)hipFree(P->d_work);
+  if (P->d_gga) (void)hipFree(P->d_gga);
   if (P->red) (void)hipFree(P->red);
   if (P->hred) (void)hipHostFree(P->hred);
   if (P->tnst) (void)hipFree(P->tnst);
@@ -3547,8 +3965,8 @@ int ofdft_functional_stress(ofdft_plan P, const void* rho, double tkf, double* o
   }
   if (f.vw) {
     if ((rc = r2c_slot(P, r, 0))) return rc;
-    hipLaunchKernelGGL(k_grad_spec, dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nhp, geom(P),
-                       1.0 / (double)P->N);
+    hipLaunchKernelGGL((k_grad_spec<double>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, P->Nhp,
+                       geom(P), 1.0 / (double)P->N);
     LAUNCHCHK();
     if ((rc = launch_axis<double>(P, spec, 0, 3, true))) return rc;
     if ((rc = launch_axis<double>(P, spec, 1, 3, true))) return rc;
@@ -3638,6 +4056,21 @@ int ofdft_ewald_stress(ofdft_plan P, int nat, const double* positions, const dou
     A.nat = nat;
   }
   return stress_recip(P, A, kStEwald, rec6);
+}
+
+int ofdft_gga(ofdft_plan P, int functional, const double* params, int nparams, const void* rho, const void* phi,
+              const void* p, double c, double s, void* V_inout, double* out) {
+  if (!valid(P) || !out || (!rho && !phi)) return fail(OFDFT_EINVAL, "null argument");
+  if (functional < 0 || functional >= OFDFT_GGA_COUNT) return fail(OFDFT_EINVAL, "unknown GGA functional");
+  if (nparams < 0 || nparams > 8 || (nparams > 0 && !params)) return fail(OFDFT_EINVAL, "bad GGA parameters");
+  GgaSpec g{};
+  g.id = functional;
+  g.np = nparams;
+  for (int k = 0; k < nparams; ++k) g.p[k] = params[k];
+  if (P->prec == OFDFT_F64)
+    return gga_impl<double>(P, g, (const double*)rho, (const double*)phi, (const double*)p, c, s, (double*)V_inout,
+                            out);
+  return gga_impl<float>(P, g, (const float*)rho, (const float*)phi, (const float*)p, c, s, (float*)V_inout, out);
 }
 
 

@@ -150,6 +150,56 @@ _KEDF_NAMES = {
 }
 
 
+# GGA kinetic functionals: name -> (device id, default parameters), the
+# registry of kedf/gga.py:577-635 (ids: OFDFT_GGA_* of the C ABI)
+_G = dict(LKTVW=0, LKT=1, DK=2, LLP=3, OL=4, THAK=5, B86A=6, B86B=7, DK87=8, PW86=9, PW91O=10, PW91=11, LG94=12,
+          P92=13, PBE=14, P82=15, VJKS00=16, VT84F=17, TFVW=18, STV=19, PBE2M=20, PG=21)
+GGA_KEDF_LIST = {
+    "LKT-VW": (_G["LKTVW"], [1.3]), "LKT": (_G["LKT"], [1.3, 1.0]),
+    "DK": (_G["DK"], [0.95, 14.28111, -19.5762, -0.05, 9.99802, 2.96085]),
+    "LLP": (_G["LLP"], [0.0044188, 0.0253]), "LLP91": (_G["LLP"], [0.0044188, 0.0253]),
+    "OL1": (_G["OL"], [0.00677, 0.0]), "OL": (_G["OL"], [0.00677, 0.0]), "OL2": (_G["OL"], [0.0887, 1.0]),
+    "T92": (_G["THAK"], [0.0055, 0.0253, 0.072]), "THAK": (_G["THAK"], [0.0055, 0.0253, 0.072]),
+    "B86A": (_G["B86A"], [0.0039, 0.004]), "B86": (_G["B86A"], [0.0039, 0.004]),
+    "B86B": (_G["B86B"], [0.00403, 0.007]),
+    "DK87": (_G["DK87"], [7.0 / 324.0 / (18.0 * np.pi ** 4) ** (1.0 / 3.0), 0.861504, 0.044286]),
+    "PW86": (_G["PW86"], [1.296, 14.0, 0.2]),
+    "PW91O": (_G["PW91O"], [0.093907, 0.26608, 0.0809615, 100.0, 76.320, 0.57767e-4]),
+    "PW91": (_G["PW91"], [0.19645, 0.2747, 0.1508, 100.0, 7.7956, 0.004]),
+    "PW91K": (_G["PW91"], [0.19645, 0.2747, 0.1508, 100.0, 7.7956, 0.004]),
+    "LG94": (_G["LG94"], [(1e-8 + 0.1234) / 0.024974, 29.790, 22.417, 12.119, 1570.1, 55.944, 0.024974]),
+    "E00": (_G["P92"], [135.0, 28.0, 5.0, 3.0]), "P92": (_G["P92"], [1.0, 88.3960, 16.3683, 88.2108]),
+    "PBE2": (_G["PBE"], [2.0309 / 0.2942, 2.0309]), "PBE3": (_G["PBE"], [-3.7425 / 4.1355, -3.7425, 50.258]),
+    "PBE4": (_G["PBE"], [-7.2333 / 1.7107, -7.2333, 61.645, -93.683]),
+    "P82": (_G["P82"], [5.0 / 27.0]), "TW02": (_G["PBE"], [0.8438, 0.2319]),
+    "APBE": (_G["PBE"], [0.804, 0.23889]), "APBEK": (_G["PBE"], [0.804, 0.23889]),
+    "REVAPBEK": (_G["PBE"], [1.245, 0.23889]), "REVAPBE": (_G["PBE"], [1.245, 0.23889]),
+    "RPBE": (_G["PBE"], [1.9632, 0.01979]), "VJKS00": (_G["VJKS00"], [0.8944, 0.6511, 0.0431]),
+    "LC94": (_G["PW91"], [0.093907, 0.26608, 0.0809615, 100.0, 76.32, 0.000057767]),
+    "VT84F": (_G["VT84F"], [2.777028126, 2.777028126 - 40.0 / 27.0]),
+    "TF": (_G["TFVW"], [1.0, 0.0]), "VW": (_G["TFVW"], [0.0, 1.0]), "X_TF_Y_VW": (_G["TFVW"], [1.0, 1.0]),
+    "TFVW": (_G["TFVW"], [1.0, 1.0]), "STV": (_G["STV"], [1.0, 1.0, 0.01, 1.0]),
+    "PBE2M": (_G["PBE2M"], [1.0, 0.2942, 2.0309]), "PG": (_G["PG"], [0.75, 1.0]),
+}
+
+
+def gga_spec(functional, params=None):
+    """(device id, parameters) with the reference's parameter completion
+    (kedf/gga.py:1263-1274)."""
+    key = str(functional).upper()
+    if key not in GGA_KEDF_LIST:
+        raise AttributeError("%s GGA KEDF to be implemented" % functional)
+    gid, p0 = GGA_KEDF_LIST[key]
+    if params is None:
+        return gid, list(p0)
+    p = [params] if isinstance(params, (float, int)) else list(params)
+    if len(p) > len(p0):
+        p = p[:len(p0)]
+    elif len(p) < len(p0):
+        p = p + list(p0[len(p):])
+    return gid, [float(x) for x in p]
+
+
 class _KernelCache(dict):
     """ke_kernel_saved (kedf/__init__.py:101-111): WT kernel density and shape."""
 
@@ -169,21 +219,33 @@ class KEDF(AbstractFunctional):
         self.type = "KEDF"
         self.name = (kedf or name)
         key = self.name.upper()
-        if key not in _KEDF_NAMES:
+        self.gga = None
+        if key == "GGA" or key.startswith("GGA_"):
+            # GGA with k_str (kedf/__init__.py:193-212 name parsing)
+            k_str = kwargs.get("k_str") if key == "GGA" else self.name[4:]
+            if k_str is None:
+                k_str = "LKT"
+            self.gga = gga_spec(k_str, kwargs.get("params"))
+            self.gga_name = str(k_str).upper()
+            key = "GGA"
+            if kwargs.get("sigma") is not None or kwargs.get("gga_remove_vw") is not None:
+                raise AttributeError("GGA sigma smoothing and gga_remove_vw are not on the device path")
+        elif key not in _KEDF_NAMES:
             raise AttributeError("%s KEDF to be implemented" % self.name)
         self.options = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=None)
         self.options.update({k: v for k, v in kwargs.items() if k in ("x", "y", "alpha", "beta", "rho0", "sigma")})
         if self.options.get("sigma") is not None:
             raise AttributeError("vW smoothing (sigma) is not on the device path")
         self.options["kedf"] = self.name
-        self.engines = _KEDF_NAMES[key]
+        self.engines = ("GGA",) if self.gga else _KEDF_NAMES[key]
         self.ke_kernel_saved = _KernelCache()
         self.energies = {}
 
     def _device_terms(self):
         o = self.options
         return dict(tf="TF" in self.engines, vw="VW" in self.engines, wt="WT-NL" in self.engines,
-                    x=o["x"], y=o["y"], alpha=o["alpha"], beta=o["beta"], rho0=o["rho0"], owner=self)
+                    x=o["x"], y=o["y"], alpha=o["alpha"], beta=o["beta"], rho0=o["rho0"], owner=self,
+                    gga=self.gga)
 
     def _total(self):
         if getattr(self, "_tot", None) is None:
@@ -338,6 +400,14 @@ def _density_stress(total, rho, energy_xc=None):
         out["XC"] = eye * ((e - raw[18] * dV) / vol)
     if t["hartree"]:
         out["HARTREE"] = sym6(raw[0:6]) * (dV * dV * 8.0 * np.pi / vol ** 2) - eye * (parts["HARTREE"] / vol)
+    if t["gga"] is not None:
+        # GGA 'S' branch (kedf/gga.py:1312-1326): p = E - sum pot1 rho - sum_i vkin2 (d_i rho)^2
+        gid, prm = t["gga"]
+        with torch._C.DisableTorchFunctionSubclass():
+            g = ev.P.gga(gid, prm, rho=r)
+        pair = sym6(g[4:10])
+        pg = g[0] - g[3] - np.trace(pair)
+        out["GGA"] = (eye * pg - pair) / vol
     return out, parts
 
 
@@ -379,20 +449,36 @@ class DeviceEvaluator:
         return False
 
     def energies(self, sums):
-        return {k: float(sums[i]) for k, i, flag in _SPLIT_KEYS if self.terms[flag]}
+        out = {k: float(sums[i]) for k, i, flag in _SPLIT_KEYS if self.terms[flag]}
+        if self.terms["gga"] is not None:
+            out["KEDF-GGA"] = float(sums[L.NSUM])
+        return out
+
+    def _with_gga(self, sums, **kw):
+        """Append the GGA term (ofdft_gga) to the fused evaluation's sums:
+        its energy in slot NSUM, its V rho and line-search derivative added
+        to S_VRHO and S_DTHETA; V (when given) already holds the sum."""
+        if self.terms["gga"] is None:
+            return sums
+        gid, prm = self.terms["gga"]
+        g = self.P.gga(gid, prm, **kw)
+        out = np.concatenate([sums, [g[0]]])
+        out[L.S_VRHO] += g[2]
+        out[L.S_DTHETA] += g[1]
+        return out
 
     def eval_rho(self, rho, V=None):
         if self.terms["wt"] and self.terms.get("rho0_fixed") is None:
             with torch._C.DisableTorchFunctionSubclass():
                 self.ensure_kernel(float(rho.to(torch.float64).mean()))
-        return self.P.eval(rho, V)
+        return self._with_gga(self.P.eval(rho, V), rho=rho, V=V)
 
     def eval_phi(self, phi, p, c, s, V, aux):
         sums = self.P.eval_phi(phi, p, c, s, V, aux)
         if self.terms["wt"] and self.terms.get("rho0_fixed") is None:
             if self.ensure_kernel(sums[L.S_RHO] / self.grid.volume):
                 sums = self.P.eval_phi(phi, p, c, s, V, aux)
-        return sums
+        return self._with_gga(sums, phi=phi, p=p, c=c, s=s, V=V)
 
 
 class TotalFunctional(AbstractFunctional):
@@ -449,7 +535,7 @@ class TotalFunctional(AbstractFunctional):
 
     def _merged_terms(self):
         t = dict(tf=False, vw=False, wt=False, lda=False, hartree=False, pseudo=False, x=1.0, y=1.0,
-                 alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0_fixed=None, pseudo_obj=None)
+                 alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0_fixed=None, pseudo_obj=None, gga=None)
         seen = set()
         for key, f in self.funcDict.items():
             d = f._device_terms()
@@ -461,6 +547,10 @@ class TotalFunctional(AbstractFunctional):
                         raise ValueError("term %s appears twice in the TotalFunctional" % flag)
                     seen.add(flag)
                     t[flag] = True
+            if d.get("gga") is not None:
+                if t["gga"] is not None:
+                    raise ValueError("two GGA kinetic terms in the TotalFunctional")
+                t["gga"] = d["gga"]
             if f.type == "KEDF":
                 t.update(x=d["x"], y=d["y"], alpha=d["alpha"], beta=d["beta"], rho0_fixed=d["rho0"])
                 if d["wt"] and abs(d["alpha"] - d["beta"]) > 1e-9:
@@ -541,7 +631,7 @@ class TotalFunctional(AbstractFunctional):
         dens, parts = _density_stress(self, rho)
         stress = {"TOTAL": np.zeros((3, 3))}
         for k, v in dens.items():
-            key = "KEDF-" + k.split("-")[-1] if k in ("TF", "VW", "WT-NL") else k
+            key = "KEDF-" + k.split("-")[-1] if k in ("TF", "VW", "WT-NL", "GGA") else k
             stress[key] = v
             stress["TOTAL"] = stress["TOTAL"] + v
         for f in self.funcDict.values():

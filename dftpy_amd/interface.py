@@ -6,7 +6,7 @@ OptimizeDensityConf and evaluator2results (interface.py:21-317).
 The option table below carries the sections, keys, value types and
 defaults of the reference's configuration schema (configentries.json) for
 the options this path runs: the density optimization of one cell with the
-local pseudopotential, LDA, Hartree and the TF / vW / TFvW / WT KEDFs,
+local pseudopotential, LDA, Hartree and the TF / vW / TFvW / WT / GGA KEDFs,
 energies, forces and stress.  Keys of the schema that select something the
 GPU path does not run (spin, atomic initial densities, multistep grids,
 LibXC strings, other KEDF options) are accepted at their defaults and
@@ -78,7 +78,8 @@ SCHEMA = {
     "EXC": {"xc": ("str", None), "libxc": ("lstrlist", ["lda_x", "lda_c_pz"]), "x_str": ("str", None),
             "c_str": ("str", None)},
     "KEDF": {"kedf": ("ustr", "WT"), "x": ("float", 1.0), "y": ("float", 1.0), "alpha": ("float", None),
-             "beta": ("float", None), "sigma": ("float", None), "rho0": ("float", None)},
+             "beta": ("float", None), "sigma": ("float", None), "rho0": ("float", None), "k_str": ("str", None),
+             "params": ("floatlist", None)},
     "OPT": {"method": ("ustr", "CG-HS"), "algorithm": ("ustr", "EMM"), "vector": ("str", "Orthogonalization"),
             "c1": ("float", 1e-4), "c2": ("float", 0.2), "maxls": ("int", 10), "econv": ("float", 1e-6),
             "maxfun": ("int", 50), "maxiter": ("int", 100), "xtol": ("float", 1e-12), "h0": ("float", 1.0)},

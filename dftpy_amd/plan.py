@@ -152,6 +152,17 @@ class Plan:
                                         self._real(V, "V"), auxp, sums))
         return L.as_numpy(sums)
 
+    def gga(self, functional, params, rho=None, phi=None, p=None, c=1.0, s=0.0, V=None):
+        """ofdft_gga: the GGA kinetic term at rho or (c phi + s p)^2; V += V_GGA
+        when V is given.  Returns the OFDFT_GGA_NOUT sums."""
+        out = L.darr(L.GGA_NOUT)
+        prm = (ctypes.c_double * max(len(params), 1))(*params)
+        self._stream()
+        L.check(self.lib.ofdft_gga(self._h, int(functional), prm, len(params), self._real(rho, "rho"),
+                                   self._real(phi, "phi"), self._real(p, "p"), float(c), float(s), self._real(V, "V"),
+                                   out))
+        return L.as_numpy(out)
+
     def hvp(self, phi, p, V, aux, mu, Ap):
         out = ctypes.c_double()
         self._stream()

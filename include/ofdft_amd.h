@@ -272,6 +272,45 @@ int ofdft_local_stress(ofdft_plan plan, const void* rho, double* out6);
 int ofdft_ewald_stress(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta,
                        double rmax, int pme, int order, double* real6, double* rec6);
 
+/* GGA kinetic functionals (GGA, kedf/gga.py:1212-1311, enhancement factors
+ * :42-575, names and default parameters :577-635): one functional id below,
+ * its parameters (the reference's list, completed with the defaults by the
+ * caller).  The density is rho (device, N values of the plan precision) or,
+ * with rho = NULL, (c phi + s p)^2 (p may be NULL).  V_inout (device, may
+ * be NULL) receives V += V_GGA; without it only the energy and the stress
+ * sums are formed.  out[OFDFT_GGA_NOUT] (host, FP64, all x dV):
+ *   [0] E_GGA = sum C_TF rho^(5/3) F(s)
+ *   [1] 2 sum V_GGA phi(t) dphi/dt   (phi mode with p, and V given)
+ *   [2] sum V_GGA rho                (V given)
+ *   [3] sum pot1 rho, [4,10) sum tf dFds2 / rho^(8/3) d_a rho d_b rho
+ *       (xx xy xz yy yz zz): the GGA stress terms of kedf/gga.py:1312-1326 */
+#define OFDFT_GGA_LKTVW 0
+#define OFDFT_GGA_LKT 1
+#define OFDFT_GGA_DK 2
+#define OFDFT_GGA_LLP 3
+#define OFDFT_GGA_OL 4
+#define OFDFT_GGA_THAK 5
+#define OFDFT_GGA_B86A 6
+#define OFDFT_GGA_B86B 7
+#define OFDFT_GGA_DK87 8
+#define OFDFT_GGA_PW86 9
+#define OFDFT_GGA_PW91O 10
+#define OFDFT_GGA_PW91 11
+#define OFDFT_GGA_LG94 12
+#define OFDFT_GGA_P92 13
+#define OFDFT_GGA_PBE 14
+#define OFDFT_GGA_P82 15
+#define OFDFT_GGA_VJKS00 16
+#define OFDFT_GGA_VT84F 17
+#define OFDFT_GGA_TFVW 18
+#define OFDFT_GGA_STV 19
+#define OFDFT_GGA_PBE2M 20
+#define OFDFT_GGA_PG 21
+#define OFDFT_GGA_COUNT 22
+#define OFDFT_GGA_NOUT 10
+int ofdft_gga(ofdft_plan plan, int functional, const double* params, int nparams, const void* rho, const void* phi,
+              const void* p, double c, double s, void* V_inout, double* out);
+
 /* ---- vector kernels used by the optimizer (optimization.py:92-280) ---- */
 /* out[0] = sum(a*b), out[1] = sum(a*a), out[2] = sum(b*b) in FP64 (host).  */
 int ofdft_dot3(ofdft_plan plan, const void* a, const void* b, double* out3);

@@ -68,7 +68,7 @@ __all__ = [
     "empty_core_table", "LocalPseudo", "Ewald", "TotalFunctional",
     "Optimization", "read_qepp", "read_poscar", "ecut2nr", "best_fft_size",
     "cbspline_Mn", "tf_stress", "vw_stress", "wt_stress", "lda_stress", "hartree_stress",
-    "lindhard_derivative", "GGA_KEDFS", "gga_functional", "gga_enhancement", "gga_params",
+    "lindhard_derivative", "GGA_KEDFS", "gga_functional", "gga_enhancement", "gga_params", "gga_stress",
 ]
 
 
@@ -661,6 +661,32 @@ def gga_enhancement(s, functional, params=None):
         F, d = fn(s / TKF0, p)
         return F, d / (TKF0 * TKF0)
     return fn(s, p)
+
+
+def gga_stress(rho, grid, functional="LKT", params=None):
+    """GGA 'S' branch: kedf/gga.py:1312-1326."""
+    rhom = np.where(rho < 1e-16, 1e-16, rho)
+    rho23 = np.cbrt(rhom * rhom)
+    tf = C_TF * rho23 * rhom
+    rho83 = (rho23 * rho23) ** 2
+    rhoG = fft(rho, grid)
+    grad = [ifft(1j * grid.g[i] * rhoG, grid) for i in range(3)]
+    s = np.sqrt(grad[0] ** 2 + grad[1] ** 2 + grad[2] ** 2) / (rho23 * rho23)
+    s = np.where(s < 1e-16, 1e-16, s)
+    F, d = gga_enhancement(s, functional, params)
+    E = (tf * F).sum() * grid.dV
+    pot1 = 5.0 / 3.0 * C_TF * rho23 * F - 4.0 / 3.0 * tf * d * s * s / rhom
+    vk = tf * d / rho83
+    p = E - (pot1 * rho).sum() * grid.dV
+    for i in range(3):
+        p -= (grad[i] * grad[i] * vk).sum() * grid.dV
+    out = np.zeros((3, 3))
+    for i in range(3):
+        out[i, i] = p
+        for j in range(i, 3):
+            out[i, j] -= (vk * grad[i] * grad[j]).sum() * grid.dV
+            out[j, i] = out[i, j]
+    return out / grid.volume
 
 
 def gga_functional(rho, grid, functional="LKT", params=None, calc=("E", "V")):
@@ -1363,8 +1389,9 @@ class TotalFunctional:
         self.ewald = ewald
         self.cache = KernelCache()
         names = {"TF": ("TF",), "VW": ("VW",), "TFVW": ("TF", "VW"), "X_TF_Y_VW": ("TF", "VW"),
-                 "XTFYVW": ("TF", "VW"), "WT": ("TF", "VW", "WT-NL")}
+                 "XTFYVW": ("TF", "VW"), "WT": ("TF", "VW", "WT-NL"), "GGA": ("GGA",)}
         self.knames = names[self.kedf]
+        # kedf="GGA": kedf_options functional (the reference's k_str) and params
 
     def compute(self, rho, calc=("E", "V"), split=False, phi=None, lphi=False):
         g = self.grid
@@ -1377,6 +1404,8 @@ class TotalFunctional:
         if "WT-NL" in self.knames:
             parts["KEDF-NL"] = wt_functional(rho, g, x=o["x"], y=o["y"], alpha=o["alpha"], beta=o["beta"],
                                              rho0=o["rho0"], calc=calc, cache=self.cache)
+        if "GGA" in self.knames:
+            parts["KEDF-GGA"] = gga_functional(rho, g, o.get("functional", "LKT"), o.get("params"), calc=calc)
         if self.xc:
             parts["XC"] = lda_functional(rho, g, calc=calc)
         if self.hartree:
@@ -1416,6 +1445,8 @@ class TotalFunctional:
             st["KEDF-VW"] = vw_stress(rho, g, y=o["y"])
         if "WT-NL" in self.knames:
             st["KEDF-NL"] = wt_stress(rho, g, parts["KEDF-NL"], y=o["y"], alpha=o["alpha"], beta=o["beta"])
+        if "GGA" in self.knames:
+            st["KEDF-GGA"] = gga_stress(rho, g, o.get("functional", "LKT"), o.get("params"))
         if self.xc:
             st["XC"] = lda_stress(rho, g)
         if self.hartree:

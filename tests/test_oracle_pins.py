@@ -341,7 +341,8 @@ def test_stress_is_strain_derivative_of_energy():
     """Every term's stress equals (1/Omega) dE/d(eps_ab) of the same oracle's
     energy under a homogeneous strain (central difference, h = 1e-5), for a
     diagonal and an off-diagonal component: TF, vW, WT, LDA, Hartree, local
-    PP (exact and PME) and Ewald (exact and PME, eta held fixed)."""
+    PP (exact and PME), Ewald (exact and PME, eta held fixed) and the GGA
+    kinetic term (LKT and APBE)."""
     from dftpy_amd import workloads as W
     wl = W.build("al4_32")
     n = wl["nr"][0]
@@ -362,7 +363,9 @@ def test_stress_is_strain_derivative_of_energy():
     def energies(eps):
         g, ions, r = _strained_al4(eps, rho, wl)
         out = {"TF": O.tf_functional(r, g)[0], "VW": O.vw_functional(r, g)[0], "NL": O.wt_functional(r, g)[0],
-               "XC": O.lda_functional(r, g)[0], "H": O.hartree_functional(r, g)[0]}
+               "XC": O.lda_functional(r, g)[0], "H": O.hartree_functional(r, g)[0],
+               "GGA": O.gga_functional(r, g, "LKT", calc=("E",))[0],
+               "GGA2": O.gga_functional(r, g, "APBE", calc=("E",))[0]}
         for pme in (True, False):
             out["PP%d" % pme] = O.LocalPseudo(g, ions, tabs, PME=pme).compute(r)[0]
             ew = O.Ewald(g, ions, PME=pme)
@@ -372,7 +375,8 @@ def test_stress_is_strain_derivative_of_energy():
 
     E0 = energies(np.zeros((3, 3)))
     st = {"TF": O.tf_stress(E0["TF"], g0), "VW": O.vw_stress(r0, g0), "NL": O.wt_stress(r0, g0, E0["NL"]),
-          "XC": O.lda_stress(r0, g0), "H": O.hartree_stress(r0, g0, E0["H"])}
+          "XC": O.lda_stress(r0, g0), "H": O.hartree_stress(r0, g0, E0["H"]),
+          "GGA": O.gga_stress(r0, g0, "LKT"), "GGA2": O.gga_stress(r0, g0, "APBE")}
     for pme in (True, False):
         st["PP%d" % pme] = O.LocalPseudo(g0, ions0, tabs, PME=pme).stress(r0)
         ew = O.Ewald(g0, ions0, PME=pme)
