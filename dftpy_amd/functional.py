@@ -560,7 +560,9 @@ class TotalFunctional(AbstractFunctional):
         return t
 
     def evaluator(self, grid, dtype=torch.float64):
-        key = (tuple(int(x) for x in grid.nrR), grid.lattice.tobytes(), bool(grid.full), dtype)
+        # a restarted pseudopotential (new ions) needs a new evaluator
+        ver = tuple(getattr(f, "_version", 0) for f in self.funcDict.values())
+        key = (tuple(int(x) for x in grid.nrR), grid.lattice.tobytes(), bool(grid.full), dtype, ver)
         ev = self._evaluators.get(key)
         if ev is None:
             ev = DeviceEvaluator(self, grid, dtype)

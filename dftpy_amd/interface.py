@@ -246,6 +246,7 @@ def OptimizeDensityConf(config, ions=None, rho=None, E_v_Evaluator=None, **kwarg
 def evaluator2results(evaluator, rho=None, calcType=("E", "F"), ions=None, print_stress=False, split=True, **kwargs):
     """interface.py:245-317: the energy table, force summary and stress
     printout, and the results dictionary."""
+    calcType = set(calcType)
     forces, stress = {}, {}
     ep = {}
     if "E" in calcType or "V" in calcType:

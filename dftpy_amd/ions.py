@@ -102,3 +102,11 @@ class Ions:
     @property
     def symbols_uniq(self):
         return np.sort(np.unique(self.symbols))
+
+    @staticmethod
+    def from_ase(atoms):
+        """Ions from an ase.Atoms-like object in Angstrom (ions.py:67-69):
+        anything with get_chemical_symbols(), get_positions() and
+        get_cell(); its initial charges are not carried over."""
+        return Ions(symbols=list(atoms.get_chemical_symbols()), positions=np.asarray(atoms.get_positions()),
+                    cell=np.asarray(atoms.get_cell()), units="ase")

@@ -282,9 +282,12 @@ class Optimization:
     def _work(self, prec):
         grid = self.rho.grid
         dt = torch.float64 if prec == "fp64" else torch.float32
-        key = (prec, id(grid))
+        # the evaluator changes with the grid and with restarted ions
+        ev = self.EnergyEvaluator.evaluator(grid, dt)
+        key = (prec, id(ev))
         if key not in self._works:
-            self._works[key] = _Work(self.EnergyEvaluator.evaluator(grid, dt))
+            self._works = {k: w for k, w in self._works.items() if k[0] != prec}
+            self._works[key] = _Work(ev)
         return self._works[key]
 
     def _residual(self, W, mu):

@@ -118,6 +118,33 @@ class LocalPseudo(AbstractFunctional):
         self._vreal32 = None
         self._ewald = None
         self._plan = None
+        self._version = 0
+
+    def restart(self, grid=None, ions=None, full=False, duplicate=False):
+        """Drop V_loc, the Ewald object and the plan so that they are built
+        again for new ions and/or a new grid (pseudo/__init__.py:152-176);
+        ``duplicate=True`` returns a new object instead."""
+        if duplicate:
+            return LocalPseudo(grid=grid if grid is not None else self.grid,
+                               ions=ions if ions is not None else self.ions,
+                               PP_list=self.PP_list, PME=self.PME, BsplineOrder=self.BsplineOrder)
+        if ions is not None:
+            for s in ions.symbols_uniq:
+                if s not in self.pps:
+                    raise ValueError("There is no pseudopotential for {:4s} atom".format(s))
+            self.ions = ions
+            self.ions.set_charges({k: p.zval for k, p in self.pps.items()})
+        if grid is not None:
+            if self._plan is not None and (tuple(grid.nrR) != tuple(self.grid.nrR)
+                                           or not np.array_equal(grid.lattice, self.grid.lattice)):
+                self._plan = None
+            self.grid = grid
+        self._vreal = None
+        self._vreal32 = None
+        self._ewald = None
+        self._tot = None
+        # device evaluators built over the old V_loc see a new version
+        self._version += 1
 
     @property
     def zval(self):
