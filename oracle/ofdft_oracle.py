@@ -69,6 +69,8 @@ __all__ = [
     "Optimization", "read_qepp", "read_poscar", "ecut2nr", "best_fft_size",
     "cbspline_Mn", "tf_stress", "vw_stress", "wt_stress", "lda_stress", "hartree_stress",
     "lindhard_derivative", "GGA_KEDFS", "gga_functional", "gga_enhancement", "gga_params", "gga_stress",
+    "LWT_DEFAULTS", "LwtCache", "lwt_functional", "lwt_kernel_tables", "lwt_kernel_kf", "lwt_kflist",
+    "guess_kf_bound", "mgp_kernel_table", "mgp_omega_e", "wt_kernel_table",
 ]
 
 
@@ -716,6 +718,312 @@ def gga_functional(rho, grid, functional="LKT", params=None, calc=("E", "V")):
 # stress of the density functionals (3x3, Ha/bohr^3)
 # ---------------------------------------------------------------------------
 _PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+# ---------------------------------------------------------------------------
+# LWT / LMGP: the nonlocal kinetic term with a density-dependent kernel,
+# K(|G| / kf(r)) interpolated between kernels at a list of kf values
+# ---------------------------------------------------------------------------
+LWT_DEFAULTS = dict(interp="linear", kerneltype="WT", symmetrization=None, lumpfactor=None,
+                    alpha=5.0 / 6.0, beta=5.0 / 6.0, etamax=50.0, maxpoints=1000, kdd=3, ratio=1.15,
+                    nsp=None, delta=None, neta=50000, order=3, kfmin=None, kfmax=None, ldw=None)
+
+
+def wt_kernel_table(eta, x=1.0):
+    """WTKernelTable: kedf/kernel.py:344-353 (vW fraction fixed at 1)."""
+    return lindhard(eta, x, 1.0) * (4.0 / 5.0 * 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0))
+
+
+def mgp_kernel_table(eta, maxpoints=1000, symmetrization=None):
+    """MGPKernelTable on a 1-D eta table: kedf/kernel.py:267-321. The
+    Lindhard function is splined on a log grid from eta[1]/10 to
+    eta[-1]+2 and the t-integral is the right-endpoint sum over
+    t = i/maxpoints."""
+    dt = 1.0 / maxpoints
+    ctf = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+    coe = 4.0 / 5.0 * ctf * 5.0 / 6.0 * dt
+    gp = np.logspace(np.log10(eta[1] / 10), np.log10(eta[-1] + 2), num=eta.size)
+    tck = splrep(gp, lindhard(gp, 1.0, 1.0))
+    kernel = np.zeros_like(eta)
+    for i in range(1, maxpoints + 1):
+        t = i * dt
+        w = t ** (-2.0 / 3.0) if symmetrization == "Geometric" else t ** (1.0 / 6.0)
+        kernel += splev(eta / np.cbrt(t), tck, ext=3) / w
+    if symmetrization == "Arithmetic":
+        return 0.5 * (kernel * coe + wt_kernel_table(eta))
+    if symmetrization == "Geometric":
+        return kernel * (2.0 * coe)
+    return kernel * coe
+
+
+def lwt_kernel_tables(kerneltype="WT", x=1.0, etamax=50.0, neta=50000, order=3, maxpoints=1000,
+                      symmetrization=None):
+    """The kernel table and its derivative table -eta dK/deta on
+    linspace(0, etamax, neta): lwt.py:533-563. order > 1 gives splrep
+    tuples of that degree, otherwise plain arrays."""
+    eta = np.linspace(0, etamax, neta)
+    if kerneltype == "WT":
+        tab = wt_kernel_table(eta, x)
+    elif kerneltype == "MGP":
+        tab = mgp_kernel_table(eta, maxpoints, symmetrization)
+    elif kerneltype == "MGPA":
+        tab = mgp_kernel_table(eta, maxpoints, "Arithmetic")
+    elif kerneltype == "MGPG":
+        tab = mgp_kernel_table(eta, maxpoints, "Geometric")
+    else:
+        raise AttributeError("kerneltype %s" % kerneltype)
+    if order > 1:
+        K = splrep(eta, tab, k=order)
+        D = splrep(eta, splev(eta, K, der=1) * (-1.0 * eta), k=order)
+        return K, D
+    tck = splrep(eta, tab, k=3)
+    return tab, splev(eta, tck, der=1) * (-1.0 * eta)
+
+
+def lwt_kernel_kf(q, kf, table, etamax):
+    """LWTKernelKf: kedf/kernel.py:379-405 (the table value at etamax beyond
+    it, nearest table entry for an array table, 0 at G = 0)."""
+    eta = q / kf
+    K = np.empty_like(q)
+    inside = eta < etamax
+    if isinstance(table, tuple):
+        limit = splev(etamax, table)
+        if inside.any():
+            K[inside] = splev(eta[inside], table)
+    else:
+        limit = table[-1]
+        deta = etamax / (table.size - 1)
+        K[inside] = table[np.around(eta[inside] / deta).astype(np.int32)]
+    K[~inside] = limit
+    if q[0, 0, 0] < ZERO:
+        K[0, 0, 0] = 0.0
+    return K
+
+
+def mgp_omega_e(q, Ne=1, lumpfactor=0.2):
+    """MGPOmegaE with revke=True: kedf/kernel.py:408-440."""
+    c, b = 1.0, None
+    if isinstance(lumpfactor, (list, tuple)):
+        a = lumpfactor[0]
+        if len(lumpfactor) > 1:
+            b = lumpfactor[1]
+        if len(lumpfactor) > 2:
+            c = lumpfactor[2]
+    else:
+        a = lumpfactor
+    if b is None:
+        a = float(a / Ne ** (2.0 / 3.0))
+        b = a
+    q = q.copy()
+    zero = q[0, 0, 0] < ZERO
+    if zero:
+        q[0, 0, 0] = 1.0
+    gg = q ** 2
+    corr = 4 * np.pi * sp.erf(c * gg) ** 2 * a * np.exp(-gg * b) / gg
+    if zero:
+        corr[0, 0, 0] = 0.0
+    return corr
+
+
+class LwtCache(dict):
+    """ke_kernel_saved for LWT: kedf/__init__.py:101-111."""
+
+    def __init__(self):
+        super().__init__(Kernel=None, rho0=0.0, shape=None, KernelTable=None, etamax=None, KernelDeriv=None,
+                         MGPKernelE=None, kfmin=None, kfmax=None)
+
+
+def guess_kf_bound(kf, kfmin=None, kfmax=None, kftol=1e-3, saved=None):
+    """lwt.py:36-80: the kf range of the kernel list; the saved range only
+    grows (kfmin down by the factors of the table, kfmax up in 0.5 steps)."""
+    prev_min = saved["kfmin"] if saved is not None else None
+    prev_max = saved["kfmax"] if saved is not None else None
+    if kfmin is not None and kfmax is not None:
+        return [kfmin, kfmax]
+    lo, hi = kf.min(), kf.max()
+    if prev_min is None:
+        prev_min = 10
+        if kfmin is None:
+            kfmin = lo
+    if prev_max is None:
+        prev_max = 1.0
+        if kfmax is None:
+            kfmax = hi
+    if kfmin is None or kfmin > prev_min:
+        kfmin = prev_min
+    if kfmax is None or kfmax < prev_max:
+        kfmax = prev_max
+    if kfmin > lo:
+        steps = [1e-5, 1e-4, 5e-3, 1e-3, 5e-3, 1e-2, 5e-2, 0.1, 0.5, 1.0]
+        r = lo / kfmin
+        for i in range(len(steps) - 1, 0, -1):
+            if r > steps[i]:
+                kfmin *= steps[i]
+                break
+    if kfmin < kftol:
+        kfmin = kftol
+    if kfmax < hi:
+        kfmax += (np.round((hi - kfmax) / 0.5) + 1) * 0.5
+    if saved is not None:
+        saved["kfmin"] = kfmin
+        saved["kfmax"] = kfmax
+    return [kfmin, kfmax]
+
+
+def lwt_kflist(kfMin, kfMax, ratio=1.15, nsp=None, delta=None):
+    """The kf nodes: lwt.py:148-163 (uniform with nsp or delta, else
+    geometric with ``ratio``), widened by 1e-16 at both ends."""
+    if nsp is not None:
+        if kfMax - kfMin < 1e-3:
+            kfl = np.array([kfMin, kfMax], dtype=np.float64)
+        else:
+            kfl = kfMin + (kfMax - kfMin) / (nsp - 1) * np.arange(nsp)
+    elif delta is not None:
+        n = int(np.ceil((kfMax - kfMin) / delta)) + 1
+        kfl = np.linspace(kfMin, kfMax, n)
+    else:
+        n = int(np.ceil(np.log(kfMax / kfMin) / np.log(ratio))) + 1
+        kfl = kfMin * ratio ** np.arange(n)
+    kfl = np.array(kfl, dtype=np.float64)
+    kfl[0] -= 1e-16
+    kfl[-1] += 1e-16
+    return kfl
+
+
+def lwt_functional(rho, grid, calc=("E", "V"), saved=None, x=1.0, y=1.0, **opts):
+    """LWT / LMGP nonlocal term: LWT lwt.py:506-566 (kernel tables once per
+    grid shape) and LWTPotentialEnergy lwt.py:83-341 (interpolation of
+    IFFT[K(|G|/kf_i) FFT(rho^beta)] between the kf nodes; the potential
+    with the kdd = 1, 2, 3 terms; the low-density weight)."""
+    o = dict(LWT_DEFAULTS)
+    o.update({k: v for k, v in opts.items() if v is not None})
+    if saved is None:
+        saved = LwtCache()
+    if tuple(grid.nr) != saved["shape"]:
+        K, D = lwt_kernel_tables(o["kerneltype"], x, o["etamax"], o["neta"], o["order"], o["maxpoints"],
+                                 o["symmetrization"])
+        saved["KernelTable"], saved["KernelDeriv"] = K, D
+        if o["lumpfactor"] is not None:
+            saved["MGPKernelE"] = mgp_omega_e(grid.q, rho.mean() * grid.volume, o["lumpfactor"])
+        saved["etamax"] = o["etamax"]
+        saved["shape"] = tuple(grid.nr)
+        saved["rho0"] = rho.mean()
+    alpha, beta, interp, etamax = o["alpha"], o["beta"], o["interp"], o["etamax"]
+    q = grid.q
+    kf = 2.0 * np.cbrt(3.0 * np.pi ** 2 * rho)
+    if abs(kf.max() - kf.min()) < 1e-8:
+        return 0.0, (np.zeros_like(rho) if "V" in calc else None)
+    kfmin, kfmax = guess_kf_bound(kf, o["kfmin"], o["kfmax"], saved=saved)
+    vcalc = "V" in calc
+    kdd = o["kdd"] if vcalc else 1
+    kf = np.where(kf > kfmax, kfmax, kf)
+    kfl = lwt_kflist(kfmin, kfmax, o["ratio"], o["nsp"], o["delta"])
+    nsp = kfl.size
+    KT, KD, ME = saved["KernelTable"], saved["KernelDeriv"], saved["MGPKernelE"]
+    mcalc = kdd > 1 or interp == "hermite"
+    rf = np.where(rho > 0, rho, 1e-30)
+    ra = rf ** alpha
+    ra1 = ra / rf
+    if abs(alpha - beta) < 1e-8:
+        rb, rb1 = ra, ra1
+    else:
+        rb = rf ** beta
+        rb1 = rb / rf
+    B = fft(rb, grid)
+    pot1 = np.zeros_like(rho)
+    pot3 = np.zeros_like(rho)
+    pot2G = None
+    ker = [None] * nsp
+    kder = [None] * nsp
+    p = [None] * nsp
+    m = [None] * nsp
+
+    def node(j):
+        if p[j] is None:
+            ker[j] = lwt_kernel_kf(q, kfl[j], KT, etamax)
+            p[j] = ifft(ker[j] * B, grid)
+            if mcalc:
+                kder[j] = lwt_kernel_kf(q, kfl[j], KD, etamax) / kfl[j]
+                m[j] = ifft(kder[j] * B, grid)
+
+    for i in range(nsp - 1):
+        node(i)
+        node(i + 1)
+        mask = (kf > kfl[i]) & (kf < kfl[i + 1] + 1e-18)
+        if i == 0:
+            # below the first node: pot1 = p_0 (kf - kf_0)/kf_0 (lwt.py:236-249;
+            # its pot3 line is overwritten by the interval below)
+            small = kf < kfl[0] + 1e-18
+            t = (kf - kfl[0]) / kfl[0]
+            pot1[small] = p[0][small] * t[small]
+            if kdd > 1:
+                pot2G = ker[0] * fft(ra * small * t, grid)
+        rU = ra * mask
+        Dkf = kfl[i + 1] - kfl[i]
+        t = (kf - kfl[i]) / Dkf
+        if interp in ("linear", "newton"):
+            t0 = 1 - t
+            pot1[mask] = p[i][mask] * t0[mask] + p[i + 1][mask] * t[mask]
+            if kdd > 1:
+                add = ker[i] * fft(rU * t0, grid) + ker[i + 1] * fft(rU * t, grid)
+                pot2G = add if pot2G is None else pot2G + add
+                if kdd == 3:
+                    pot3[mask] = m[i][mask] * t0[mask] + m[i + 1][mask] * t[mask]
+        elif interp == "hermite":
+            t2 = t * t
+            t3 = t2 * t
+            h00 = 2.0 * t3 - 3.0 * t2 + 1.0
+            h10 = t3 - 2.0 * t2 + t
+            h01 = 1.0 - h00
+            h11 = t3 - t2
+            pot1[mask] = h00[mask] * p[i][mask] + h01[mask] * p[i + 1][mask] \
+                + Dkf * (h10[mask] * m[i][mask] + h11[mask] * m[i + 1][mask])
+            if kdd > 1:
+                add = ker[i] * fft(rU * h00, grid) + ker[i + 1] * fft(rU * h01, grid) \
+                    + Dkf * (kder[i] * fft(rU * h10, grid) + kder[i + 1] * fft(rU * h11, grid))
+                pot2G = add if pot2G is None else pot2G + add
+                if kdd == 3:
+                    tm, t2m = t[mask], t2[mask]
+                    h00D = 6.0 * t2m - 6 * tm
+                    h10D = 3.0 * t2m - 4 * tm + 1.0
+                    h01D = -h00D
+                    h11D = 3.0 * t2m - 2 * tm
+                    pot3[mask] = (h00D * p[i][mask] + h01D * p[i + 1][mask]) / Dkf \
+                        + h10D * m[i][mask] + h11D * m[i + 1][mask]
+        else:
+            raise AttributeError("interp %s" % interp)
+        # free the node below: it is not used again
+        p[i] = m[i] = None
+    mgpe = None
+    if ME is not None:
+        mgpe = ifft(ME * B, grid)
+        pot1 = pot1 + mgpe
+    pot2 = None
+    if vcalc:
+        if pot2G is not None:
+            pot2 = ifft(pot2G, grid)
+            if mgpe is not None:
+                pot2 = pot2 + mgpe
+        else:
+            pot2 = pot1.copy()
+    ldw = o["ldw"] if o["ldw"] is not None else 1.0 / 6.0
+    factor = np.ones_like(rho)
+    rhov = rho.max()
+    low = rho < 1e-6
+    ld = max(0.1, ldw)
+    factor[low] = np.abs(rho[low]) ** ld / (rhov ** ld)
+    factor[rho < 0] = 0.0
+    pot1 = pot1 * factor
+    E = (ra * pot1).sum() * grid.dV
+    V = None
+    if vcalc:
+        pot2 = pot2 * factor
+        pot3 = pot3 * factor
+        pot1 = pot1 * (alpha * ra1)
+        pot2 = pot2 * (beta * rb1)
+        pot3 = pot3 * ((kf / 3.0) * ra1)
+        V = pot1 + (pot2 + pot3)
+    return E, V
 
 
 def tf_stress(E_tf, grid):
@@ -1389,8 +1697,14 @@ class TotalFunctional:
         self.ewald = ewald
         self.cache = KernelCache()
         names = {"TF": ("TF",), "VW": ("VW",), "TFVW": ("TF", "VW"), "X_TF_Y_VW": ("TF", "VW"),
-                 "XTFYVW": ("TF", "VW"), "WT": ("TF", "VW", "WT-NL"), "GGA": ("GGA",)}
+                 "XTFYVW": ("TF", "VW"), "WT": ("TF", "VW", "WT-NL"), "GGA": ("GGA",),
+                 "LWT": ("TF", "VW", "LWT-NL"), "LMGP": ("TF", "VW", "LWT-NL"),
+                 "LMGPA": ("TF", "VW", "LWT-NL"), "LMGPG": ("TF", "VW", "LWT-NL")}
         self.knames = names[self.kedf]
+        if self.kedf.startswith("LMGP"):
+            # LMGP, LMGPA, LMGPG: LWT with the MGP kernel tables (lwt.py:569-579)
+            self.opts["kerneltype"] = self.kedf[1:]
+        self.lwt_cache = LwtCache()
         # kedf="GGA": kedf_options functional (the reference's k_str) and params
 
     def compute(self, rho, calc=("E", "V"), split=False, phi=None, lphi=False):
@@ -1406,6 +1720,9 @@ class TotalFunctional:
                                              rho0=o["rho0"], calc=calc, cache=self.cache)
         if "GGA" in self.knames:
             parts["KEDF-GGA"] = gga_functional(rho, g, o.get("functional", "LKT"), o.get("params"), calc=calc)
+        if "LWT-NL" in self.knames:
+            lo = {k: v for k, v in o.items() if k in LWT_DEFAULTS}
+            parts["KEDF-NL"] = lwt_functional(rho, g, calc=calc, saved=self.lwt_cache, x=o["x"], y=o["y"], **lo)
         if self.xc:
             parts["XC"] = lda_functional(rho, g, calc=calc)
         if self.hartree:
