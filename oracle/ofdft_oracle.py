@@ -46,6 +46,17 @@ lindhard_derivative), lda,  under a homogeneous strain, central difference,
 hartree, LocalPseudo        every term; test_pme.py:35-36 and
 (exact, PME), Ewald (real,  test_ewald.py:51-53 exact-vs-PME agreement on
 rec, rec PME)               the Al_fde_rho.pp fixture
+lwt_functional (LWT, LMGP,  LMGP energy on Al_fde_rho.pp (test_functional.py:
+LMGPA, LMGPG), its kernel   33, 2.9145872498710492, matched to all digits);
+tables, guess_kf_bound,     Hermite kdd=3 potential = central difference of
+mgp_omega_e                 the energy (WT and MGP tables, with and without
+                            the MGP kinetic-electron term); linear potential
+                            within the interpolation error, shrinking with
+                            the spacing; energy converging to the brute-force
+                            pointwise-kf sum; zero for a uniform gas
+mgp_kernel_table            adaptive quadrature of its t-integral (None,
+                            Arithmetic, Geometric) to the O(1/maxpoints) sum
+                            error; the LMGP known answer above
 ==========================  ==================================================
 
 Unit constants: ase's CODATA-2014 values Bohr = 0.5291772105638411 Angstrom,
