@@ -50,6 +50,25 @@ class VLine(ctypes.Structure):
     ]
 
 
+class LwtTable(ctypes.Structure):
+    """ofdft_lwt_table"""
+    _fields_ = [
+        ("kind", ctypes.c_int), ("knots", ctypes.POINTER(ctypes.c_double)),
+        ("coefs", ctypes.POINTER(ctypes.c_double)), ("n", ctypes.c_int), ("x0", ctypes.c_double),
+        ("dx", ctypes.c_double), ("etamax", ctypes.c_double), ("limit", ctypes.c_double),
+    ]
+
+
+class LwtParams(ctypes.Structure):
+    """ofdft_lwt_params"""
+    _fields_ = [
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("kfmax", ctypes.c_double),
+        ("ld", ctypes.c_double), ("rhomax_ld", ctypes.c_double), ("kdd", ctypes.c_int),
+        ("hermite", ctypes.c_int), ("mgpe", ctypes.c_int), ("mgpe_a", ctypes.c_double),
+        ("mgpe_b", ctypes.c_double), ("mgpe_c", ctypes.c_double),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -90,6 +109,9 @@ _SIGS = {
     "ofdft_local_stress": (_i, [_vp, _vp, _dp]),
     "ofdft_ewald_stress": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
     "ofdft_gga": (_i, [_vp, _i, _dp, _i, _vp, _vp, _vp, _d, _d, _vp, _dp]),
+    "ofdft_lwt_set_kernel": (_i, [_vp, ctypes.POINTER(LwtTable), ctypes.POINTER(LwtTable)]),
+    "ofdft_lwt_range": (_i, [_vp, _vp, _vp, _vp, _d, _d, _dp]),
+    "ofdft_lwt": (_i, [_vp, ctypes.POINTER(LwtParams), _dp, _i, _vp, _vp, _vp, _d, _d, _vp, _dp]),
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
     "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),

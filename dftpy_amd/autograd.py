@@ -41,8 +41,8 @@ def _hessian_product(ev, phi, V, aux, u):
     """(dV/drho)[u] at rho = phi^2: p = u / (2 phi) turns the phi-space
     product of the kernels into the density-space one."""
     P = ev.P
-    if ev.terms.get("gga") is not None:
-        raise NotImplementedError("the Hessian product of the GGA kinetic terms is not on the device path")
+    if ev.terms.get("gga") is not None or ev.terms.get("lwt") is not None:
+        raise NotImplementedError("the Hessian product of the GGA / LWT kinetic terms is not on the device path")
     with torch._C.DisableTorchFunctionSubclass():
         u = u.to(phi.dtype).contiguous()
         p = torch.where(phi > 0, u / (2.0 * torch.where(phi > 0, phi, torch.ones_like(phi))), torch.zeros_like(u))

@@ -146,6 +146,11 @@ struct ofdft_plan_s {
   double* d_bm;      // PME b-moduli: complex n0 + n1 + nbm2
   double* d_work;    // N doubles: PME Q array / rhoPB
   void* d_gga;       // 4 N (plan precision): GGA pot1 and vector field
+  // LWT / LMGP: kernel tables (knots and coefficients on the device), the
+  // kf node list, and the workspace B | Acc (spectra) | pot1 | pot3 (fields)
+  ofdft_lwt_table lwt_tab[2];
+  double* d_lwttab;
+  void* d_lwt;
   // per-kernel-class event timing (ofdft_plan_set_profiling)
   int prof_on;
   std::vector<hipEvent_t> ev_pool;  // reused across profile windows
@@ -3173,6 +3178,477 @@ int gga_impl(ofdft_plan_s* P, const GgaSpec& g, const T* rho, const T* phi, cons
   return 0;
 }
 
+// ===========================================================================
+// LWT / LMGP: nonlocal kinetic term whose kernel K(|G| / kf(r)) follows the
+// local kf = 2 (3 pi^2 rho)^(1/3), interpolated between kernels at the kf
+// nodes (LWTPotentialEnergy, kedf/lwt.py:83-341).  The reference loops over
+// intervals with two nodes each; here every node j is one pass: its two
+// convolutions p_j = IFFT[K_j B] and m_j = IFFT[K'_j B] (B = FFT(rho^beta))
+// go straight into the interpolation sums pot1, pot3 of the points whose
+// interval has j as an end, and its share K_j FFT(rho^alpha w_j) of the pot2
+// spectrum is accumulated in reciprocal space, so that pot2 needs one
+// inverse transform in all.
+// ===========================================================================
+struct LwtTabDev {
+  const double* t;  // knots (kind 0)
+  const double* c;  // coefficients (kind 0) or values (kind 1)
+  int n, kind;
+  double x0, dx, etamax, limit;
+};
+
+// Cubic B-spline value (FITPACK splev; the last interval beyond the end):
+// the knot interval comes from the uniform data grid the spline was fitted
+// on, corrected by a step or two at the not-a-knot ends.
+__device__ double lwt_spline(const LwtTabDev& s, double x) {
+  const int k = 3;
+  const int hi = s.n - k - 2;
+  const double* t = s.t;
+  int l = (int)floor((x - s.x0) / s.dx) + 2;
+  l = l < k ? k : (l > hi ? hi : l);
+  while (l > k && t[l] > x) --l;
+  while (l < hi && t[l + 1] <= x) ++l;
+  double h[4] = {1.0, 0.0, 0.0, 0.0};
+  double hh[3];
+#pragma unroll
+  for (int j = 1; j <= k; ++j) {
+#pragma unroll
+    for (int i = 0; i < j; ++i) hh[i] = h[i];
+    h[0] = 0.0;
+#pragma unroll
+    for (int i = 1; i <= j; ++i) {
+      const int li = l + i, lj = li - j;
+      const double f = hh[i - 1] / (t[li] - t[lj]);
+      h[i - 1] += f * (t[li] - x);
+      h[i] = f * (x - t[lj]);
+    }
+  }
+  double v = 0.0;
+#pragma unroll
+  for (int j = 0; j <= k; ++j) v += s.c[l - k + j] * h[j];
+  return v;
+}
+
+// LWTKernelKf (kedf/kernel.py:379-405) at one eta: the table value below
+// etamax, `limit` from there on; the nearest entry for a plain table.
+__device__ __forceinline__ double lwt_kval(const LwtTabDev& s, double eta) {
+  if (!(eta < s.etamax)) return s.limit;
+  if (s.kind == 1) return s.c[(int)rint(eta / s.dx)];
+  return lwt_spline(s, eta);
+}
+
+// MGPOmegaE (kernel.py:408-440, revke): 4 pi erf(c G^2)^2 a exp(-b G^2) / G^2
+__device__ __forceinline__ double lwt_mgpe(double gg, double a, double b, double c) {
+  const double e = erf(c * gg);
+  return 4 * kPi * e * e * a * exp(-gg * b) / gg;
+}
+
+// local kf with the clamp at the upper bound (lwt.py:99, :138-139)
+__device__ __forceinline__ double lwt_kf(double rho, double kfmax) {
+  const double kf = 2.0 * cbrt(3.0 * (kPi * kPi) * rho);
+  return kf > kfmax ? kfmax : kf;
+}
+
+// node j and its neighbours in the kf list
+struct LwtNode {
+  double km, k0, kp;  // kf_{j-1}, kf_j, kf_{j+1}
+  int hasm, hasp, j;
+};
+
+// Weights of node j at one point: pot1 += ap p_j + am m_j, pot3 += bp p_j +
+// bm m_j, and the densities rho^alpha wk, rho^alpha wd whose transforms are
+// multiplied by K_j, K'_j for pot2 (lwt.py:213-315: the interval below the
+// node makes it the upper end "1", the interval above the lower end "0";
+// below the first node pot1 = p_0 (kf - kf_0)/kf_0, lwt.py:236-249).
+struct LwtW {
+  double ap, am, bp, bm, wk, wd;
+};
+__device__ __forceinline__ LwtW lwt_node_w(const LwtNode& nd, double kf, int herm) {
+  LwtW w = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (nd.j == 0 && kf < nd.k0 + 1e-18) {
+    const double t = (kf - nd.k0) / nd.k0;
+    w.ap = t;
+    w.wk = t;
+    return w;
+  }
+  if (nd.hasm && kf > nd.km && kf < nd.k0 + 1e-18) {
+    const double Dkf = nd.k0 - nd.km;
+    const double t = (kf - nd.km) / Dkf;
+    if (!herm) {
+      w.ap = t;
+      w.bm = t;
+      w.wk = t;
+    } else {
+      const double t2 = t * t, t3 = t2 * t;
+      const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0;
+      const double h01 = 1.0 - h00, h11 = t3 - t2;
+      const double h00D = 6.0 * t2 - 6 * t, h11D = 3.0 * t2 - 2 * t;
+      w.ap = h01;
+      w.am = Dkf * h11;
+      w.bp = -h00D / Dkf;
+      w.bm = h11D;
+      w.wk = h01;
+      w.wd = Dkf * h11;
+    }
+  } else if (nd.hasp && kf > nd.k0 && kf < nd.kp + 1e-18) {
+    const double Dkf = nd.kp - nd.k0;
+    const double t = (kf - nd.k0) / Dkf;
+    if (!herm) {
+      const double t0 = 1 - t;
+      w.ap = t0;
+      w.bm = t0;
+      w.wk = t0;
+    } else {
+      const double t2 = t * t, t3 = t2 * t;
+      const double h00 = 2.0 * t3 - 3.0 * t2 + 1.0, h10 = t3 - 2.0 * t2 + t;
+      const double h00D = 6.0 * t2 - 6 * t, h10D = 3.0 * t2 - 4 * t + 1.0;
+      w.ap = h00;
+      w.am = Dkf * h10;
+      w.bp = h00D / Dkf;
+      w.bm = h10D;
+      w.wk = h00;
+      w.wd = Dkf * h10;
+    }
+  }
+  return w;
+}
+
+// rho^e with rho floored at 1e-30 where it is not positive (lwt.py:188-191)
+template <typename T>
+struct PreLwtPow {
+  static constexpr int kProf = PC_ROWS_FWD;
+  PreRho<T> r;
+  double e;
+  __host__ __device__ int nin() const { return r.nin(); }
+  using In = typename PreRho<T>::In;
+  __device__ __forceinline__ In load(long long i) const { return r.load(i); }
+  __device__ __forceinline__ void apply(const In& x, T* out) const {
+    const double rho = r.dens(x);
+    out[0] = T(pow(rho > 0.0 ? rho : 1e-30, e));
+  }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+};
+
+// node j's weighted densities rho^alpha wk (and rho^alpha wd, Hermite)
+template <typename T>
+struct PreLwtW {
+  static constexpr int kProf = PC_ROWS_FWD;
+  PreRho<T> r;
+  LwtNode nd;
+  double kfmax, alpha;
+  int herm;
+  __host__ __device__ int nin() const { return r.nin(); }
+  using In = typename PreRho<T>::In;
+  __device__ __forceinline__ In load(long long i) const { return r.load(i); }
+  __device__ __forceinline__ void apply(const In& x, T* out) const {
+    const double rho = r.dens(x);
+    const LwtW w = lwt_node_w(nd, lwt_kf(rho, kfmax), herm);
+    const double ra = pow(rho > 0.0 ? rho : 1e-30, alpha);
+    out[0] = T(ra * w.wk);
+    if (herm) out[1] = T(ra * w.wd);
+  }
+  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
+};
+
+// node j's convolutions c[0] = p_j (c[1] = m_j) into pot1 (A1) and pot3
+// (A3); the first node writes every point, the others only their points.
+template <typename T>
+struct PostLwtNode {
+  static constexpr int NS = 1;
+  static constexpr int kProf = PC_ROWS_INV;
+  PreRho<T> r;
+  LwtNode nd;
+  double kfmax;
+  int herm, kdd3, first;
+  T* __restrict__ A1;
+  T* __restrict__ A3;
+  __host__ __device__ int nio() const { return r.nin() + (first ? 1 + kdd3 : 0); }
+  using In = typename PreRho<T>::In;
+  __device__ __forceinline__ In load(long long i) const { return r.load(i); }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double*) const {
+    const LwtW w = lwt_node_w(nd, lwt_kf(r.dens(in), kfmax), herm);
+    const double pv = c[0], mv = c[1];
+    if (first) {
+      A1[i] = T(w.ap * pv + w.am * mv);
+      if (kdd3) A3[i] = T(w.bp * pv + w.bm * mv);
+    } else if (w.ap != 0.0 || w.am != 0.0 || w.bp != 0.0 || w.bm != 0.0) {
+      A1[i] = T((double)A1[i] + (w.ap * pv + w.am * mv));
+      if (kdd3) A3[i] = T((double)A3[i] + (w.bp * pv + w.bm * mv));
+    }
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+};
+
+// The last pass (lwt.py:317-341): pot1 (+ mgpe), pot2 = c[0] (kdd > 1;
+// pot1 otherwise), pot3, the low-density weight, E = sum rho^alpha pot1 and
+// V += alpha rho^(alpha-1) pot1 + beta rho^(beta-1) pot2 + kf/3 rho^(alpha-1)
+// pot3.  acc: [E, 2 sum V_LWT phi(t) dphi/dt, sum V_LWT rho].
+template <typename T>
+struct PostLwtFinal {
+  static constexpr int NS = 3;
+  static constexpr int kProf = PC_ROWS_INV;
+  PreRho<T> r;
+  double kfmax, alpha, beta, ld, rhomax_ld;
+  int same, has2, mg;
+  const T* __restrict__ A1;
+  const T* __restrict__ A3;  // null: pot3 = 0
+  T* __restrict__ V;
+  __host__ __device__ int nio() const { return r.nin() + 1 + (A3 ? 1 : 0) + (V ? 2 : 0); }
+  struct In {
+    typename PreRho<T>::In x;
+    T a1, a3, v;
+  };
+  __device__ __forceinline__ In load(long long i) const {
+    return {r.load(i), A1[i], A3 ? A3[i] : T(0), V ? V[i] : T(0)};
+  }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const double rho = r.dens(in.x);
+    const double kf = lwt_kf(rho, kfmax);
+    const double rf = rho > 0.0 ? rho : 1e-30;
+    const double ra = pow(rf, alpha), ra1 = ra / rf;
+    const double rb1 = same ? ra1 : pow(rf, beta) / rf;
+    double pot1 = in.a1;
+    if (mg) pot1 = pot1 + (double)(has2 ? c[1] : c[0]);
+    double pot2 = has2 ? (double)c[0] : pot1;
+    double pot3 = in.a3;
+    double f = 1.0;
+    if (rho < 1e-6) f = pow(fabs(rho), ld) / rhomax_ld;
+    if (rho < 0.0) f = 0.0;
+    pot1 *= f;
+    acc[0] += ra * pot1;
+    if (V) {
+      pot2 *= f;
+      pot3 *= f;
+      const double v = pot1 * (alpha * ra1) + (pot2 * (beta * rb1) + pot3 * ((kf / 3.0) * ra1));
+      V[i] = T((double)in.v + v);
+      acc[2] += v * rho;
+      if (!r.rho && r.p) {
+        const double ph = (double)r.c * in.x.a + (double)r.s * in.x.b;
+        const double dph = (double)r.c * in.x.b - (double)r.s * in.x.a;
+        acc[1] += 2.0 * v * ph * dph;
+      }
+    }
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
+};
+
+template <typename T>
+__global__ void k_lwt_final_pw(PostLwtFinal<T> post, long long N, double* __restrict__ partials) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  const T z[3] = {T(0), T(0), T(0)};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
+    post.apply(i, post.load(i), z, acc);
+  block_sums<3>(acc, partials + 3 * (long long)blockIdx.x);
+}
+
+// per-block min and max of the density into partials[2b], partials[2b+1]
+template <typename T>
+__global__ void k_lwt_range(PreRho<T> r, long long N, double* __restrict__ partials) {
+  double lo = HUGE_VAL, hi = -HUGE_VAL;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
+    const double v = r.dens(r.load(i));
+    lo = fmin(lo, v);
+    hi = fmax(hi, v);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off, 64));
+    hi = fmax(hi, __shfl_xor(hi, off, 64));
+  }
+  __shared__ double red[16][2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[wave][0] = lo;
+    red[wave][1] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 1; w < nw; ++w) {
+      lo = fmin(lo, red[w][0]);
+      hi = fmax(hi, red[w][1]);
+    }
+    partials[2 * (long long)blockIdx.x] = lo;
+    partials[2 * (long long)blockIdx.x + 1] = hi;
+  }
+}
+
+// Node j in reciprocal space: Acc (+)= (K_j W_K + K'_j W_D) / N from the
+// transforms of the node's weighted densities in slots 0 (and 1), then
+// slot 0 = K_j B / N (and slot 1 = K'_j B / N) for the inverse transforms.
+// Kernels are 0 at G = 0; with full=1 frequencies the Hermitian part
+// (K(G) + K(-G))/2 gives Re(ifftn) of the full array.
+template <typename T>
+__global__ void k_lwt_node(cplx<T>* __restrict__ spec, long long Nhp, const cplx<T>* __restrict__ B,
+                           cplx<T>* __restrict__ acc, GridGeom geo, LwtTabDev K, LwtTabDev D, double kf, int nw,
+                           int mcalc, int first, double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  const bool need_d = mcalc || nw == 2;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const long long ph = r * geo.nhp + c;
+    double kv = 0.0, dv = 0.0;
+    if (idx != 0) {
+      const double eta = sqrt(gsq(geo, k0, k1, c)) / kf;
+      kv = lwt_kval(K, eta);
+      if (need_d) dv = lwt_kval(D, eta) / kf;
+      if (geo.full) {
+        const double etp = sqrt(gsq(geo, k0, k1, c, 1)) / kf;
+        kv = 0.5 * (kv + lwt_kval(K, etp));
+        if (need_d) dv = 0.5 * (dv + lwt_kval(D, etp) / kf);
+      }
+    }
+    if (nw) {
+      double ax = 0.0, ay = 0.0;
+      if (!first) {
+        const cplx<T> a = acc[ph];
+        ax = a.x;
+        ay = a.y;
+      }
+      const cplx<T> w0 = spec[ph];
+      ax += kv * (double)w0.x * scale;
+      ay += kv * (double)w0.y * scale;
+      if (nw == 2) {
+        const cplx<T> w1 = spec[Nhp + ph];
+        ax += dv * (double)w1.x * scale;
+        ay += dv * (double)w1.y * scale;
+      }
+      acc[ph] = cplx<T>{T(ax), T(ay)};
+    }
+    const cplx<T> b = B[ph];
+    spec[ph] = cplx<T>{T(kv * (double)b.x * scale), T(kv * (double)b.y * scale)};
+    if (mcalc) spec[Nhp + ph] = cplx<T>{T(dv * (double)b.x * scale), T(dv * (double)b.y * scale)};
+  }
+}
+
+// The last spectra: slot 0 = Acc (+ M B / N) for pot2 (has2), then the MGP
+// kinetic-electron convolution M B / N (mg) in the next slot.
+template <typename T>
+__global__ void k_lwt_final_spec(cplx<T>* __restrict__ spec, long long Nhp, const cplx<T>* __restrict__ B,
+                                 const cplx<T>* __restrict__ acc, GridGeom geo, int has2, int mg, double ma,
+                                 double mb, double mc, double scale) {
+  const long long Nh = (long long)geo.n0 * geo.n1 * geo.nh;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < Nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % geo.nh);
+    const long long r = idx / geo.nh;
+    const int k1 = (int)(r % geo.n1), k0 = (int)(r / geo.n1);
+    const long long ph = r * geo.nhp + c;
+    double mx = 0.0, my = 0.0;
+    if (mg) {
+      double m = 0.0;
+      if (idx != 0) {
+        m = lwt_mgpe(gsq(geo, k0, k1, c), ma, mb, mc);
+        if (geo.full) m = 0.5 * (m + lwt_mgpe(gsq(geo, k0, k1, c, 1), ma, mb, mc));
+      }
+      const cplx<T> b = B[ph];
+      mx = m * (double)b.x * scale;
+      my = m * (double)b.y * scale;
+    }
+    if (has2) {
+      const cplx<T> a = acc[ph];
+      spec[ph] = cplx<T>{T((double)a.x + mx), T((double)a.y + my)};
+    }
+    if (mg) spec[(has2 ? Nhp : 0) + ph] = cplx<T>{T(mx), T(my)};
+  }
+}
+
+LwtTabDev lwt_tab_dev(const ofdft_plan_s* P, int i) {
+  const ofdft_lwt_table& t = P->lwt_tab[i];
+  return LwtTabDev{t.knots, t.coefs, t.n, t.kind, t.x0, t.dx, t.etamax, t.limit};
+}
+
+template <typename T>
+int lwt_range_impl(ofdft_plan_s* P, const T* rho, const T* phi, const T* p, double c, double s, double* out) {
+  const int nb = vec_blocks(P->N);
+  PreRho<T> pr{rho, phi, p, T(c), T(s)};
+  hipLaunchKernelGGL((k_lwt_range<T>), dim3(nb), dim3(256), 0, P->stream, pr, P->N, P->partials);
+  LAUNCHCHK();
+  std::vector<double> h(2 * (size_t)nb);
+  HIPCHK(hipMemcpyAsync(h.data(), P->partials, sizeof(double) * h.size(), hipMemcpyDeviceToHost, P->stream));
+  HIPCHK(hipStreamSynchronize(P->stream));
+  double lo = h[0], hi = h[1];
+  for (int b = 1; b < nb; ++b) {
+    lo = std::min(lo, h[2 * b]);
+    hi = std::max(hi, h[2 * b + 1]);
+  }
+  out[0] = lo;
+  out[1] = hi;
+  return 0;
+}
+
+template <typename T>
+int lwt_impl(ofdft_plan_s* P, const ofdft_lwt_params& L, const double* kfl, int nsp, const T* rho, const T* phi,
+             const T* p, double c, double s, T* V, double* out) {
+  const long long N = P->N, Nhp = P->Nhp;
+  if (!P->d_lwt) HIPCHK(hipMalloc(&P->d_lwt, 2 * (size_t)Nhp * sizeof(cplx<T>) + 2 * (size_t)N * sizeof(T)));
+  auto* B = reinterpret_cast<cplx<T>*>(P->d_lwt);
+  cplx<T>* Acc = B + Nhp;
+  T* A1 = reinterpret_cast<T*>(Acc + Nhp);
+  T* A3 = A1 + N;
+  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
+  PreRho<T> pr{rho, phi, p, T(c), T(s)};
+  const int kdd = V ? L.kdd : 1;  // energy only: kdd = 1 (lwt.py:128-132)
+  const int herm = L.hermite ? 1 : 0;
+  const int mcalc = (kdd > 1 || herm) ? 1 : 0;
+  const int nw = (V && kdd > 1) ? (herm ? 2 : 1) : 0;
+  const int same = std::fabs(L.alpha - L.beta) < 1e-8 ? 1 : 0;
+  const double beta = same ? L.alpha : L.beta;
+  const double scale = 1.0 / (double)N;
+  const LwtTabDev tK = lwt_tab_dev(P, 0), tD = lwt_tab_dev(P, 1);
+  int rc;
+  // B = FFT(rho^beta)
+  if ((rc = launch_rows_fwd<T>(P, PreLwtPow<T>{pr, beta}, 1))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 1, 1, false))) return rc;
+  if ((rc = launch_axis<T>(P, spec, 0, 1, false))) return rc;
+  HIPCHK(hipMemcpyAsync(B, spec, sizeof(cplx<T>) * Nhp, hipMemcpyDeviceToDevice, P->stream));
+  for (int j = 0; j < nsp; ++j) {
+    LwtNode nd{j > 0 ? kfl[j - 1] : 0.0, kfl[j], j + 1 < nsp ? kfl[j + 1] : 0.0, j > 0 ? 1 : 0, j + 1 < nsp ? 1 : 0,
+               j};
+    if (nw) {
+      if ((rc = launch_rows_fwd<T>(P, PreLwtW<T>{pr, nd, L.kfmax, L.alpha, herm}, nw))) return rc;
+      if ((rc = launch_axis<T>(P, spec, 1, nw, false))) return rc;
+      if ((rc = launch_axis<T>(P, spec, 0, nw, false))) return rc;
+    }
+    hipLaunchKernelGGL((k_lwt_node<T>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, Nhp,
+                       (const cplx<T>*)B, Acc, geom(P), tK, tD, kfl[j], nw, mcalc, j == 0 ? 1 : 0, scale);
+    LAUNCHCHK();
+    const int F = 1 + mcalc;
+    if ((rc = launch_axis<T>(P, spec, 0, F, true))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+    PostLwtNode<T> pn{pr, nd, L.kfmax, herm, kdd == 3 ? 1 : 0, j == 0 ? 1 : 0, A1, A3};
+    if ((rc = launch_rows_inv<T>(P, pn, F, spec, false))) return rc;
+  }
+  const int has2 = nw ? 1 : 0, mg = L.mgpe ? 1 : 0;
+  PostLwtFinal<T> pf{pr, L.kfmax, L.alpha, beta, L.ld, L.rhomax_ld, same, has2, mg, A1, kdd == 3 ? A3 : nullptr, V};
+  double sm[3];
+  if (has2 + mg) {
+    hipLaunchKernelGGL((k_lwt_final_spec<T>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream, spec, Nhp,
+                       (const cplx<T>*)B, (const cplx<T>*)Acc, geom(P), has2, mg, L.mgpe_a, L.mgpe_b, L.mgpe_c,
+                       scale);
+    LAUNCHCHK();
+    const int F = has2 + mg;
+    if ((rc = launch_axis<T>(P, spec, 0, F, true))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+    if ((rc = launch_rows_inv<T>(P, pf, F, spec, true))) return rc;
+    if ((rc = fetch_sums(P, 3, sm))) return rc;
+  } else {
+    const int nb = vec_blocks(N);
+    hipLaunchKernelGGL((k_lwt_final_pw<T>), dim3(nb), dim3(256), 0, P->stream, pf, N, P->partials);
+    LAUNCHCHK();
+    if ((rc = reduce_blocks(P, nb, 3, sm))) return rc;
+  }
+  const double dV = P->dV;
+  out[0] = sm[0] * dV;
+  out[1] = sm[1] * dV;
+  out[2] = sm[2] * dV;
+  return 0;
+}
+
 // six stress sums of k_stress_recip over the half spectrum into out[0..6)
 int stress_recip(ofdft_plan_s* P, const StressArgs& A, int mode, double* out) {
   const int nb = vec_blocks(P->Nh);
@@ -3527,6 +4003,8 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->d_bm) (void)hipFree(P->d_bm);
   if (P->d_work) (void)hipFree(P->d_work);
   if (P->d_gga) (void)hipFree(P->d_gga);
+  if (P->d_lwttab) (void)hipFree(P->d_lwttab);
+  if (P->d_lwt) (void)hipFree(P->d_lwt);
   if (P->red) (void)hipFree(P->red);
   if (P->hred) (void)hipHostFree(P->hred);
   if (P->tnst) (void)hipFree(P->tnst);
@@ -4073,6 +4551,74 @@ int ofdft_gga(ofdft_plan P, int functional, const double* params, int nparams, c
   return gga_impl<float>(P, g, (const float*)rho, (const float*)phi, (const float*)p, c, s, (float*)V_inout, out);
 }
 
+
+int ofdft_lwt_set_kernel(ofdft_plan P, const ofdft_lwt_table* K, const ofdft_lwt_table* D) {
+  if (!valid(P) || !K || !D) return fail(OFDFT_EINVAL, "null argument");
+  const ofdft_lwt_table* in[2] = {K, D};
+  size_t tot = 0;
+  for (int i = 0; i < 2; ++i) {
+    const ofdft_lwt_table& t = *in[i];
+    if (t.kind == 0) {
+      if (t.n < 8 || !t.knots || !t.coefs || !(t.dx > 0.0)) return fail(OFDFT_EINVAL, "bad LWT spline table");
+      tot += 2 * (size_t)t.n;
+    } else if (t.kind == 1) {
+      if (t.n < 2 || !t.coefs || !(t.dx > 0.0)) return fail(OFDFT_EINVAL, "bad LWT table");
+      // eta < etamax reads entry rint(eta / dx): it must stay below n
+      if (t.etamax / t.dx > (double)t.n - 0.5) return fail(OFDFT_EINVAL, "LWT table shorter than etamax / dx");
+      tot += (size_t)t.n;
+    } else {
+      return fail(OFDFT_EINVAL, "unknown LWT table kind");
+    }
+    if (!(t.etamax > 0.0)) return fail(OFDFT_EINVAL, "bad LWT etamax");
+  }
+  if (P->d_lwttab) {
+    HIPCHK(hipStreamSynchronize(P->stream));
+    (void)hipFree(P->d_lwttab);
+    P->d_lwttab = nullptr;
+  }
+  HIPCHK(hipMalloc(&P->d_lwttab, tot * sizeof(double)));
+  double* cur = P->d_lwttab;
+  for (int i = 0; i < 2; ++i) {
+    ofdft_lwt_table t = *in[i];
+    if (t.kind == 0) {
+      HIPCHK(hipMemcpy(cur, t.knots, sizeof(double) * t.n, hipMemcpyHostToDevice));
+      t.knots = cur;
+      cur += t.n;
+    } else {
+      t.knots = nullptr;
+    }
+    HIPCHK(hipMemcpy(cur, t.coefs, sizeof(double) * t.n, hipMemcpyHostToDevice));
+    t.coefs = cur;
+    cur += t.n;
+    P->lwt_tab[i] = t;
+  }
+  return 0;
+}
+
+int ofdft_lwt_range(ofdft_plan P, const void* rho, const void* phi, const void* p, double c, double s,
+                    double* out2) {
+  if (!valid(P) || !out2 || (!rho && !phi)) return fail(OFDFT_EINVAL, "null argument");
+  if (P->prec == OFDFT_F64)
+    return lwt_range_impl<double>(P, (const double*)rho, (const double*)phi, (const double*)p, c, s, out2);
+  return lwt_range_impl<float>(P, (const float*)rho, (const float*)phi, (const float*)p, c, s, out2);
+}
+
+int ofdft_lwt(ofdft_plan P, const ofdft_lwt_params* prm, const double* kflist, int nsp, const void* rho,
+              const void* phi, const void* p, double c, double s, void* V_inout, double* out3) {
+  if (!valid(P) || !prm || !kflist || !out3 || (!rho && !phi)) return fail(OFDFT_EINVAL, "null argument");
+  if (!P->d_lwttab) return fail(OFDFT_ESTATE, "LWT kernel tables not set");
+  if (nsp < 2) return fail(OFDFT_EINVAL, "LWT needs at least two kf nodes");
+  for (int j = 0; j < nsp; ++j)
+    if (!(kflist[j] > 0.0) || (j > 0 && !(kflist[j] > kflist[j - 1])))
+      return fail(OFDFT_EINVAL, "LWT kf nodes must be positive and increasing");
+  if (prm->kdd < 1 || prm->kdd > 3) return fail(OFDFT_EINVAL, "LWT kdd must be 1, 2 or 3");
+  if (!(prm->rhomax_ld > 0.0)) return fail(OFDFT_EINVAL, "LWT rhomax_ld must be positive");
+  if (P->prec == OFDFT_F64)
+    return lwt_impl<double>(P, *prm, kflist, nsp, (const double*)rho, (const double*)phi, (const double*)p, c, s,
+                            (double*)V_inout, out3);
+  return lwt_impl<float>(P, *prm, kflist, nsp, (const float*)rho, (const float*)phi, (const float*)p, c, s,
+                         (float*)V_inout, out3);
+}
 
 int ofdft_dot3(ofdft_plan P, const void* a, const void* b, double* out3) {
   if (!valid(P) || !a || !b || !out3) return fail(OFDFT_EINVAL, "null argument");

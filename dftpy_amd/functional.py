@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from . import lwt as lwt_mod
 from .field import DirectField
 from .plancache import plan_for
 
@@ -147,6 +148,8 @@ class ZeroFunctional(AbstractFunctional):
 _KEDF_NAMES = {
     "TF": ("TF",), "VW": ("VW",), "TFVW": ("TF", "VW"), "X_TF_Y_VW": ("TF", "VW"), "XTFYVW": ("TF", "VW"),
     "WT": ("TF", "VW", "WT-NL"), "WT-NL": ("WT-NL",), "NONE": (),
+    "LWT": ("TF", "VW", "LWT-NL"), "LMGP": ("TF", "VW", "LWT-NL"), "LMGPA": ("TF", "VW", "LWT-NL"),
+    "LMGPG": ("TF", "VW", "LWT-NL"),
 }
 
 
@@ -233,11 +236,20 @@ class KEDF(AbstractFunctional):
         elif key not in _KEDF_NAMES:
             raise AttributeError("%s KEDF to be implemented" % self.name)
         self.options = dict(x=1.0, y=1.0, alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0=None)
-        self.options.update({k: v for k, v in kwargs.items() if k in ("x", "y", "alpha", "beta", "rho0", "sigma")})
+        self.options.update({k: v for k, v in kwargs.items() if k in ("x", "y", "alpha", "beta", "rho0", "sigma")
+                             and v is not None})
         if self.options.get("sigma") is not None:
             raise AttributeError("vW smoothing (sigma) is not on the device path")
         self.options["kedf"] = self.name
         self.engines = ("GGA",) if self.gga else _KEDF_NAMES[key]
+        self.lwt = None
+        if "LWT-NL" in self.engines:
+            # LWT, and LMGP / LMGPA / LMGPG = LWT with the MGP kernel tables (lwt.py:569-579)
+            lo = {k: v for k, v in kwargs.items() if k in lwt_mod.DEFAULTS}
+            lo.update(alpha=self.options["alpha"], beta=self.options["beta"])
+            if key != "LWT":
+                lo["kerneltype"] = key[1:]
+            self.lwt = lwt_mod.LwtTerm(x=self.options["x"], **lo)
         self.ke_kernel_saved = _KernelCache()
         self.energies = {}
 
@@ -245,7 +257,7 @@ class KEDF(AbstractFunctional):
         o = self.options
         return dict(tf="TF" in self.engines, vw="VW" in self.engines, wt="WT-NL" in self.engines,
                     x=o["x"], y=o["y"], alpha=o["alpha"], beta=o["beta"], rho0=o["rho0"], owner=self,
-                    gga=self.gga)
+                    gga=self.gga, lwt=self.lwt)
 
     def _total(self):
         if getattr(self, "_tot", None) is None:
@@ -372,6 +384,9 @@ def _density_stress(total, rho, energy_xc=None):
     grid = rho.grid
     ev = total.evaluator(grid, torch.float64)
     t = ev.terms
+    if t["lwt"] is not None:
+        # kedf/__init__.py:214 (name2functions over the stress engines, utils.py:289-302)
+        raise AttributeError("'LWT' has not implemented.")
     with torch._C.DisableTorchFunctionSubclass():
         r = rho.to(torch.float64).contiguous()
         sums = ev.eval_rho(r, None)
@@ -452,16 +467,22 @@ class DeviceEvaluator:
         out = {k: float(sums[i]) for k, i, flag in _SPLIT_KEYS if self.terms[flag]}
         if self.terms["gga"] is not None:
             out["KEDF-GGA"] = float(sums[L.NSUM])
+        if self.terms["lwt"] is not None:
+            out["KEDF-NL"] = float(sums[L.NSUM])
         return out
 
     def _with_gga(self, sums, **kw):
-        """Append the GGA term (ofdft_gga) to the fused evaluation's sums:
-        its energy in slot NSUM, its V rho and line-search derivative added
-        to S_VRHO and S_DTHETA; V (when given) already holds the sum."""
-        if self.terms["gga"] is None:
+        """Append the GGA (ofdft_gga) or LWT (ofdft_lwt) kinetic term to the
+        fused evaluation's sums: its energy in slot NSUM, its V rho and
+        line-search derivative added to S_VRHO and S_DTHETA; V (when given)
+        already holds the sum."""
+        if self.terms["gga"] is not None:
+            gid, prm = self.terms["gga"]
+            g = self.P.gga(gid, prm, **kw)
+        elif self.terms["lwt"] is not None:
+            g = self.terms["lwt"].evaluate(self.P, self.grid, **kw)
+        else:
             return sums
-        gid, prm = self.terms["gga"]
-        g = self.P.gga(gid, prm, **kw)
         out = np.concatenate([sums, [g[0]]])
         out[L.S_VRHO] += g[2]
         out[L.S_DTHETA] += g[1]
@@ -535,7 +556,7 @@ class TotalFunctional(AbstractFunctional):
 
     def _merged_terms(self):
         t = dict(tf=False, vw=False, wt=False, lda=False, hartree=False, pseudo=False, x=1.0, y=1.0,
-                 alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0_fixed=None, pseudo_obj=None, gga=None)
+                 alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0_fixed=None, pseudo_obj=None, gga=None, lwt=None)
         seen = set()
         for key, f in self.funcDict.items():
             d = f._device_terms()
@@ -547,10 +568,11 @@ class TotalFunctional(AbstractFunctional):
                         raise ValueError("term %s appears twice in the TotalFunctional" % flag)
                     seen.add(flag)
                     t[flag] = True
-            if d.get("gga") is not None:
-                if t["gga"] is not None:
-                    raise ValueError("two GGA kinetic terms in the TotalFunctional")
-                t["gga"] = d["gga"]
+            for side in ("gga", "lwt"):
+                if d.get(side) is not None:
+                    if t["gga"] is not None or t["lwt"] is not None:
+                        raise ValueError("two GGA / LWT kinetic terms in the TotalFunctional")
+                    t[side] = d[side]
             if f.type == "KEDF":
                 t.update(x=d["x"], y=d["y"], alpha=d["alpha"], beta=d["beta"], rho0_fixed=d["rho0"])
                 if d["wt"] and abs(d["alpha"] - d["beta"]) > 1e-9:

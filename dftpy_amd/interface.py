@@ -6,8 +6,8 @@ OptimizeDensityConf and evaluator2results (interface.py:21-317).
 The option table below carries the sections, keys, value types and
 defaults of the reference's configuration schema (configentries.json) for
 the options this path runs: the density optimization of one cell with the
-local pseudopotential, LDA, Hartree and the TF / vW / TFvW / WT / GGA KEDFs,
-energies, forces and stress.  Keys of the schema that select something the
+local pseudopotential, LDA, Hartree and the TF / vW / TFvW / WT / GGA / LWT /
+LMGP KEDFs, energies, forces and stress.  Keys of the schema that select something the
 GPU path does not run (spin, atomic initial densities, multistep grids,
 LibXC strings, other KEDF options) are accepted at their defaults and
 rejected with AttributeError otherwise.
@@ -79,7 +79,11 @@ SCHEMA = {
             "c_str": ("str", None)},
     "KEDF": {"kedf": ("ustr", "WT"), "x": ("float", 1.0), "y": ("float", 1.0), "alpha": ("float", None),
              "beta": ("float", None), "sigma": ("float", None), "rho0": ("float", None), "k_str": ("str", None),
-             "params": ("floatlist", None)},
+             "params": ("floatlist", None), "nsp": ("int", None), "interp": ("str", "hermite"),
+             "kerneltype": ("str", None), "symmetrization": ("str", None), "lumpfactor": ("floatlist", None),
+             "neta": ("int", 50000), "etamax": ("float", 50.0), "order": ("int", 3), "ratio": ("float", 1.15),
+             "maxpoints": ("int", 1000), "delta": ("float", None), "kdd": ("int", 3), "kfmin": ("float", None),
+             "kfmax": ("float", None), "ldw": ("float", None)},
     "OPT": {"method": ("ustr", "CG-HS"), "algorithm": ("ustr", "EMM"), "vector": ("str", "Orthogonalization"),
             "c1": ("float", 1e-4), "c2": ("float", 0.2), "maxls": ("int", 10), "econv": ("float", 1e-6),
             "maxfun": ("int", 50), "maxiter": ("int", 100), "xtol": ("float", 1e-12), "h0": ("float", 1.0)},

@@ -80,9 +80,10 @@ class Optimization:
                 self.optimization_options["vector"] != "Orthogonalization":
             raise AttributeError("only algorithm='EMM' with vector='Orthogonalization' is on the device path")
         self.EnergyEvaluator = EnergyEvaluator
-        if EnergyEvaluator._merged_terms()["gga"] is not None and self.optimization_options["hvp"] == "exact":
-            # no closed-form second derivative of the GGA kinetic terms on the
-            # device: the reference's forward-difference product
+        terms = EnergyEvaluator._merged_terms()
+        if (terms["gga"] is not None or terms["lwt"] is not None) and self.optimization_options["hvp"] == "exact":
+            # no closed-form second derivative of the GGA / LWT kinetic terms
+            # on the device: the reference's forward-difference product
             self.optimization_options["hvp"] = "fd"
         self.optimization_method = optimization_method
         if optimization_method not in ("TN", "LBFGS", "GD") and not optimization_method.startswith("CG"):

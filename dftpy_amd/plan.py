@@ -163,6 +163,31 @@ class Plan:
                                    out))
         return L.as_numpy(out)
 
+    def lwt_set_kernel(self, kernel, deriv):
+        """ofdft_lwt_set_kernel with two L.LwtTable (host arrays kept alive
+        by the caller for the duration of the call)."""
+        self._stream()
+        L.check(self.lib.ofdft_lwt_set_kernel(self._h, ctypes.byref(kernel), ctypes.byref(deriv)))
+
+    def lwt_range(self, rho=None, phi=None, p=None, c=1.0, s=0.0):
+        """(min, max) of rho or (c phi + s p)^2."""
+        out = L.darr(2)
+        self._stream()
+        L.check(self.lib.ofdft_lwt_range(self._h, self._real(rho, "rho"), self._real(phi, "phi"),
+                                         self._real(p, "p"), float(c), float(s), out))
+        return L.as_numpy(out)
+
+    def lwt(self, params, kflist, rho=None, phi=None, p=None, c=1.0, s=0.0, V=None):
+        """ofdft_lwt: the LWT term with the node list kflist; V += V_LWT when
+        V is given.  Returns [E, 2 sum V phi dphi, sum V rho] (x dV)."""
+        out = L.darr(3)
+        kfl = np.ascontiguousarray(kflist, dtype=np.float64)
+        self._stream()
+        L.check(self.lib.ofdft_lwt(self._h, ctypes.byref(params), kfl.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   int(kfl.size), self._real(rho, "rho"), self._real(phi, "phi"), self._real(p, "p"),
+                                   float(c), float(s), self._real(V, "V"), out))
+        return L.as_numpy(out)
+
     def hvp(self, phi, p, V, aux, mu, Ap):
         out = ctypes.c_double()
         self._stream()

@@ -311,6 +311,48 @@ int ofdft_ewald_stress(ofdft_plan plan, int nat, const double* positions, const 
 int ofdft_gga(ofdft_plan plan, int functional, const double* params, int nparams, const void* rho, const void* phi,
               const void* p, double c, double s, void* V_inout, double* out);
 
+/* LWT / LMGP nonlocal kinetic term (LWTPotentialEnergy, kedf/lwt.py:83-341;
+ * LWT / LMGP :506-579).  The kernel of a node kf_j is K(|G| / kf_j) from a
+ * table in eta, its derivative table -eta dK/deta divided by kf_j, both
+ * 0 at G = 0 and constant (`limit`) from etamax on (LWTKernelKf,
+ * kedf/kernel.py:379-405).  A table is a cubic B-spline (kind 0: knots and
+ * coefficients as scipy's splrep returns them, fitted on the uniform grid
+ * x0 + i dx) or a plain array read at the nearest entry (kind 1: `coefs`
+ * holds the n values, dx the spacing).  Set once per plan (the
+ * reference's ke_kernel_saved tables):                                     */
+typedef struct ofdft_lwt_table {
+  int kind;             /* 0 cubic spline, 1 nearest table entry            */
+  const double* knots;  /* host, n knots (kind 0)                           */
+  const double* coefs;  /* host, n coefficients (kind 0) or n values (1)    */
+  int n;
+  double x0, dx;        /* uniform data grid (kind 0) / table spacing (1)  */
+  double etamax, limit; /* the kernel is `limit` for eta >= etamax           */
+} ofdft_lwt_table;
+int ofdft_lwt_set_kernel(ofdft_plan plan, const ofdft_lwt_table* kernel, const ofdft_lwt_table* deriv);
+/* out[0] = min, out[1] = max of the density (rho, or (c phi + s p)^2 with
+ * rho = NULL): the kf range of guess_kf_bound (lwt.py:36-80) and the
+ * largest density of the low-density weight (lwt.py:322-329).             */
+int ofdft_lwt_range(ofdft_plan plan, const void* rho, const void* phi, const void* p, double c, double s,
+                    double* out2);
+typedef struct ofdft_lwt_params {
+  double alpha, beta;   /* rho^alpha (energy side), rho^beta (convolved)    */
+  double kfmax;         /* kf = 2 (3 pi^2 rho)^(1/3) is clamped to it       */
+  double ld;            /* low-density weight |rho|^ld / rhomax^ld where     */
+  double rhomax_ld;     /*   rho < 1e-6 (0 where rho < 0); rhomax_ld = that  */
+  int kdd;              /* 1, 2 or 3 potential terms (lwt.py:262-341)       */
+  int hermite;          /* 0 linear, 1 Hermite interpolation in kf          */
+  int mgpe;             /* 1: add the MGP kinetic-electron kernel           */
+  double mgpe_a, mgpe_b, mgpe_c; /* 4 pi erf(c G^2)^2 a exp(-b G^2) / G^2
+                                    (MGPOmegaE, kernel.py:408-440)          */
+} ofdft_lwt_params;
+/* The LWT term at rho (or (c phi + s p)^2) with the nsp node list kflist
+ * (host; lwt.py:148-165 with the 1e-16 widening), V_inout (device, may be
+ * NULL: energy only, the reference's kdd = 1) receives V += V_LWT.
+ * out[3] (host, x dV): [0] E = sum rho^alpha pot1, [1] 2 sum V_LWT phi(t)
+ * dphi/dt (phi mode with p, V given), [2] sum V_LWT rho (V given).        */
+int ofdft_lwt(ofdft_plan plan, const ofdft_lwt_params* params, const double* kflist, int nsp, const void* rho,
+              const void* phi, const void* p, double c, double s, void* V_inout, double* out3);
+
 /* ---- vector kernels used by the optimizer (optimization.py:92-280) ---- */
 /* out[0] = sum(a*b), out[1] = sum(a*a), out[2] = sum(b*b) in FP64 (host).  */
 int ofdft_dot3(ofdft_plan plan, const void* a, const void* b, double* out3);
