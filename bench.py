@@ -80,12 +80,12 @@ def build_system(name, D, W):
     return wl, grid, ions, ev
 
 
-def solve(D, ev, rho, precision, econv, verbose, phase1=False):
+def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact"):
     """One TN solve from ``rho`` (a DirectField already resident on the GPU;
     optimize_rho does not modify its guess)."""
     opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
                          optimization_options={"econv": econv, "precision": precision, "maxiter": 200,
-                                               "mixed_phase1": phase1})
+                                               "mixed_phase1": phase1, "hvp": hvp})
     opt.verbose = verbose
     opt.optimize_rho(guess_rho=rho)
     niter = sum(1 for l in opt.log[1:] if l["prec"] != "switch->fp64")
@@ -261,6 +261,18 @@ def main():
         out["mixed_energy_error_eV_per_atom"] = err
         out["mixed_energy_Ha"] = energies[-1]
         evals = sum(l["Nd"] + l["Nls"] for l in o64.log[1:]) / max(n64, 1)
+        # the reference's own TN on the FP64 kernels: the Hessian product as
+        # a forward difference of the potential (optimization.py:106-114)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ofd, nfd = solve(D, ev, rho_ini, "fp64", args.econv, False, hvp="fd")
+        torch.cuda.synchronize()
+        tfd = time.perf_counter() - t1
+        out["fp64_fd_gpu"] = {"value": nfd / tfd, "unit": "TN iterations/s", "iterations": nfd, "seconds": tfd,
+                              "energy_Ha": ofd.energy, "converged": ofd.converged,
+                              "hvp": "forward difference of V (the reference's TN), FP64 kernels"}
+        out["mixed_over_fp64_fd"] = value / (nfd / tfd)
+        out["time_to_solution_ratio_fd"] = tfd / (elapsed / args.steps)
     else:
         evals = sum(l["Nd"] + l["Nls"] for l in logs[-1][1:]) / max(iters / args.steps, 1)
     if not args.no_cpu_baseline:
