@@ -1049,7 +1049,8 @@ struct PostHvpFast {
   };
   __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i], W0[i]}; }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
-  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+  // (H p)(i) from the point's phi, p, W0 and the convolutions c
+  __device__ __forceinline__ T ap_of(const In& in, const T* c) const {
     const T ph = in.ph, pp = in.pp;
     const T r = ph * ph;
     if constexpr (sizeof(T) == 4) {
@@ -1065,9 +1066,7 @@ struct PostHvpFast {
         const T g = fp.beta56 ? sgn_pow23(ph) : ph * dev_pow(r, T(fp.beta) - T(1));
         ap += T(2 * fp.alpha) * g * pick3(c, fp.fW);
       }
-      Ap[i] = ap;
-      acc[0] += (double)(pp * ap);
-      return;
+      return ap;
     }
     T dv = T(0);
     if (fp.fH >= 0) dv += pick3(c, fp.fH);
@@ -1079,9 +1078,12 @@ struct PostHvpFast {
         dv += T(2 * fp.alpha) * rb1 * pick3(c, fp.fW);
       }
     }
-    const T ap = in.w0 * pp + ph * dv;
+    return in.w0 * pp + ph * dv;
+  }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const T ap = ap_of(in, c);
     Ap[i] = ap;
-    acc[0] += (double)(pp * ap);
+    acc[0] += (double)(in.pp * ap);
   }
 };
 
@@ -1333,7 +1335,8 @@ __global__ __launch_bounds__(FFT::TH) void k_axis_c2c(FFT fft, cplx<T>* __restri
 
 struct ConvKinds {
   int kind[3];
-  double scale;  // 1/N
+  double scale;    // 1/N
+  double coef[3];  // weight of field f in the spectral part of p.Ap (TN inner loop)
 };
 
 // |G|^2 in the transform precision (FP32 plans use the FP32 lattice copy).
@@ -1469,7 +1472,8 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec,
 template <typename T, int N1, int N2, int W>
 __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
                                                        const cplx<T>* __restrict__ tw, ConvKinds ck,
-                                                       const T* __restrict__ wtk, const int* __restrict__ skip) {
+                                                       const T* __restrict__ wtk, const int* __restrict__ skip,
+                                                       double* __restrict__ cgp) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
   const int t = threadIdx.x, w = t % W, q = t / W;
@@ -1498,6 +1502,9 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) lds[(w * N1 + k1) * (N2 + 1) + q] = v[k1];
   __syncthreads();
+  // sum over this thread's points of m |X|^2: the field's share of p.Ap in
+  // the TN inner loop (Hermitian weight 2 off the c = 0 and Nyquist planes)
+  double pap = 0.0;
   if (q < N1) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
@@ -1519,12 +1526,14 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
           const float g2 = gx * gx + gy * gy + gz * gz;
           float m = hartree ? (g2 > 0.f ? float(4.0 * kPi) * __builtin_amdgcn_rcpf(g2) : 0.f) : g2;
           m *= scale;
+          pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
           v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
         }
       } else {
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) {
           const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+          pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
           v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
         }
       }
@@ -1532,6 +1541,7 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
         const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+        pap += (double)m * ((double)(v[k2].x * v[k2].x) + (double)(v[k2].y * v[k2].y));
         v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
       }
     }
@@ -1553,6 +1563,11 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
   if (ok) {
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1) base[(long long)(N2 * n1 + q) * stride + c] = v[n1];
+  }
+  if (cgp) {
+    const bool edge = c == 0 || ((geo.n2 & 1) == 0 && c == geo.n2 / 2);
+    double acc[1] = {live ? pap * ck.coef[f] * (edge ? 1.0 : 2.0) : 0.0};
+    block_sums<1>(acc, cgp + ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
   }
 }
 
@@ -1682,6 +1697,7 @@ __global__ void k_cg_update_dev(const double* __restrict__ red, double r0, const
 // ---------------------------------------------------------------------------
 struct TnState {
   double r0, rconv, rmin, r1, pAp, beta, scale, pad;
+  double wpp;  // sum W0 p^2 of the current direction p (the pointwise part of pAp)
   int done;    // loop finished
   int it;      // index of the CG iteration being run
   int niter;   // iterations taken (the reference's `number`)
@@ -1696,19 +1712,21 @@ constexpr int kActRestore = 2;  // direction = Best
 constexpr int kActScale = 4;    // direction = r0 / pAp * p
 constexpr int kActNext = 8;     // p = res + beta * p
 
-// d = 0, r = -res, p = r; partial sums of r^2
+// d = 0, r = -res, p = r; partial sums of r^2 and (with W0) of W0 r^2
 template <typename T>
-__global__ void k_tn_init(const T* __restrict__ res, T* __restrict__ d, T* __restrict__ r, T* __restrict__ p,
-                          long long N, double* __restrict__ partials) {
-  double acc[1] = {0.0};
+__global__ void k_tn_init(const T* __restrict__ res, const T* __restrict__ W0, T* __restrict__ d, T* __restrict__ r,
+                          T* __restrict__ p, long long N, double* __restrict__ partials) {
+  double acc[2] = {0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x) {
     const T v = -res[i];
     d[i] = T(0);
     r[i] = v;
     p[i] = v;
-    acc[0] += (double)v * (double)v;
+    const double v2 = (double)v * (double)v;
+    acc[0] += v2;
+    if (W0) acc[1] += (double)W0[i] * v2;
   }
-  block_sums<1>(acc, partials + blockIdx.x);
+  block_sums<2>(acc, partials + 2 * (long long)blockIdx.x);
 }
 
 __global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit) {
@@ -1718,6 +1736,7 @@ __global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit
   st->rconv = 0.1 * r0;
   st->rmin = 1e10;  // rLists = [1e10]
   st->r1 = r0;
+  st->wpp = rr[1];
   st->pAp = 0.0;
   st->beta = 0.0;
   st->scale = 0.0;
@@ -1819,6 +1838,51 @@ __global__ void k_tn_reduce_ctrl(TnState* st, const double* __restrict__ partial
   if (threadIdx.x == 0) tn_ctrl(st, red[0], rr[0]);
 }
 
+// pAp = the pencil's spectral sums + sum W0 p^2 (kept in the state) into
+// red[0], before the inverse row pass that takes the CG step.
+__global__ void k_tn_pap(const TnState* st, const double* __restrict__ partials, long long nblk,
+                         double* __restrict__ red) {
+  if (st->done) return;
+  double tmp[1] = {thread_partials<1>(partials, nblk, 0)};
+  block_sums<1>(tmp, red);
+  if (threadIdx.x == 0) red[0] += st->wpp;
+}
+
+// r1 and the three W0 sums of the inverse row pass, the tests of
+// optimization.py:116-145, and sum W0 p^2 of the next direction
+// p' = r + beta p from W0 r^2 + 2 beta W0 r p + beta^2 W0 p^2.
+__global__ void k_tn_reduce_ctrl4(TnState* st, const double* __restrict__ partials, long long nblk,
+                                  const double* __restrict__ red, double* __restrict__ rr) {
+  if (st->done) return;
+  // the four sums of a block are adjacent: one pass over the blocks with
+  // two independent accumulator sets keeps two 32-byte loads in flight
+  double tmp[4] = {0.0, 0.0, 0.0, 0.0}, tm2[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long st2 = 2LL * blockDim.x;
+  long long b = threadIdx.x;
+  for (; b + blockDim.x < nblk; b += st2) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      tmp[s] += partials[b * 4 + s];
+      tm2[s] += partials[(b + blockDim.x) * 4 + s];
+    }
+  }
+  if (b < nblk) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) tmp[s] += partials[b * 4 + s];
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) tmp[s] += tm2[s];
+  block_sums<4>(tmp, rr);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    tn_ctrl(st, red[0], rr[0]);
+    if (st->act & kActNext) {
+      const double b = st->beta;
+      st->wpp = rr[1] + 2.0 * b * rr[2] + b * b * rr[3];
+    }
+  }
+}
+
 // p = r + beta p when the control asked for it: used only where no forward
 // row pass precedes the next product (no convolution fields).
 template <typename T>
@@ -1868,6 +1932,47 @@ struct PreHvpCG {
     h.apply(typename PreHvp<T>::In{x.ph, pp}, out);
   }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out, i); }
+};
+
+// The HVP's inverse row functor with the CG step folded in (the step of
+// optimization.py:116-128 with pAp = red[0] already known from the pencil's
+// spectral sums and the pointwise part): Best = direction when the previous
+// iteration asked for it, d += a p, r -= a (H p) with a = r0 / pAp; H p is
+// never stored.  acc: r^2 and, for the pointwise part of the next pAp,
+// W0 r^2, W0 r p, W0 p^2.  Nothing changes when pAp < 0 (the reference
+// leaves the loop before the update).
+template <typename T>
+struct PostHvpCG {
+  static constexpr int NS = 4;
+  static constexpr int kProf = PC_ROWS_INV_HVP;
+  PostHvpFast<T> h;  // phi, p, W0 and the functional; its Ap is not written
+  T* __restrict__ d;
+  T* __restrict__ r;
+  T* __restrict__ best;
+  const TnState* __restrict__ st;
+  const double* __restrict__ red;
+  __host__ __device__ int nio() const { return 7; }
+  struct In {
+    typename PostHvpFast<T>::In x;
+    T dd, rr;
+  };
+  __device__ __forceinline__ In load(long long i) const { return {h.load(i), d[i], r[i]}; }
+  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+    const double pAp = red[0];
+    if (pAp < 0.0) return;
+    const T ap = h.ap_of(in.x, c);
+    const T a = T(st->r0 / pAp);
+    if (st->act & kActKeep) best[i] = in.dd;
+    d[i] = in.dd + a * in.x.pp;
+    const T rn = in.rr - a * ap;
+    r[i] = rn;
+    const double rv = (double)rn, pv = (double)in.x.pp, w = (double)in.x.w0;
+    acc[0] += rv * rv;
+    acc[1] += w * rv * rv;
+    acc[2] += w * rv * pv;
+    acc[3] += w * pv * pv;
+  }
+  __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
 };
 
 // The five sums one outer iteration needs after the direction is found:
@@ -1954,6 +2059,13 @@ GridGeom geom(const ofdft_plan_s* P) {
 // Compile-time FFT lengths; any other 2^a 3^b 5^c length takes the runtime path.
 #define OFDFT_FIXED_SIZES(X) X(32) X(48) X(64) X(90) X(96) X(108) X(128) X(160) X(192)
 
+// length n has compile-time row / column / pencil kernels
+constexpr bool pencil_fixed(int n) {
+#define OFDFT_IS(NN) || n == NN
+  return false OFDFT_FIXED_SIZES(OFDFT_IS);
+#undef OFDFT_IS
+}
+
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // columns per workgroup of the register column kernels
 template <typename T>
@@ -2026,7 +2138,8 @@ void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const
 // reduce: the post functor's sums end in P->red, and also in the mapped host
 // buffer when to_host (for a fetch_sums that follows).
 template <typename T, class Post>
-int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce, bool to_host = true) {
+int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce, bool to_host = true,
+                    bool final_reduce = true) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
   const int PB = kRowPB<T>;
@@ -2051,7 +2164,8 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
     }
   }
   LAUNCHCHK();
-  if (reduce) {
+  // final_reduce = false: the block partials stay for the caller's own reduction
+  if (reduce && final_reduce) {
     hipLaunchKernelGGL((k_reduce_partials<Post::NS>), dim3(Post::NS), dim3(kRedThreads), 0, P->stream, P->partials, nblk, P->red,
                        to_host ? P->dhred : nullptr, P->skip);
     LAUNCHCHK();
@@ -2115,13 +2229,15 @@ void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t l
 }
 
 template <typename T>
-int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
+int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullptr, const double* coef = nullptr) {
   const int W = kColTileC;
   const size_t lds = (size_t)(W * (P->n[0] + 1) + P->n[0]) * sizeof(cplx<T>);
   dim3 grid((unsigned)((P->nh + W - 1) / W), (unsigned)P->n[1], (unsigned)F);
   ConvKinds ck;
   for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
   ck.scale = 1.0 / (double)P->N;
+  for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
+  if (cgp && !pencil_fixed(P->n[0])) return fail(OFDFT_EINVAL, "p.Ap sums need a compile-time pencil length");
   for (int f = 0; f < F; ++f)
     if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
   GridGeom g = geom(P);
@@ -2136,7 +2252,7 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds) {
                               (unsigned)F),                                                              \
                          dim3(RegTile<T>::W * Split<NN>::N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), \
                          P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                      \
-                         reinterpret_cast<const T*>(P->wtk), P->skip);                                   \
+                         reinterpret_cast<const T*>(P->wtk), P->skip, cgp);                              \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -2267,6 +2383,55 @@ int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap,
     if ((rc = fetch_sums(P, 1, &t))) return rc;
     *pAp = t;
   }
+  return 0;
+}
+
+// One inner CG step of the device TN loop: the HVP's transforms with the
+// spectral part of pAp summed in the pencil, pAp completed with sum W0 p^2
+// (k_tn_pap), and the step taken in the inverse row pass (PostHvpCG).
+// *nblk_rows: the partial-sum blocks that pass leaves for k_tn_reduce_ctrl4.
+template <typename T>
+int hvp_cg_impl(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T* best, const TnState* st,
+                long long* nblk_rows) {
+  FnParams fp = make_fn(P->fn);
+  const int F = fp.F;
+  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
+  int rc;
+  long long nblk_pen = 0;
+  if (F > 0) {
+    PreHvp<T> pre{phi, p, fp};
+    PreHvpCG<T> pcg{pre, phi, p, r, st};
+    if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+    int kinds[3] = {0, 0, 0};
+    double coef[3] = {0.0, 0.0, 0.0};
+    // p.Ap = sum W0 p^2 + sum_f coef_f g_f.(K_f * g_f) with g_f the field the
+    // forward pass transforms: Hartree g = 2 phi p, vW g = sign(phi) p, WT
+    // g = 2 beta rho^(beta-1) phi p (the weights of PostHvpFast over these)
+    if (fp.fH >= 0) {
+      kinds[fp.fH] = OFDFT_CONV_HARTREE;
+      coef[fp.fH] = 0.5;
+    }
+    if (fp.fV >= 0) {
+      kinds[fp.fV] = OFDFT_CONV_GG;
+      coef[fp.fV] = 0.5 * fp.y;
+    }
+    if (fp.fW >= 0) {
+      kinds[fp.fW] = OFDFT_CONV_WT;
+      coef[fp.fW] = fp.alpha / fp.beta;
+    }
+    if ((rc = launch_pencil<T>(P, F, kinds, P->partials, coef))) return rc;
+    nblk_pen = (long long)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W) * P->n[1] * F;
+    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+  }
+  hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
+                     P->red);
+  LAUNCHCHK();
+  PostHvpFast<T> h{phi, p, W0, nullptr, fp};
+  PostHvpCG<T> post{h, d, r, best, st, (const double*)P->red};
+  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
+  const long long npairs = ((long long)P->n[0] * P->n[1] + 1) / 2;
+  *nblk_rows = (npairs + kRowPB<T> - 1) / kRowPB<T>;
   return 0;
 }
 
@@ -3744,11 +3909,12 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   P->skip = nullptr;
   {
     ProfScope ps(P, PC_VEC, (long long)P->rs * 4 * N);
-    OFDFT_LAUNCH((k_tn_init<T>), dim3(nb), dim3(256), 0, P, res, d, r, p, N, P->partials);
+    OFDFT_LAUNCH((k_tn_init<T>), dim3(nb), dim3(256), 0, P, res, W0, d, r, p, N, P->partials);
   }
   LAUNCHCHK();
-  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr, (double*)nullptr,
-                     (const int*)nullptr);
+  // rr[0] = r0, rr[1] = sum W0 p^2 of the first direction
+  hipLaunchKernelGGL((k_reduce_partials<2>), dim3(2), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr,
+                     (double*)nullptr, (const int*)nullptr);
   LAUNCHCHK();
   hipLaunchKernelGGL(k_tn_start, dim3(1), dim3(1), 0, P->stream, st, (const double*)rr, maxfun);
   LAUNCHCHK();
@@ -3756,6 +3922,8 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   const size_t prof_mark = P->prof_pending.size();
   P->skip = &st->done;
   const bool fused = make_fn(P->fn).F > 0;  // a forward row pass carries the update of p
+  // the CG step rides on the inverse row pass when the pencil can sum p.Ap
+  const bool cg_fused = !fused || pencil_fixed(P->n[0]);
   int rc = 0, queued = 0, k = std::max(1, chunk);
   while (queued < maxfun) {
     for (int j = 0; j < k && queued < maxfun; ++j, ++queued) {
@@ -3764,14 +3932,21 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
         ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
         OFDFT_LAUNCH((k_tn_next<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, (const T*)r, p, N);
       }
-      if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr, (const T*)r, (const TnState*)st))) break;
-      {
-        ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * N);
-        OFDFT_LAUNCH((k_tn_step<T>), dim3(nb), dim3(256), 0, P, st, (const double*)P->red,
-                     (const T*)p, (const T*)Ap, d, r, best, N, P->partials);
+      if (cg_fused) {
+        long long nbr = 0;
+        if ((rc = hvp_cg_impl<T>(P, phi, p, W0, d, r, best, (const TnState*)st, &nbr))) break;
+        hipLaunchKernelGGL(k_tn_reduce_ctrl4, dim3(1), dim3(kRedThreads), 0, P->stream, st,
+                           (const double*)P->partials, nbr, (const double*)P->red, rr);
+      } else {
+        if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr, (const T*)r, (const TnState*)st))) break;
+        {
+          ProfScope ps(P, PC_VEC, (long long)P->rs * 6 * N);
+          OFDFT_LAUNCH((k_tn_step<T>), dim3(nb), dim3(256), 0, P, st, (const double*)P->red,
+                       (const T*)p, (const T*)Ap, d, r, best, N, P->partials);
+        }
+        hipLaunchKernelGGL(k_tn_reduce_ctrl, dim3(1), dim3(kRedThreads), 0, P->stream, st,
+                           (const double*)P->partials, (long long)nb, (const double*)P->red, rr);
       }
-      hipLaunchKernelGGL(k_tn_reduce_ctrl, dim3(1), dim3(kRedThreads), 0, P->stream, st,
-                         (const double*)P->partials, (long long)nb, (const double*)P->red, rr);
       if (hipGetLastError() != hipSuccess) {
         rc = fail(OFDFT_EHIP, "kernel launch failed in the TN loop");
         break;
