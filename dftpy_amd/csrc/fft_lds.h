@@ -12,6 +12,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "roots.h"
+
 namespace ofdft {
 
 template <typename T>
@@ -123,6 +125,40 @@ __device__ __forceinline__ void dft8(cplx<T>* v, bool inv) {
   v[7] = csub(e[3], o3);
 }
 
+// Odd prime R (7, 11, 13): inputs paired as v_j +- v_(R-j), then
+// y_k = a_k -+ i b_k and y_(R-k) = a_k +- i b_k with a_k = v_0 + sum_j
+// cos(2 pi jk/R)(v_j + v_(R-j)), b_k = sum_j sin(2 pi jk/R)(v_j - v_(R-j)).
+template <typename T, int R>
+__device__ __forceinline__ void dftp(cplx<T>* v, bool inv) {
+  constexpr int H = (R - 1) / 2;
+  cplx<T> t[H], d[H];
+  cplx<T> y0 = v[0];
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    t[j - 1] = cadd(v[j], v[R - j]);
+    d[j - 1] = csub(v[j], v[R - j]);
+    y0 = cadd(y0, t[j - 1]);
+  }
+  cplx<T> y[R];
+  y[0] = y0;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    cplx<T> a = v[0], b = {T(0), T(0)};
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const T c = T(root_re<R>((j * k) % R));
+      const T sn = T(-root_im<R>((j * k) % R));
+      a = {a.x + c * t[j - 1].x, a.y + c * t[j - 1].y};
+      b = {b.x + sn * d[j - 1].x, b.y + sn * d[j - 1].y};
+    }
+    const cplx<T> rb = rotq(b, inv);  // -+ i b
+    y[k] = cadd(a, rb);
+    y[R - k] = csub(a, rb);
+  }
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = y[k];
+}
+
 template <typename T, int R>
 __device__ __forceinline__ void bfly_load(const cplx<T>* __restrict__ x, int j, int nb, int k, int twstep,
                                           const cplx<T>* __restrict__ tw, bool inv, cplx<T>* v) {
@@ -140,6 +176,7 @@ __device__ __forceinline__ void bfly_load(const cplx<T>* __restrict__ x, int j, 
   if constexpr (R == 3) dft3(v, inv);
   if constexpr (R == 4) dft4(v, inv);
   if constexpr (R == 5) dft5(v, inv);
+  if constexpr (R == 7 || R == 11 || R == 13) dftp<T, R>(v, inv);
   if constexpr (R == 8) dft8(v, inv);
 }
 
@@ -153,13 +190,55 @@ __device__ __forceinline__ void bfly_store(cplx<T>* __restrict__ x, int obase, i
 // butterfly slots per pass; the caller sizes blockDim.x so that
 // KB * blockDim.x >= ntr * max_pass(n / R).  Must be called by the whole
 // workgroup (contains barriers).
+// One pass of radix 11 or 13 (register arrays of 13 points per slot, kept
+// out of the common passes so that they do not size every runtime kernel).
 template <typename T, int KB>
+__device__ void lds_pass_large(cplx<T>* lds, int ld, int ntr, int n, int R, int Ns, const cplx<T>* __restrict__ tw,
+                               bool inv) {
+  const int nb = n / R;
+  const int total = ntr * nb;
+  const int twstep = n / (Ns * R);
+  cplx<T> v[KB][13];
+  int ob[KB];
+#pragma unroll
+  for (int kk = 0; kk < KB; ++kk) {
+    const int t = threadIdx.x + kk * blockDim.x;
+    ob[kk] = -1;
+    if (t < total) {
+      const int b = t / nb;
+      const int j = t - b * nb;
+      const int k = j % Ns;
+      ob[kk] = b * ld + (j - k) * R + k;
+      if (R == 11) bfly_load<T, 11>(lds + b * ld, j, nb, k, twstep, tw, inv, v[kk]);
+      else bfly_load<T, 13>(lds + b * ld, j, nb, k, twstep, tw, inv, v[kk]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int kk = 0; kk < KB; ++kk) {
+    if (ob[kk] >= 0) {
+      if (R == 11) bfly_store<T, 11>(lds, ob[kk], Ns, v[kk]);
+      else bfly_store<T, 13>(lds, ob[kk], Ns, v[kk]);
+    }
+  }
+  __syncthreads();
+}
+
+// LG: the schedule may hold radix 11 or 13 passes.
+template <typename T, int KB, bool LG = false>
 __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const cplx<T>* __restrict__ tw,
                         bool inv) {
   const int n = pl.n;
   int Ns = 1;
   for (int ip = 0; ip < pl.npass; ++ip) {
     const int R = pl.radix[ip];
+    if constexpr (LG) {
+      if (R > 8) {
+        lds_pass_large<T, KB>(lds, ld, ntr, n, R, Ns, tw, inv);
+        Ns *= R;
+        continue;
+      }
+    }
     const int nb = n / R;
     const int total = ntr * nb;
     const int twstep = n / (Ns * R);
@@ -180,6 +259,7 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
           case 3: bfly_load<T, 3>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
           case 4: bfly_load<T, 4>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
           case 5: bfly_load<T, 5>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
+          case 7: bfly_load<T, 7>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
           default: bfly_load<T, 8>(xs, j, nb, k, twstep, tw, inv, v[kk]); break;
         }
       }
@@ -193,6 +273,7 @@ __device__ void lds_fft(cplx<T>* lds, int ld, int ntr, const Pass1D& pl, const c
           case 3: bfly_store<T, 3>(lds, ob[kk], Ns, v[kk]); break;
           case 4: bfly_store<T, 4>(lds, ob[kk], Ns, v[kk]); break;
           case 5: bfly_store<T, 5>(lds, ob[kk], Ns, v[kk]); break;
+          case 7: bfly_store<T, 7>(lds, ob[kk], Ns, v[kk]); break;
           default: bfly_store<T, 8>(lds, ob[kk], Ns, v[kk]); break;
         }
       }
@@ -295,7 +376,7 @@ struct FFTFixed {
 };
 
 // FFT policy for any 2^a 3^b 5^c length, schedule read at run time.
-template <int KB_>
+template <int KB_, bool LG = false>
 struct FFTRuntime {
   static constexpr int KB = KB_;
   static constexpr int TH = 1024;
@@ -305,7 +386,7 @@ struct FFTRuntime {
   __device__ static int th() { return blockDim.x; }
   template <typename T>
   __device__ void run(cplx<T>* lds, int ld, int ntr, const cplx<T>* __restrict__ tw, bool inv) const {
-    lds_fft<T, KB>(lds, ld, ntr, pl, tw, inv);
+    lds_fft<T, KB, LG>(lds, ld, ntr, pl, tw, inv);
   }
 };
 

@@ -70,19 +70,32 @@ bool make_pass(int n, Pass1D* pl, bool asc) {
     pl->radix[pl->npass++] = r;
     return true;
   };
-  int c8 = 0, c4 = 0, c2 = 0, c3 = 0, c5 = 0;
+  int c8 = 0, c4 = 0, c2 = 0, c3 = 0, c5 = 0, c7 = 0, c11 = 0, c13 = 0;
   while (m % 8 == 0) { ++c8; m /= 8; }
   while (m % 4 == 0) { ++c4; m /= 4; }
   while (m % 2 == 0) { ++c2; m /= 2; }
   while (m % 3 == 0) { ++c3; m /= 3; }
   while (m % 5 == 0) { ++c5; m /= 5; }
-  const int order[5] = {asc ? 3 : 8, asc ? 5 : 4, 2, asc ? 4 : 3, asc ? 8 : 5};
+  while (m % 7 == 0) { ++c7; m /= 7; }
+  while (m % 11 == 0) { ++c11; m /= 11; }
+  while (m % 13 == 0) { ++c13; m /= 13; }
+  // 7, 11 and 13 (the other factors of the reference's max_prime = 13
+  // grids) run last, after the order of the 2/3/5 passes
+  const int order[8] = {asc ? 3 : 8, asc ? 5 : 4, 2, asc ? 4 : 3, asc ? 8 : 5, 7, 11, 13};
   for (int r : order) {
-    const int c = r == 8 ? c8 : r == 4 ? c4 : r == 2 ? c2 : r == 3 ? c3 : c5;
+    const int c = r == 8 ? c8 : r == 4 ? c4 : r == 2 ? c2 : r == 3 ? c3 : r == 5 ? c5 : r == 7 ? c7 : r == 11 ? c11 : c13;
     for (int i = 0; i < c; ++i)
       if (!push(r)) return false;
   }
   return m == 1;
+}
+
+// the schedule holds a radix 11 or 13 pass (runtime kernels with larger
+// butterfly registers)
+bool has_large(const Pass1D& pl) {
+  for (int i = 0; i < pl.npass; ++i)
+    if (pl.radix[i] > 8) return true;
+  return false;
 }
 
 int max_butterflies(const Pass1D& pl) {
@@ -2113,9 +2126,9 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
 #undef OFDFT_CASE
     default: {
       FFTLaunch L = fft_launch(P->pl[2], PB * F);
-      if (L.kb == 1) go_rows_fwd<T>(P, pre, FFTRuntime<1>{P->pl[2]}, F, grid, L.threads, lds);
-      else if (L.kb == 2) go_rows_fwd<T>(P, pre, FFTRuntime<2>{P->pl[2]}, F, grid, L.threads, lds);
-      else go_rows_fwd<T>(P, pre, FFTRuntime<4>{P->pl[2]}, F, grid, L.threads, lds);
+      if (L.kb == 1) has_large(P->pl[2]) ? go_rows_fwd<T>(P, pre, FFTRuntime<1, true>{P->pl[2]}, F, grid, L.threads, lds) : go_rows_fwd<T>(P, pre, FFTRuntime<1>{P->pl[2]}, F, grid, L.threads, lds);
+      else if (L.kb == 2) has_large(P->pl[2]) ? go_rows_fwd<T>(P, pre, FFTRuntime<2, true>{P->pl[2]}, F, grid, L.threads, lds) : go_rows_fwd<T>(P, pre, FFTRuntime<2>{P->pl[2]}, F, grid, L.threads, lds);
+      else has_large(P->pl[2]) ? go_rows_fwd<T>(P, pre, FFTRuntime<4, true>{P->pl[2]}, F, grid, L.threads, lds) : go_rows_fwd<T>(P, pre, FFTRuntime<4>{P->pl[2]}, F, grid, L.threads, lds);
     }
   }
   LAUNCHCHK();
@@ -2157,9 +2170,9 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
 #undef OFDFT_CASE
       default: {
         FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
-        if (L.kb == 1) go_rows_inv<T>(P, post, FFTRuntime<1>{P->pl[2]}, F, src, grid, L.threads, lds, part);
-        else if (L.kb == 2) go_rows_inv<T>(P, post, FFTRuntime<2>{P->pl[2]}, F, src, grid, L.threads, lds, part);
-        else go_rows_inv<T>(P, post, FFTRuntime<4>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        if (L.kb == 1) has_large(P->pl[2]) ? go_rows_inv<T>(P, post, FFTRuntime<1, true>{P->pl[2]}, F, src, grid, L.threads, lds, part) : go_rows_inv<T>(P, post, FFTRuntime<1>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else if (L.kb == 2) has_large(P->pl[2]) ? go_rows_inv<T>(P, post, FFTRuntime<2, true>{P->pl[2]}, F, src, grid, L.threads, lds, part) : go_rows_inv<T>(P, post, FFTRuntime<2>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else has_large(P->pl[2]) ? go_rows_inv<T>(P, post, FFTRuntime<4, true>{P->pl[2]}, F, src, grid, L.threads, lds, part) : go_rows_inv<T>(P, post, FFTRuntime<4>{P->pl[2]}, F, src, grid, L.threads, lds, part);
       }
     }
   }
@@ -2211,9 +2224,9 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
     default: {
       const Pass1D& pl = P->pl[axis];
       FFTLaunch L = fft_launch(pl, W);
-      if (L.kb == 1) go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
-      else if (L.kb == 2) go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
-      else go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      if (L.kb == 1) has_large(pl) ? go_axis<T>(P, FFTRuntime<1, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv) : go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else if (L.kb == 2) has_large(pl) ? go_axis<T>(P, FFTRuntime<2, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv) : go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else has_large(pl) ? go_axis<T>(P, FFTRuntime<4, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv) : go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
     }
   }
   LAUNCHCHK();
@@ -2258,9 +2271,9 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
 #undef OFDFT_CASE
     default: {
       FFTLaunch L = fft_launch(P->pl[0], W);
-      if (L.kb == 1) go_pencil<T>(P, FFTRuntime<1>{P->pl[0]}, grid, L.threads, lds, g, ck);
-      else if (L.kb == 2) go_pencil<T>(P, FFTRuntime<2>{P->pl[0]}, grid, L.threads, lds, g, ck);
-      else go_pencil<T>(P, FFTRuntime<4>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      if (L.kb == 1) has_large(P->pl[0]) ? go_pencil<T>(P, FFTRuntime<1, true>{P->pl[0]}, grid, L.threads, lds, g, ck) : go_pencil<T>(P, FFTRuntime<1>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else if (L.kb == 2) has_large(P->pl[0]) ? go_pencil<T>(P, FFTRuntime<2, true>{P->pl[0]}, grid, L.threads, lds, g, ck) : go_pencil<T>(P, FFTRuntime<2>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else has_large(P->pl[0]) ? go_pencil<T>(P, FFTRuntime<4, true>{P->pl[0]}, grid, L.threads, lds, g, ck) : go_pencil<T>(P, FFTRuntime<4>{P->pl[0]}, grid, L.threads, lds, g, ck);
     }
   }
   LAUNCHCHK();
@@ -4085,7 +4098,7 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   for (int i = 0; i < 3; ++i) {
     if (!make_pass(P->n[i], &P->pl[i], P->prec == OFDFT_F32)) {
       delete P;
-      return fail(OFDFT_EINVAL, "grid sizes must be products of 2, 3 and 5 (at most 12 passes)");
+      return fail(OFDFT_EINVAL, "grid sizes must be products of 2, 3, 5, 7, 11 and 13 (at most 12 passes)");
     }
   }
   // LDS limits (64 KiB per workgroup)

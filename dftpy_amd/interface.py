@@ -183,9 +183,8 @@ def ConfigParser(config, ions=None, rhoini=None, pseudo=None, grid=None):
         nr = np.asarray(field.grid.nrR)
         rhoini = field
     else:
-        # FFT lengths on the GPU are 2^a 3^b 5^c: the prime bound is at most 5
         nr = ecut2nr(lattice=lattice, spacing=config["GRID"]["spacing"] / BOHR,
-                     max_prime=min(int(config["GRID"]["maxprime"]), 5))
+                     max_prime=int(config["GRID"]["maxprime"]))
     print("The final grid size is ", nr)
     if grid is None:
         grid = DirectGrid(lattice=lattice, nr=nr, full=config["GRID"]["gfull"])

@@ -17,7 +17,9 @@ from tests.conftest import DATA
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(32, 32, 32), (20, 30, 20), (36, 36, 72), (48, 40, 30), (96, 96, 96), (24, 50, 18), (15, 9, 20), (20, 18, 15),
-          (160, 24, 192), (108, 90, 48), (90, 64, 108)]
+          (160, 24, 192), (108, 90, 48), (90, 64, 108),
+          # factors 7, 11, 13 (grids of the reference's max_prime = 13 sizing)
+          (20, 30, 22), (14, 22, 26), (7, 11, 13), (143, 12, 10), (28, 44, 52), (12, 10, 143)]
 
 
 def _plan(nr, lattice, prec="fp64", full=False):
