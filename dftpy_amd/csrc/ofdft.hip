@@ -2126,15 +2126,11 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
 #undef OFDFT_CASE
     default: {
       FFTLaunch L = fft_launch(P->pl[2], PB * F);
-      if (L.kb == 1) has_large(P->pl[2])
-        ? go_rows_fwd<T>(P, pre, FFTRuntime<1, true>{P->pl[2]}, F, grid, L.threads, lds)
-        : go_rows_fwd<T>(P, pre, FFTRuntime<1>{P->pl[2]}, F, grid, L.threads, lds);
-      else if (L.kb == 2) has_large(P->pl[2])
-        ? go_rows_fwd<T>(P, pre, FFTRuntime<2, true>{P->pl[2]}, F, grid, L.threads, lds)
-        : go_rows_fwd<T>(P, pre, FFTRuntime<2>{P->pl[2]}, F, grid, L.threads, lds);
-      else has_large(P->pl[2])
-        ? go_rows_fwd<T>(P, pre, FFTRuntime<4, true>{P->pl[2]}, F, grid, L.threads, lds)
-        : go_rows_fwd<T>(P, pre, FFTRuntime<4>{P->pl[2]}, F, grid, L.threads, lds);
+      // radix 11 / 13 schedules: one instantiation, up to four slots per thread
+      if (has_large(P->pl[2])) go_rows_fwd<T>(P, pre, FFTRuntime<4, true>{P->pl[2]}, F, grid, L.threads, lds);
+      else if (L.kb == 1) go_rows_fwd<T>(P, pre, FFTRuntime<1>{P->pl[2]}, F, grid, L.threads, lds);
+      else if (L.kb == 2) go_rows_fwd<T>(P, pre, FFTRuntime<2>{P->pl[2]}, F, grid, L.threads, lds);
+      else go_rows_fwd<T>(P, pre, FFTRuntime<4>{P->pl[2]}, F, grid, L.threads, lds);
     }
   }
   LAUNCHCHK();
@@ -2176,15 +2172,11 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
 #undef OFDFT_CASE
       default: {
         FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
-        if (L.kb == 1) has_large(P->pl[2])
-          ? go_rows_inv<T>(P, post, FFTRuntime<1, true>{P->pl[2]}, F, src, grid, L.threads, lds, part)
-          : go_rows_inv<T>(P, post, FFTRuntime<1>{P->pl[2]}, F, src, grid, L.threads, lds, part);
-        else if (L.kb == 2) has_large(P->pl[2])
-          ? go_rows_inv<T>(P, post, FFTRuntime<2, true>{P->pl[2]}, F, src, grid, L.threads, lds, part)
-          : go_rows_inv<T>(P, post, FFTRuntime<2>{P->pl[2]}, F, src, grid, L.threads, lds, part);
-        else has_large(P->pl[2])
-          ? go_rows_inv<T>(P, post, FFTRuntime<4, true>{P->pl[2]}, F, src, grid, L.threads, lds, part)
-          : go_rows_inv<T>(P, post, FFTRuntime<4>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        // radix 11 / 13 schedules: one instantiation, up to four slots per thread
+        if (has_large(P->pl[2])) go_rows_inv<T>(P, post, FFTRuntime<4, true>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else if (L.kb == 1) go_rows_inv<T>(P, post, FFTRuntime<1>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else if (L.kb == 2) go_rows_inv<T>(P, post, FFTRuntime<2>{P->pl[2]}, F, src, grid, L.threads, lds, part);
+        else go_rows_inv<T>(P, post, FFTRuntime<4>{P->pl[2]}, F, src, grid, L.threads, lds, part);
       }
     }
   }
@@ -2236,15 +2228,11 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
     default: {
       const Pass1D& pl = P->pl[axis];
       FFTLaunch L = fft_launch(pl, W);
-      if (L.kb == 1) has_large(pl)
-        ? go_axis<T>(P, FFTRuntime<1, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv)
-        : go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
-      else if (L.kb == 2) has_large(pl)
-        ? go_axis<T>(P, FFTRuntime<2, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv)
-        : go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
-      else has_large(pl)
-        ? go_axis<T>(P, FFTRuntime<4, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv)
-        : go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      // radix 11 / 13 schedules: one instantiation, up to four slots per thread
+      if (has_large(pl)) go_axis<T>(P, FFTRuntime<4, true>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else if (L.kb == 1) go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else if (L.kb == 2) go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      else go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
     }
   }
   LAUNCHCHK();
@@ -2289,15 +2277,11 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
 #undef OFDFT_CASE
     default: {
       FFTLaunch L = fft_launch(P->pl[0], W);
-      if (L.kb == 1) has_large(P->pl[0])
-        ? go_pencil<T>(P, FFTRuntime<1, true>{P->pl[0]}, grid, L.threads, lds, g, ck)
-        : go_pencil<T>(P, FFTRuntime<1>{P->pl[0]}, grid, L.threads, lds, g, ck);
-      else if (L.kb == 2) has_large(P->pl[0])
-        ? go_pencil<T>(P, FFTRuntime<2, true>{P->pl[0]}, grid, L.threads, lds, g, ck)
-        : go_pencil<T>(P, FFTRuntime<2>{P->pl[0]}, grid, L.threads, lds, g, ck);
-      else has_large(P->pl[0])
-        ? go_pencil<T>(P, FFTRuntime<4, true>{P->pl[0]}, grid, L.threads, lds, g, ck)
-        : go_pencil<T>(P, FFTRuntime<4>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      // radix 11 / 13 schedules: one instantiation, up to four slots per thread
+      if (has_large(P->pl[0])) go_pencil<T>(P, FFTRuntime<4, true>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else if (L.kb == 1) go_pencil<T>(P, FFTRuntime<1>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else if (L.kb == 2) go_pencil<T>(P, FFTRuntime<2>{P->pl[0]}, grid, L.threads, lds, g, ck);
+      else go_pencil<T>(P, FFTRuntime<4>{P->pl[0]}, grid, L.threads, lds, g, ck);
     }
   }
   LAUNCHCHK();
