@@ -48,8 +48,13 @@ def test_r2c_c2r_match_numpy(nr, prec):
     assert torch.equal(Ft, F0)
 
 
-def _al4(full=False, kedf="TFVW", y=1.0 / 9.0):
+def _al4(full=False, kedf="TFVW", y=1.0 / 9.0, shear=None):
     wl = W.build("al4_32")
+    if shear is not None:
+        # the same fractional positions in a sheared cell
+        frac = np.linalg.solve(wl["cell"].T, wl["positions"].T).T
+        wl["cell"] = wl["cell"] @ (np.eye(3) + shear)
+        wl["positions"] = frac @ wl["cell"]
     grid = O.Grid(wl["cell"], wl["nr"], full=full)
     ions = O.Ions(wl["symbols"], wl["positions"], wl["cell"])
     tabs = {k: O.empty_core_table(*W.EMPTY_CORE[k]) for k in wl["tables"]}
@@ -267,15 +272,15 @@ def _tn_host(hvp, res0, maxfun):
     return direction, it + 1, stat, pAp
 
 
-def _tn_case(mu, maxfun, chunk, local=False):
+def _tn_case(mu, maxfun, chunk, local=False, full=False, shear=None):
     """Device TN loop against the host restatement on the 4-atom cell; with
     `local` only TF + LDA + PP (no convolution fields, so no forward row pass
     carries the CG update)."""
     from dftpy_amd import _lib as L
-    print("case tn_direction", mu, maxfun, chunk, local, flush=True)
-    wl, grid, ions, pp = _al4()
+    print("case tn_direction", mu, maxfun, chunk, local, full, shear is not None, flush=True)
+    wl, grid, ions, pp = _al4(full=full, shear=shear)
     rho = wl["rho_ini"]
-    P = _plan(wl["nr"], wl["cell"], "fp64")
+    P = _plan(wl["nr"], wl["cell"], "fp64", full=full)
     P.set_functional(tf=True, vw=not local, wt=not local, lda=True, hartree=not local, pseudo=True, rho0=rho.mean())
     P.set_vloc(torch.tensor(pp.vreal, dtype=P.dtype, device=P.device))
     t = lambda a: torch.tensor(a, dtype=P.dtype, device=P.device)
@@ -323,6 +328,17 @@ def test_tn_direction_matches_host_loop(mu, maxfun, branch, chunk):
     returns its direction and iteration count."""
     n, status = _tn_case(mu, maxfun, chunk)
     assert status == branch
+
+
+@pytest.mark.parametrize("full", [False, True])
+def test_tn_direction_sheared_cell(full):
+    """p.Ap of the device loop comes from the pencil's spectral sums with the
+    Hermitian weights (and, with full=1 frequencies, the Hermitian part of
+    the multipliers); on a sheared cell |G(k)| and |G(-k)| differ on the
+    Nyquist planes, and the loop still follows the host loop."""
+    shear = np.array([[0.0, 0.04, -0.03], [0.02, 0.0, 0.05], [-0.01, 0.03, 0.0]])
+    n, status = _tn_case(0.31, 50, 3, full=full, shear=shear)
+    assert status == "CONV"
 
 
 @pytest.mark.parametrize("mu,branch", [(-0.3, "CONV"), (0.0, "FAILED"), (0.15, "WARN : Not reduce"), (40.0, "WARN")])
