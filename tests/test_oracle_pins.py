@@ -558,3 +558,17 @@ def test_mgp_kernel_table_is_the_t_integral(sym):
             ref = 0.5 * (ref + O.wt_kernel_table(np.array([e]))[0])
         got = np.interp(e, eta, tab)
         assert abs(got - ref) < 3e-3 * max(abs(ref), 0.1), (e, got, ref)
+
+
+def test_lindhard_large_eta_series_against_closed_form():
+    """The large-eta series (kernel.py:80-120, used above eta = 3.65) against
+    the closed form 1/(1/2 + (1 - eta^2)/(4 eta) ln|(1+eta)/(1-eta)|) -
+    3 mu eta^2 - lambda where the closed form still has its digits (eta up to
+    5; beyond that its cancellation grows and the series is the accurate
+    one): a wrong or mistyped series coefficient fails at 1e-12."""
+    from oracle.ofdft_oracle import _lindhard_large
+    eta = np.linspace(3.65, 5.0, 28)
+    for lb, mu in ((1.0, 1.0), (0.6, 1.0), (1.0, 0.0)):
+        closed = 1.0 / (0.5 + 0.25 * (1 - eta ** 2) * np.log((1 + eta) / np.abs(1 - eta)) / eta) \
+            - 3 * mu * eta ** 2 - lb
+        assert np.abs(_lindhard_large(eta, lb, mu) - closed).max() < 1e-12 * np.abs(closed).max()
