@@ -18,7 +18,8 @@ tf_functional               closed form: uniform gas, finite difference of E
 lda_functional              PZ81 constants, FD of E, uniform-gas value
 vw_functional               zero for uniform rho; single-cosine closed form
 wt_functional, lindhard     test_functional.py:27 known answer 2.916818700014412
-                            on Al_fde_rho.pp; F(eta->0)=1-lambda limit; FD of E
+                            on Al_fde_rho.pp; F(eta->0)=1-lambda limit; FD of E;
+                            large-eta series = closed form (1e-12, eta 3.65-5)
 hartree_functional          single-cosine closed form E = pi*Omega*d^2/G^2
 LocalPseudo (PME, exact)    test_read_pp.py:12 (1.3497046, atol 1e-4);
                             test_pme.py PME-vs-exact energies and forces;
