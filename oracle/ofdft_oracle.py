@@ -25,7 +25,8 @@ LocalPseudo (PME, exact)    test_read_pp.py:12 (1.3497046, atol 1e-4);
                             test_pme.py PME-vs-exact energies and forces;
                             brute-force structure factor on a tiny grid
 local_forces (exact, PME)   test_pme.py:32-34 mutual agreement; FD of energy
-Ewald                       test_ewald.py:33-37 (e_corr, e_real, e_rec)
+Ewald                       test_ewald.py:33-37 (e_corr, e_real, e_rec); the
+                            total independent of eta (0.08-0.4, 1e-11)
 optimize_rho (TN)           notebook density_optimization.ipynb:174-183 log
                             (E0 -8.090977710718, final -8.2811389962163 Ha,
                             mu 0.30115196532053357); test_density_opt.py:46

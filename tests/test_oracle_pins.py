@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from dftpy_amd import workloads as W
 from tests.conftest import DATA
 
 NOTEBOOK_E = [-8.090977710718, -8.273226052119, -8.280858141000, -8.281101062862, -8.281135978745,
@@ -572,3 +573,22 @@ def test_lindhard_large_eta_series_against_closed_form():
         closed = 1.0 / (0.5 + 0.25 * (1 - eta ** 2) * np.log((1 + eta) / np.abs(1 - eta)) / eta) \
             - 3 * mu * eta ** 2 - lb
         assert np.abs(_lindhard_large(eta, lb, mu) - closed).max() < 1e-12 * np.abs(closed).max()
+
+
+def test_ewald_total_is_independent_of_eta():
+    """The Ewald split (ewald.py:280-322 real-space erfc sum, :448-465
+    reciprocal sum, :468-482 self and G = 0 terms) at Gaussian widths eta
+    from 0.08 (real-space sum 1.26 Ha) to 0.4 (reciprocal sum 3.3 Ha): the
+    total is the same lattice sum, so a wrong factor or sign in any of the
+    three parts fails at 1e-11."""
+    wl = W.build("al4_32")
+    grid = O.Grid(wl["cell"], (32, 32, 32), full=False)
+    pos = wl["positions"] + np.array([[0.1, 0.2, -0.1], [0, 0, 0], [0.05, 0, 0], [0, -0.07, 0.02]])
+    ions = O.Ions(wl["symbols"], pos, wl["cell"])
+    ions.set_charges({"Al": 3.0})
+    ew = O.Ewald(grid, ions, PME=False, precision=1e-12)
+    tot = []
+    for eta in (0.08, 0.15, 0.25, 0.4):
+        ew.eta = eta
+        tot.append(ew.energy_corr() + ew.energy_real() + ew.energy_rec())
+    assert np.ptp(tot) < 1e-11 * abs(tot[0])
