@@ -51,9 +51,16 @@ def _hessian_product(ev, phi, V, aux, u):
     return out
 
 
+def _device_only(total):
+    if total.plugin_terms():
+        raise NotImplementedError("autograd covers the device terms only; %s are plugin terms"
+                                  % [k for k, _ in total.plugin_terms()])
+
+
 class _Potential(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, total, grid):
+        _device_only(total)
         ev = total.evaluator(grid, rho.dtype)
         with torch._C.DisableTorchFunctionSubclass():
             r = rho.detach().contiguous()
@@ -75,6 +82,7 @@ class _Potential(torch.autograd.Function):
 class _Energy(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, total, grid):
+        _device_only(total)
         ev = total.evaluator(grid, rho.dtype)
         with torch._C.DisableTorchFunctionSubclass():
             sums = ev.eval_rho(rho.detach().contiguous(), None)

@@ -267,9 +267,11 @@ class KEDF(AbstractFunctional):
     def compute(self, density, calcType=("E", "V"), split=False, **kwargs):
         tot = self._total()
         if split:
+            # keyed by engine name as kedf/__init__.py:158-172 (name2functions)
             out = tot.get_energy_potential(density, calcType=calcType, split=True)
-            return {k.split("-", 1)[1] if k.startswith("KEDF-") else k: v for k, v in out.items()
-                    if k.startswith("KEDF-")}
+            nl = "WT-NL" if self.lwt is None else self.name.upper() + "-NL"
+            names = {"KEDF-TF": "TF", "KEDF-VW": "VW", "KEDF-NL": nl, "KEDF-GGA": "GGA"}
+            return {names[k]: v for k, v in out.items() if k in names}
         return tot.compute(density, calcType=calcType)
 
 
@@ -430,7 +432,7 @@ class DeviceEvaluator:
     """The fused evaluation of a TotalFunctional on one grid and precision:
     holds the device plan with the merged functional switches and V_loc."""
 
-    def __init__(self, total, grid, dtype):
+    def __init__(self, total, grid, dtype, only=None):
         from .plan import Plan
         self.total = total
         self.grid = grid
@@ -440,6 +442,13 @@ class DeviceEvaluator:
                       device=grid.get_device())
         self.dtype = dtype
         self.terms = total._merged_terms()
+        if only is not None:
+            # one term of the sum on its own (split=True potentials)
+            for flag in ("tf", "vw", "wt", "lda", "hartree", "pseudo"):
+                self.terms[flag] = self.terms[flag] and flag == only
+            for side in ("gga", "lwt"):
+                if side != only:
+                    self.terms[side] = None
         self.configured_rho0 = None
         if self.terms["pseudo"]:
             self.P.set_vloc(self.terms["pseudo_obj"].vreal_device(dtype))
@@ -502,6 +511,25 @@ class DeviceEvaluator:
         return self._with_gga(sums, phi=phi, p=p, c=c, s=s, V=V)
 
 
+def _on_device(o, rho):
+    """A plugin term's FunctionalOutput with its fields as device
+    DirectFields of rho's grid, dtype and device (a plugin may return numpy
+    arrays, torch tensors or DirectFields)."""
+    if not isinstance(o, FunctionalOutput):
+        raise TypeError("a functional must return a FunctionalOutput (got %s)" % type(o).__name__)
+    r = FunctionalOutput(name=o.name)
+    for k, v in o:
+        if k in ("potential", "energydensity", "v2rho2", "v3rho3", "v4rho4") and v is not None:
+            t = v if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v))
+            with torch._C.DisableTorchFunctionSubclass():
+                t = t.to(device=rho.device, dtype=rho.dtype).reshape(rho.shape)
+            v = DirectField(grid=rho.grid, data=t, dtype=rho.dtype)
+        elif k == "energy":
+            v = float(v)
+        setattr(r, k, v)
+    return r
+
+
 class TotalFunctional(AbstractFunctional):
     """Sum of functionals plus the ion-ion constant (total_functional.py:5-174)."""
 
@@ -554,6 +582,14 @@ class TotalFunctional(AbstractFunctional):
             return None
         return self._ewald
 
+    def plugin_terms(self):
+        """(key, functional) of the terms without a device implementation:
+        any AbstractFunctional the caller wrote (the reference's plugin
+        contract, total_functional.py:69-77).  They are evaluated by their
+        own compute() on the DirectField and summed with the fused device
+        terms."""
+        return [(k, f) for k, f in self.funcDict.items() if f._device_terms() is None]
+
     def _merged_terms(self):
         t = dict(tf=False, vw=False, wt=False, lda=False, hartree=False, pseudo=False, x=1.0, y=1.0,
                  alpha=5.0 / 6.0, beta=5.0 / 6.0, rho0_fixed=None, pseudo_obj=None, gga=None, lwt=None)
@@ -561,7 +597,7 @@ class TotalFunctional(AbstractFunctional):
         for key, f in self.funcDict.items():
             d = f._device_terms()
             if d is None:
-                raise TypeError("%s (%s) has no device implementation" % (key, type(f).__name__))
+                continue  # a plugin term (plugin_terms)
             for flag in ("tf", "vw", "wt", "lda", "hartree", "pseudo"):
                 if d.get(flag):
                     if flag in seen:
@@ -581,13 +617,19 @@ class TotalFunctional(AbstractFunctional):
                 t["pseudo_obj"] = d["obj"]
         return t
 
-    def evaluator(self, grid, dtype=torch.float64):
+    def has_device_terms(self):
+        return any(f._device_terms() is not None for f in self.funcDict.values())
+
+    def evaluator(self, grid, dtype=torch.float64, only=None):
+        """The fused device evaluation of this sum's device terms on ``grid``
+        (``only``: one term alone, by its switch name: tf, vw, wt, lda,
+        hartree, pseudo, gga, lwt)."""
         # a restarted pseudopotential (new ions) needs a new evaluator
         ver = tuple(getattr(f, "_version", 0) for f in self.funcDict.values())
-        key = (tuple(int(x) for x in grid.nrR), grid.lattice.tobytes(), bool(grid.full), dtype, ver)
+        key = (tuple(int(x) for x in grid.nrR), grid.lattice.tobytes(), bool(grid.full), dtype, ver, only)
         ev = self._evaluators.get(key)
         if ev is None:
-            ev = DeviceEvaluator(self, grid, dtype)
+            ev = DeviceEvaluator(self, grid, dtype, only=only)
             self._evaluators[key] = ev
         return ev
 
@@ -598,29 +640,108 @@ class TotalFunctional(AbstractFunctional):
         return self.compute(rho, calcType=("E",)).energy
 
     def get_energy_potential(self, rho, calcType=("E", "V"), split=False, **kwargs):
+        """Sum of every term (total_functional.py:115-146).  The device terms
+        run fused (one evaluation for all of them); with ``split`` and a
+        potential or energy density requested, each device term runs on its
+        own so that its FunctionalOutput carries its own potential, as the
+        reference returns them.  Plugin terms are called with the same
+        calcType and keyword arguments.  calcType: 'E', 'V' and 'D' (the
+        energy density of the kinetic terms: TF C_TF rho^(5/3), vW V_vW rho,
+        WT V_NL rho / (2 alpha), kedf/tf.py:66-75, vw.py:125-126,
+        wt.py:123-129)."""
         if not isinstance(rho, DirectField):
             raise TypeError("rho must be a DirectField")
         if rho.dim() != 3:
             raise NotImplementedError("spin-polarised densities are not on the device path")
-        ev = self.evaluator(rho.grid, rho.dtype)
-        V = None
-        if "V" in calcType or split:
-            V = DirectField(grid=rho.grid, dtype=rho.dtype)
-        with torch._C.DisableTorchFunctionSubclass():
-            sums = ev.eval_rho(rho.contiguous(), V)
-        parts = ev.energies(sums)
+        calc = set(calcType)
+        plugins = self.plugin_terms()
+        device = self.has_device_terms()
+        if device:
+            bad = calc - {"E", "V", "D"}
+            if bad:
+                raise AttributeError("calcType %s is not on the device path (E, V and D are)" % sorted(bad))
+        parts = {}
+        if device:
+            parts = self._device_parts(rho, calc, split)
+        for key, f in plugins:
+            if split and f.type == "KEDF":
+                res = f(rho, calcType=calcType, split=True, **kwargs)
+                for k2, o in res.items():
+                    k = "KEDF-" + k2.split("-")[-1]
+                    parts[k] = _on_device(o, rho) + parts[k] if k in parts else _on_device(o, rho)
+            else:
+                o = _on_device(f(rho, calcType=calcType, **kwargs), rho)
+                k = f.type if split else key
+                parts[k] = o + parts[k] if k in parts else o
         ew = self.ewald
-        ii = ew.energy if ew else None
-        total = FunctionalOutput(name="TOTAL", energy=float(sum(parts.values())) + (ii or 0.0))
-        if "V" in calcType:
-            total.potential = V
+        total = FunctionalOutput(name="TOTAL", energy=0.0)
+        for o in parts.values():
+            total = total + o
+        if ew:
+            ii = FunctionalOutput(name="II", energy=ew.energy)
+            total = total + ii
+        total.name = "TOTAL"
         if not split:
             return total
         out = {"TOTAL": total}
-        for k, e in parts.items():
-            out[k] = FunctionalOutput(name=k, energy=e)
-        if ii is not None:
-            out["II"] = FunctionalOutput(name="II", energy=ii)
+        out.update(parts)
+        if ew:
+            out["II"] = ii
+        return out
+
+    def _device_parts(self, rho, calc, split):
+        """FunctionalOutputs of the device terms: {"DEVICE": sum} when not
+        split and no energy density is asked, else one per term keyed like the
+        reference's split output."""
+        grid = rho.grid
+        ev = self.evaluator(grid, rho.dtype)
+        per_term = ("V" in calc and split) or "D" in calc
+        if not per_term:
+            V = DirectField(grid=grid, dtype=rho.dtype) if "V" in calc else None
+            with torch._C.DisableTorchFunctionSubclass():
+                sums = ev.eval_rho(rho.contiguous(), V)
+            energies = ev.energies(sums)
+            if not split:
+                o = FunctionalOutput(name="DEVICE", energy=float(sum(energies.values())))
+                if V is not None:
+                    o.potential = V
+                return {"DEVICE": o}
+            return {k: FunctionalOutput(name=k, energy=e) for k, e in energies.items()}
+        t = ev.terms
+        if "D" in calc and (t["gga"] is not None or t["lwt"] is not None):
+            raise AttributeError("the energy density of the GGA / LWT kinetic terms is not on the device path")
+        out = {}
+        for key, idx, flag in _SPLIT_KEYS + [("KEDF-GGA", None, "gga"), ("KEDF-NL", None, "lwt")]:
+            active = (t[flag] is not None) if flag in ("gga", "lwt") else bool(t[flag])
+            if not active:
+                continue
+            one = self.evaluator(grid, rho.dtype, only=flag)
+            V = DirectField(grid=grid, dtype=rho.dtype)
+            with torch._C.DisableTorchFunctionSubclass():
+                sums = one.eval_rho(rho.contiguous(), V)
+            e = float(sum(one.energies(sums).values()))
+            o = FunctionalOutput(name=key, energy=e)
+            if "V" in calc:
+                o.potential = V
+            if "D" in calc and key.startswith("KEDF-"):
+                with torch._C.DisableTorchFunctionSubclass():
+                    r = rho.contiguous()
+                    if flag == "tf":
+                        # C_TF cbrt(rho^5), not scaled by x (kedf/tf.py:66-75)
+                        ctf = 0.3 * (3.0 * np.pi ** 2) ** (2.0 / 3.0)
+                        ed = ctf * r * torch.pow(r * r, 1.0 / 3.0)
+                    elif flag == "vw":
+                        ed = V * r
+                    else:
+                        ed = V * r / (2.0 * t["alpha"])
+                o.energydensity = DirectField(grid=grid, data=ed, dtype=rho.dtype)
+            out[key] = o
+        if not split:
+            tot = None
+            for o in out.values():
+                tot = o if tot is None else tot + o
+            tot.name = "DEVICE"
+            return {"DEVICE": tot}
         return out
 
     def get_forces(self, rho, ions=None, split=False, **kwargs):

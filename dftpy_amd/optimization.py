@@ -30,13 +30,111 @@ import time
 
 import numpy as np
 import torch
-from scipy.optimize._dcsrch import DCSRCH
 
 from . import _lib as L
+from . import linesearch as LS
 from .field import DirectField
-from .functional import TotalFunctional, FunctionalOutput
+from .functional import TotalFunctional, FunctionalOutput, AbstractFunctional
 
 HARTREE_EV = 27.21138602436724
+
+
+class OptimizationError(ValueError):
+    """A non-finite energy, residual or curvature: the solve cannot go on."""
+
+
+def _finite(x, what):
+    if not math.isfinite(x):
+        raise OptimizationError("non-finite %s (%r): the density or the functional produced NaN/Inf" % (what, x))
+    return x
+
+
+class HostTermsEvaluator:
+    """Evaluation for an optimizer whose energy is not entirely made of this
+    package's device terms: a TotalFunctional that holds plugin terms (any
+    AbstractFunctional, total_functional.py:69-77), or any callable returning
+    an object with ``.energy`` and ``.potential`` (the reference's
+    EnergyEvaluator contract, optimization.py:83-86, :251, :275).
+
+    The device terms of a TotalFunctional still run fused; each plugin term
+    (or the callable) is called on DirectFields of the same grid, dtype and
+    device with calcType {'E', 'V'} and the keyword arguments ``phi`` and
+    ``lphi`` the reference passes, and its energy and potential are added.
+    Every field operation around the calls (the trial field phi cos t +
+    p sin t, its square, the sums for the chemical potential and the
+    line-search derivative) is a device kernel of the plan.  Mirrors the
+    DeviceEvaluator interface that the optimizer drives."""
+
+    def __init__(self, evaluator, grid, dtype):
+        from .plan import Plan
+        self.grid = grid
+        self.dtype = dtype
+        if isinstance(evaluator, TotalFunctional):
+            self.total = evaluator
+            self.calls = evaluator.plugin_terms()
+            self.dev = evaluator.evaluator(grid, dtype) if evaluator.has_device_terms() else None
+        else:
+            self.total = None
+            self.calls = [("EVALUATOR", evaluator)]
+            self.dev = None
+        if self.dev is not None:
+            self.P = self.dev.P
+            self.terms = self.dev.terms
+        else:
+            self.P = Plan(tuple(int(x) for x in grid.nrR), grid.lattice,
+                          precision="fp64" if dtype == torch.float64 else "fp32", full=grid.full,
+                          device=grid.get_device())
+            self.terms = dict(tf=False, vw=False, wt=False, lda=False, hartree=False, pseudo=False, gga=None,
+                              lwt=None, rho0_fixed=None)
+        self._tmp = None
+
+    @property
+    def configured_rho0(self):
+        return self.dev.configured_rho0 if self.dev is not None else None
+
+    def ensure_kernel(self, rho0):
+        return self.dev.ensure_kernel(rho0) if self.dev is not None else False
+
+    def energies(self, sums):
+        out = self.dev.energies(sums[:-1]) if self.dev is not None else {}
+        out["HOST"] = float(sums[-1])
+        return out
+
+    def eval_phi(self, phi, p, c, s, V, aux):
+        P = self.P
+        if self.dev is not None:
+            sums = self.dev.eval_phi(phi, p, c, s, V, aux)
+        else:
+            sums = np.zeros(L.NSUM)
+            V.zero_()
+        if self._tmp is None:
+            self._tmp = (P.empty(), P.empty())
+        newphi, p2 = self._tmp
+        if p is None:
+            newphi.copy_(phi)
+        else:
+            P.lincomb(c, phi, s, p, newphi)
+        with torch._C.DisableTorchFunctionSubclass():
+            rho = newphi * newphi
+        rf = DirectField(grid=self.grid, data=rho, dtype=self.dtype)
+        pf = DirectField(grid=self.grid, data=newphi, dtype=self.dtype)
+        e_host = 0.0
+        dV = P.dV
+        for key, f in self.calls:
+            out = f(rf, calcType={"E", "V"}, phi=pf, lphi=False)
+            e_host += _finite(float(out.energy), "energy of %s" % key)
+            pot = out.potential
+            pot = pot if isinstance(pot, torch.Tensor) else torch.as_tensor(np.asarray(pot))
+            with torch._C.DisableTorchFunctionSubclass():
+                pot = pot.to(device=V.device, dtype=V.dtype).reshape(V.shape).contiguous()
+                V.add_(pot)
+            sums[L.S_VRHO] += P.dot3(pot, rho)[0] * dV
+            if p is not None:
+                P.lincomb(c, p, -s, phi, p2)
+                sums[L.S_DTHETA] += 2.0 * P.dot_vphip(pot, newphi, p2) * dV
+        if self.dev is None:
+            sums[L.S_RHO] = P.dot3(newphi, newphi)[1] * dV
+        return np.concatenate([sums, [e_host]])
 
 
 class _Work:
@@ -74,16 +172,38 @@ class Optimization:
             self.optimization_options.setdefault(k, v)
         if EnergyEvaluator is None:
             raise AttributeError("Must provide an energy evaluator")
-        if not isinstance(EnergyEvaluator, TotalFunctional):
-            raise TypeError("the device optimizer needs a dftpy_amd TotalFunctional as EnergyEvaluator")
+        if not callable(EnergyEvaluator):
+            raise TypeError("EnergyEvaluator must be callable: evaluator(rho, calcType=..., phi=..., lphi=...) "
+                            "returning an object with .energy and .potential")
         if self.optimization_options["algorithm"] != "EMM" or \
                 self.optimization_options["vector"] != "Orthogonalization":
             raise AttributeError("only algorithm='EMM' with vector='Orthogonalization' is on the device path")
         self.EnergyEvaluator = EnergyEvaluator
-        terms = EnergyEvaluator._merged_terms()
-        if (terms["gga"] is not None or terms["lwt"] is not None) and self.optimization_options["hvp"] == "exact":
+        # host_terms: energy terms evaluated through their own compute() (a
+        # plugin AbstractFunctional, or an evaluator that is not a
+        # TotalFunctional); their second derivative is unknown
+        self.host_terms = not isinstance(EnergyEvaluator, TotalFunctional) or bool(EnergyEvaluator.plugin_terms())
+        if isinstance(EnergyEvaluator, TotalFunctional):
+            terms = EnergyEvaluator._merged_terms()
+            no_exact = terms["gga"] is not None or terms["lwt"] is not None or self.host_terms
+            if (terms["gga"] is not None or terms["lwt"] is not None) and \
+                    self.optimization_options["precision"] in ("fp32", "mixed"):
+                # their potentials carry grad rho / rho^(4/3)-type fields that
+                # grow without bound where rho -> 0 (trial rotations reach
+                # such points); in FP32 the transforms' rounding of those
+                # fields swamps the potential
+                raise ValueError("GGA / LWT kinetic functionals run in FP64 on the device path "
+                                 "(precision=%r)" % self.optimization_options["precision"])
+        else:
+            no_exact = True
+            prec = self.optimization_options["precision"]
+            if prec not in (None, "fp64"):
+                raise ValueError("an EnergyEvaluator that is not a TotalFunctional runs in FP64 (precision=%r)" % prec)
+            self.optimization_options["precision"] = "fp64"
+        if no_exact and self.optimization_options["hvp"] == "exact":
             # no closed-form second derivative of the GGA / LWT kinetic terms
-            # on the device: the reference's forward-difference product
+            # or of plugin terms on the device: the reference's
+            # forward-difference product
             self.optimization_options["hvp"] = "fd"
         self.optimization_method = optimization_method
         if optimization_method not in ("TN", "LBFGS", "GD") and not optimization_method.startswith("CG"):
@@ -94,6 +214,7 @@ class Optimization:
         self.nhvp = 0
         self._tn_chunk = 4
         self._tn_hist = []
+        self._host_evs = {}
         self.verbose = True
 
     # ------------------------------------------------------------------
@@ -107,6 +228,8 @@ class Optimization:
         return W.ev.eval_phi(phi, p, c, s, W.Vt, W.auxt)
 
     def _ii(self):
+        if not isinstance(self.EnergyEvaluator, TotalFunctional):
+            return 0.0  # the evaluator's energy is the whole energy
         ew = self.EnergyEvaluator.ewald
         return ew.energy if ew else 0.0
 
@@ -130,6 +253,21 @@ class Optimization:
             nd = self._tn_loop(Wi, mu)
             W.P.convert_from(Wi.tn()["d"], W.p)
             return nd
+        if self.optimization_options["hvp"] != "exact" and W.P.dtype != torch.float64:
+            # the forward difference (eps = 1e-9) is below FP32 resolution:
+            # the product runs on FP64 copies of phi and the residual
+            # (the residual and mu are re-evaluated in FP64 at that phi: the
+            # difference quotient subtracts them from an FP64 potential)
+            W64 = self._work("fp64")
+            W64.P.bind_stream(True)
+            W64.P.convert_from(W.phi, W64.phi)
+            sums = self._eval(W64, W64.phi, None, 0.0)
+            W64.V, W64.Vt = W64.Vt, W64.V
+            mu64 = sums[L.S_VRHO] / sums[L.S_RHO]
+            W64.P.residual(W64.V, W64.phi, mu64, W64.res)
+            nd = self._tn_loop(W64, mu64)
+            W.P.convert_from(W64.tn()["d"], W.p)
+            return nd
         nd = self._tn_loop(W, mu)
         W.p.copy_(W.tn()["d"])
         return nd
@@ -147,6 +285,7 @@ class Optimization:
             # the device queues 2 more per extra synchronisation after that
             self._tn_hist = (self._tn_hist + [n])[-3:]
             self._tn_chunk = max(2, min(self._tn_hist) - 1)
+            _finite(pAp, "p.Ap in the truncated-Newton loop")
             if stat == L.TN_FAILED:
                 self._print("!WARN : pAp small than zero :iter = ", n - 1, pAp)
             return n
@@ -167,14 +306,23 @@ class Optimization:
             P.lincomb(1.0, W.phi, eps, cgp, tmp)
             P.residual(Ap, tmp, mu, Ap)
             P.lincomb(1.0 / eps, Ap, -1.0 / eps, W.res, Ap)
-            pAp = P.dot3(cgp, Ap)[0]
+            pAp = _finite(P.dot3(cgp, Ap)[0], "p.Ap in the truncated-Newton loop")
             if pAp > 0.0:
-                r1 = P.cg_update(r0 / pAp, cgp, Ap, d, r)
+                r1 = _finite(P.cg_update(r0 / pAp, cgp, Ap, d, r), "CG residual norm")
             if pAp < 0.0:
                 if it == 0:
                     P.lincomb(r0 / pAp, cgp, 0.0, d, d)
                 else:
                     self._print("!WARN : pAp small than zero :iter = ", it, pAp)
+                break
+            if pAp == 0.0:
+                # no curvature information along p: the reference's r0 / pAp
+                # would be infinite.  First step: the steepest-descent
+                # direction p = -res (the orthonormalisation rescales it);
+                # later: keep the direction built so far.
+                if it == 0:
+                    d.copy_(cgp)
+                self._print("!WARN : pAp equal to zero :iter = ", it)
                 break
             if r1 < min(rl):
                 best.copy_(d)
@@ -275,8 +423,11 @@ class Optimization:
             return float(cache["sums"][L.S_DTHETA])
 
         a1 = theta if o["ls_first_step"] == "theta" else 1.0
-        ls = DCSRCH(phi_, dphi_, o["c1"], o["c2"], o["xtol"], 0.0, np.pi)
-        stp, _, _, task = ls(a1, phi0=E0, derphi0=g0, maxiter=maxiter)
+        if not 0 < o["c1"] < o["c2"] < 1:
+            # scipy's scalar_search_wolfe1 check (the reference's call path)
+            raise ValueError("'c1' and 'c2' do not satisfy '0 < c1 < c2 < 1'.")
+        stp, task = LS.search(phi_, dphi_, E0, g0, a1, c1=o["c1"], c2=o["c2"], xtol=o["xtol"], stpmin=0.0,
+                              stpmax=np.pi, maxiter=maxiter)
         return stp, count[0], cache
 
     # --- precision switching ------------------------------------------
@@ -284,7 +435,13 @@ class Optimization:
         grid = self.rho.grid
         dt = torch.float64 if prec == "fp64" else torch.float32
         # the evaluator changes with the grid and with restarted ions
-        ev = self.EnergyEvaluator.evaluator(grid, dt)
+        if self.host_terms:
+            hk = (prec, tuple(grid.nrR), grid.lattice.tobytes())
+            if hk not in self._host_evs:
+                self._host_evs[hk] = HostTermsEvaluator(self.EnergyEvaluator, grid, dt)
+            ev = self._host_evs[hk]
+        else:
+            ev = self.EnergyEvaluator.evaluator(grid, dt)
         key = (prec, id(ev))
         if key not in self._works:
             self._works = {k: w for k, w in self._works.items() if k[0] != prec}
@@ -349,6 +506,8 @@ class Optimization:
             # electron count fixed in FP64 from the initial phi (optimization.py:305-315)
             self._nelec = float((phi * phi).sum()) * W.P.dV
         E, mu, resN, sums = self._start(W)
+        _finite(E, "energy")
+        _finite(resN, "residual norm")
         hist = [E]
         self.log = [dict(step=0, energy=E, dE=E, dP=resN, Nd=1, Nls=1, time=time.perf_counter() - t0, prec=cur)]
         self._print("{:8s}{:24s}{:16s}{:16s}{:8s}{:8s}{:16s}".format("Step", "Energy(a.u.)", "dE", "dP", "Nd", "Nls",
@@ -400,10 +559,10 @@ class Optimization:
                 W.P.lincomb(math.cos(t), W.phi, math.sin(t), W.p, W.phi)
                 W.V, W.Vt = W.Vt, W.V
                 W.aux, W.auxt = W.auxt, W.aux
-                Enew = self._energy(W, sums)
+                Enew = _finite(self._energy(W, sums), "energy")
                 mu = sums[L.S_VRHO] / sums[L.S_RHO]
                 W.res_prev.copy_(W.res)
-                resN = self._residual(W, mu)
+                resN = _finite(self._residual(W, mu), "residual norm")
                 if m == "LBFGS":
                     self._lbfgs_update(W, old_phi)
                 if m.startswith("CG"):
@@ -430,6 +589,7 @@ class Optimization:
                 W.P.lincomb(math.sqrt(self._nelec / nphi), W.phi, 0.0, W.phi, W.phi)
                 E32 = hist[-1]
                 E, mu, resN, sums = self._start(W)
+                _finite(E, "energy")
                 # the convergence test restarts on FP64 energies
                 hist = [E]
                 theta = 0.1

@@ -75,7 +75,7 @@ HARTREE = 27.21138602436724  # eV per hartree (ase.units.Ha)
 ZERO = 1e-30  # constants.py:85
 
 __all__ = [
-    "BOHR", "HARTREE", "ZERO", "Grid", "Ions", "fft", "ifft",
+    "BOHR", "HARTREE", "ZERO", "Grid", "Ions", "fft", "ifft", "set_fft_workers", "fft_workers", "KernelCache",
     "tf_functional", "vw_functional", "wt_functional", "wt_kernel", "lindhard",
     "lda_functional", "lda_v2rho2", "hartree_functional", "PseudoTable", "read_recpot",
     "empty_core_table", "LocalPseudo", "Ewald", "TotalFunctional",
@@ -201,8 +201,29 @@ class Grid:
         return 2.0 * self.guess_ecut()
 
 
+# Transform threads.  1 (the default) is numpy.fft, single-threaded like the
+# reference (pyfftw threads=1, math_utils.py:114,129); above 1 the same
+# transforms run through scipy.fft with that many workers (CPU timing of the
+# all-cores baseline only).
+FFT_WORKERS = 1
+
+
+def set_fft_workers(n):
+    global FFT_WORKERS
+    FFT_WORKERS = max(1, int(n))
+
+
+def fft_workers():
+    return FFT_WORKERS
+
+
 def fft(f, grid):
     """DirectField.fft: field.py:413-430 (x dV), backend field.py:159-172."""
+    if FFT_WORKERS > 1:
+        import scipy.fft as sf
+        if grid.full:
+            return sf.fftn(f, workers=FFT_WORKERS) * grid.dV
+        return sf.rfftn(f, workers=FFT_WORKERS) * grid.dV
     if grid.full:
         return np.fft.fftn(f) * grid.dV
     return np.fft.rfftn(f) * grid.dV
@@ -210,6 +231,11 @@ def fft(f, grid):
 
 def ifft(F, grid):
     """ReciprocalField.ifft(force_real=True): field.py:779-804 (/ dV)."""
+    if FFT_WORKERS > 1:
+        import scipy.fft as sf
+        if grid.full:
+            return sf.ifftn(F, workers=FFT_WORKERS).real / grid.dV
+        return sf.irfftn(F, s=tuple(grid.nr), axes=(0, 1, 2), workers=FFT_WORKERS) / grid.dV
     if grid.full:
         return np.fft.ifftn(F).real / grid.dV
     return np.fft.irfftn(F, s=tuple(grid.nr), axes=(0, 1, 2)) / grid.dV
