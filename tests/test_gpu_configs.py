@@ -85,3 +85,50 @@ def test_al864_192_fp64_evaluation_and_fp32_error():
     E = sum(parts[k] for k in KEYS)
     print("FP32 total error %.3e eV/atom" % (abs(tot32 - E) * W.HARTREE / nat))
     assert abs(tot32 - E) * W.HARTREE / nat < 1e-5
+
+
+def _golden(name):
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "oracle_converged.json")) as fh:
+        return json.load(fh)[name]
+
+
+def test_al864_192_converged_energy_against_oracle_minimum():
+    """The 864-atom 192^3 solve, mixed precision (FP32 outer phase, FP64
+    refinement) and FP64, against the oracle's converged TN minimum
+    (tests/golden/oracle_converged.json, made by make_converged.py in the
+    build container): within 1e-6 eV/atom.  At the mixed solve's converged
+    density, the FP64 per-term energies and V against the oracle evaluated
+    there (1e-10 relative, 1e-9 Ha), and the FP32 evaluation's error there."""
+    print("case al864_192 converged TN vs oracle minimum", flush=True)
+    gold = _golden("al864_192")
+    wl, grid, tot, og, otot = _pair("al864_192")
+    nat = len(wl["symbols"])
+    res = {}
+    for prec, phase1 in (("mixed", True), ("fp64", False)):
+        opt = D.Optimization(EnergyEvaluator=tot, optimization_method="TN",
+                             optimization_options={"econv": 1e-8, "precision": prec, "mixed_phase1": phase1})
+        opt.optimize_rho(guess_rho=D.DirectField(grid=grid, data=wl["rho_ini"]))
+        err = abs(opt.energy - gold["energy_Ha"]) * W.HARTREE / nat
+        print("%s: E %.10f Ha, oracle minimum %.10f Ha, error %.3e eV/atom" % (prec, opt.energy, gold["energy_Ha"], err))
+        assert opt.converged == 0
+        assert err < 1e-6
+        res[prec] = opt
+    rho = res["mixed"].rho
+    r = rho.as_subclass(torch.Tensor).double().cpu().numpy()
+    E, V, parts = otot.compute(r, split=True)
+    out = tot.get_energy_potential(D.DirectField(grid=grid, data=r), split=True)
+    o32 = tot.get_energy_potential(D.DirectField(grid=grid, data=r, dtype=torch.float32), split=True)
+    for k in KEYS:
+        e32 = abs(o32[k].energy - parts[k]) * W.HARTREE / nat
+        print("%-8s FP64 %.14e oracle %.14e   FP32 error %.3e eV/atom" % (k, out[k].energy, parts[k], e32))
+        assert abs(out[k].energy - parts[k]) <= 1e-10 * max(abs(parts[k]), 1e-3), k
+    full = tot.compute(D.DirectField(grid=grid, data=r))
+    dv = float((full.potential.as_subclass(torch.Tensor) - torch.tensor(V, device="cuda")).abs().max())
+    f32 = tot.compute(D.DirectField(grid=grid, data=r, dtype=torch.float32))
+    dv32 = float((f32.potential.as_subclass(torch.Tensor).double() - torch.tensor(V, device="cuda")).abs().max())
+    print("at the converged density: max|V - V_oracle| FP64 %.3e Ha, FP32 %.3e Ha; total FP64 %.3e, FP32 %.3e eV/atom"
+          % (dv, dv32, abs(full.energy - E) * W.HARTREE / nat, abs(f32.energy - E) * W.HARTREE / nat))
+    assert dv < 1e-9
+    assert abs(full.energy - E) <= 1e-10 * abs(E)

@@ -111,6 +111,35 @@ def test_al4_32_lbfgs_and_cg_reach_tn_minimum():
         assert abs(opt.energy - ref) < 1e-8
 
 
+def test_al4_32_lbfgs_trajectory_matches_oracle():
+    """LBFGS (optimization.py:150-173, math_utils.py:307-325) iteration by
+    iteration against the oracle's restatement, both FP64: the first loop
+    of the two-loop recursion skips index 0 and the memory keeps maxcor + 1
+    pairs in both, so the energies, line-search counts and iteration
+    counts agree."""
+    print("case LBFGS trajectory al4_32", flush=True)
+    wl, og, otot, grid, ev = _al4_pair()
+    opts = {"econv": 1e-10, "maxiter": 400}
+    oopt = O.Optimization(lambda r, c, p: otot.compute(r, c, phi=p), og, "LBFGS", dict(opts))
+    oopt.optimize_rho(wl["rho_ini"])
+    opt = D.Optimization(EnergyEvaluator=ev, optimization_method="LBFGS", optimization_options=dict(opts))
+    opt.optimize_rho(guess_rho=D.DirectField(grid=grid, data=wl["rho_ini"]))
+    print("GPU %d iterations, oracle %d; final %.12f vs %.12f" % (len(opt.log) - 1, len(oopt.log) - 1, opt.energy,
+                                                                 oopt.energy))
+    assert opt.converged == 0 and oopt.converged == 0
+    n = min(len(opt.log), len(oopt.log))
+    for a, b in zip(opt.log[:n], oopt.log[:n]):
+        print("%3d %.12f %.12f %d %d" % (a["step"], a["energy"], b[1], a["Nls"], b[5]))
+    for a, b in zip(opt.log[:12], oopt.log[:12]):
+        assert abs(a["energy"] - b[1]) < 1e-9
+        assert a["Nls"] == b[5]
+    assert abs(len(opt.log) - len(oopt.log)) <= 2
+    assert abs(opt.energy - oopt.energy) < 1e-9
+    # memory of maxcor + 1 pairs (the reference pops only above Bound)
+    assert len(opt._lbfgs["s"]) == min(opts.get("maxcor", 6) + 1, len(opt.log) - 1)
+    assert len(oopt.lb_s) == min(7, len(oopt.log) - 1)
+
+
 @pytest.mark.parametrize("phase1", [False, True])
 def test_al4_32_mixed_precision_within_1e6_ev_per_atom(phase1):
     wl, og, otot, grid, ev = _al4_pair()
