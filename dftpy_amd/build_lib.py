@@ -1,15 +1,20 @@
 """Build recipe for the gfx950 shared library ``dftpy_amd/_lib/libofdft_amd.so``.
 
-``python -m dftpy_amd.build_lib`` compiles with hipcc for gfx950.  The
-library is built in-tree so that it travels with the source tree.
+``python -m dftpy_amd.build_lib`` compiles every ``csrc/*.hip`` translation
+unit with hipcc for gfx950, in parallel (the transform pipelines of each
+family and precision sit in their own unit), and links them.  The library is
+built in-tree so that it travels with the source tree.
 """
+import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", "ofdft.hip")]
+SRC = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
 OUT_DIR = os.path.join(HERE, "_lib")
+OBJ_DIR = os.path.join(OUT_DIR, "obj")
 OUT = os.path.join(OUT_DIR, "libofdft_amd.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -19,17 +24,39 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # packed FP32 instructions are kept out of generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
 
 
+def _jobs():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16, len(SRC)))
+
+
 def build(verbose=True):
-    os.makedirs(OUT_DIR, exist_ok=True)
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    inc = ["-I", os.path.join(HERE, "..", "include")]
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, os.path.basename(src)[:-4] + ".o")
+        cmd = [HIPCC] + FLAGS + inc + ["-c", src, "-o", obj + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed on %s:\n%s" % (src, r.stderr[-4000:]))
+        os.replace(obj + ".tmp", obj)
+        return obj
+
+    with ThreadPoolExecutor(_jobs()) as ex:
+        objs = list(ex.map(compile_one, SRC))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-I", os.path.join(HERE, "..", "include")] + SRC + ["-o", tmp]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
