@@ -105,6 +105,7 @@ _SIGS = {
     "ofdft_local_forces": (_i, [_vp, _vp, _dp]),
     "ofdft_ewald": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
     "ofdft_ewald_forces": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp]),
+    "ofdft_ewald_noalloc": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),
     "ofdft_functional_stress": (_i, [_vp, _vp, _d, _dp]),
     "ofdft_local_stress": (_i, [_vp, _vp, _dp]),
     "ofdft_ewald_stress": (_i, [_vp, _i, _dp, _dp, _d, _d, _i, _i, _dp, _dp]),

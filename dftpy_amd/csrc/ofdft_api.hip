@@ -16,6 +16,50 @@ OFDFT_TU_GGA(extern, double)
 OFDFT_TU_LWT(extern, float)
 OFDFT_TU_LWT(extern, double)
 
+#include <mutex>
+
+// ---------------------------------------------------------------------------
+// Allocation-free Ewald sums (ofdft_ewald_noalloc): the per-call staging of
+// ofdft_ewald (ion positions, charges, fractional coordinates, image shifts)
+// and its PME charge grid and b-moduli live in one static device array per
+// GPU, held until the process ends, instead of buffers allocated per call.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = size_t(8) << 20;  // 8 MiB on every GPU the library runs on
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the scratch array on the GPU of `stream`: looked up once
+// per device (hipGetSymbolAddress with that device current) and kept; in host
+// code the array's name is not a device address.
+int scratch_address(hipStream_t stream, unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipStreamGetDevice(stream, &dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) return fail(OFDFT_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != dev) HIPCHK(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(OFDFT_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
 extern "C" {
 
 const char* ofdft_last_error(void) { return g_err.c_str(); }
@@ -491,6 +535,48 @@ int ofdft_ewald(ofdft_plan P, int nat, const double* positions, const double* ch
   if (pme && ((rc = ensure_work(P)) || (rc = ensure_bm(P, order)))) return rc;
   return ewald_energy_sums(P, EwaldBufs{E.pos, E.q, E.sh, E.frac, E.nshift, P->d_work, P->d_bm}, nat, eta, rmax, pme,
                            order, e_real, e_rec);
+}
+
+int ofdft_ewald_noalloc(ofdft_plan P, int nat, const double* positions, const double* charges, double eta,
+                        double rmax, int pme, int order, double* e_real, double* e_rec) {
+  if (!valid(P) || !positions || !charges || !e_real || !e_rec) return fail(OFDFT_EINVAL, "null argument");
+  int rc;
+  if ((rc = ewald_check(P, nat, eta, rmax, pme, order))) return rc;
+  std::vector<double> sh, frac, bm;
+  ewald_host_arrays(P, nat, positions, rmax, sh, frac);
+  if (pme) bm = make_bm(P, order);
+  // layout: pos | q | frac | shifts | [charge grid | b-moduli], 256-byte aligned
+  const size_t o_pos = 0;
+  const size_t o_q = o_pos + align256(sizeof(double) * 3 * nat);
+  const size_t o_frac = o_q + align256(sizeof(double) * nat);
+  const size_t o_sh = o_frac + align256(sizeof(double) * 3 * nat);
+  const size_t o_work = o_sh + align256(sizeof(double) * sh.size());
+  const size_t o_bm = o_work + (pme ? align256(sizeof(double) * P->N) : 0);
+  const size_t total = o_bm + align256(sizeof(double) * bm.size());
+  if (total > kScratchBytes)
+    return fail(OFDFT_EINVAL, "ofdft_ewald_noalloc needs " + std::to_string(total) + " bytes of scratch (" +
+                                  std::to_string(nat) + " atoms, " + std::to_string(sh.size() / 3) +
+                                  " image shifts" + (pme ? ", PME grid of " + std::to_string(P->N) + " points" : "") +
+                                  "); its static array holds " + std::to_string(kScratchBytes) + " bytes");
+  unsigned char* base = nullptr;
+  if ((rc = scratch_address(P->stream, &base))) return rc;
+  auto* pos = reinterpret_cast<double*>(base + o_pos);
+  auto* q = reinterpret_cast<double*>(base + o_q);
+  auto* fr = reinterpret_cast<double*>(base + o_frac);
+  auto* shd = reinterpret_cast<double*>(base + o_sh);
+  double* work = pme ? reinterpret_cast<double*>(base + o_work) : nullptr;
+  double* bmd = pme ? reinterpret_cast<double*>(base + o_bm) : nullptr;
+  HIPCHK(hipMemcpyAsync(pos, positions, sizeof(double) * 3 * nat, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(hipMemcpyAsync(q, charges, sizeof(double) * nat, hipMemcpyHostToDevice, P->stream));
+  HIPCHK(hipMemcpyAsync(fr, frac.data(), sizeof(double) * frac.size(), hipMemcpyHostToDevice, P->stream));
+  HIPCHK(hipMemcpyAsync(shd, sh.data(), sizeof(double) * sh.size(), hipMemcpyHostToDevice, P->stream));
+  if (pme) HIPCHK(hipMemcpyAsync(bmd, bm.data(), sizeof(double) * bm.size(), hipMemcpyHostToDevice, P->stream));
+  // the sums read their results back on the plan stream; on an error the
+  // stream is drained here, so the host arrays above outlive every copy
+  rc = ewald_energy_sums(P, EwaldBufs{pos, q, shd, fr, (int)(sh.size() / 3), work, bmd}, nat, eta, rmax, pme, order,
+                         e_real, e_rec);
+  if (rc) (void)hipStreamSynchronize(P->stream);
+  return rc;
 }
 
 int ofdft_ewald_forces(ofdft_plan P, int nat, const double* positions, const double* charges, double eta,

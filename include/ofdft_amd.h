@@ -238,6 +238,22 @@ int ofdft_ewald_forces(ofdft_plan plan, int nat, const double* positions, const 
 int ofdft_ewald(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta, double rmax,
                 int pme, int order, double* e_real, double* e_rec);
 
+/* ofdft_ewald_noalloc: ofdft_ewald (same arguments, same kernels, same
+ * results) without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, molecular-dynamics or serving loops, harnesses
+ * that forbid hipMalloc).  The per-call staging (positions, charges,
+ * fractional coordinates, image shifts) and, with pme=1, the charge grid
+ * and b-moduli live in a static 8 MiB device array, one per GPU, held until
+ * the process ends: (7 nat + 3 nshift) doubles plus, with pme=1,
+ * n0 n1 n2 + 2 (n0 + n1 + n2/2 + 1) doubles, each part rounded up to 256
+ * bytes, must fit (about 96^3 with PME); a larger request returns
+ * OFDFT_EINVAL with the sizes.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (several host threads or streams) race.  The plan's own
+ * workspace (spectra, partial sums) was allocated at ofdft_plan_create.     */
+int ofdft_ewald_noalloc(ofdft_plan plan, int nat, const double* positions, const double* charges, double eta,
+                        double rmax, int pme, int order, double* e_real, double* e_rec);
+
 /* Stress (TotalFunctional.get_stress, total_functional.py:176-209).  The
  * six components of each symmetric 3x3 sum are ordered xx xy xz yy yz zz;
  * the callers apply the reference's factors, volumes and energy terms.
