@@ -80,12 +80,12 @@ def build_system(name, D, W):
     return wl, grid, ions, ev
 
 
-def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact"):
+def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact", tn_graph=False):
     """One TN solve from ``rho`` (a DirectField already resident on the GPU;
     optimize_rho does not modify its guess)."""
     opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
                          optimization_options={"econv": econv, "precision": precision, "maxiter": 200,
-                                               "mixed_phase1": phase1, "hvp": hvp})
+                                               "mixed_phase1": phase1, "hvp": hvp, "tn_graph": tn_graph})
     opt.verbose = verbose
     opt.optimize_rho(guess_rho=rho)
     niter = sum(1 for l in opt.log[1:] if l["prec"] != "switch->fp64")
@@ -246,6 +246,20 @@ def main():
                             round(v[2] / v[1] / (v[0] / v[1] / 1e3) / 1e9, 0)] for k, v in pr.items() if v[1]}
                     for p, pr in breakdown.items()},
     }
+    # the same solves with the device TN loop replaying one captured hipGraph
+    # per CG iteration (ofdft_plan_set_graph) instead of its kernel launches
+    solve(D, ev, rho_ini, "mixed", args.econv, False, args.phase1, tn_graph=True)  # capture outside the timing
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    giters, genergy = 0, None
+    for _ in range(args.steps):
+        og, ng = solve(D, ev, rho_ini, "mixed", args.econv, False, args.phase1, tn_graph=True)
+        giters += ng
+        genergy = og.energy
+    torch.cuda.synchronize()
+    tg = time.perf_counter() - tg
+    out["tn_graph"] = {"value": giters / tg, "unit": "TN iterations/s", "ms_per_step": tg / args.steps * 1e3,
+                       "same_energy": bool(genergy == energies[-1])}
     if not args.no_fp64:
         torch.cuda.synchronize()
         t1 = time.perf_counter()

@@ -114,6 +114,8 @@ _SIGS = {
     "ofdft_lwt_range": (_i, [_vp, _vp, _vp, _vp, _d, _d, _dp]),
     "ofdft_lwt": (_i, [_vp, ctypes.POINTER(LwtParams), _dp, _i, _vp, _vp, _vp, _d, _d, _vp, _dp]),
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
+    "ofdft_plan_set_graph": (_i, [_vp, _i]),
+    "ofdft_plan_set_tiles": (_i, [_vp, _i, _i]),
     "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),
     "ofdft_version": (_i, []),

@@ -42,7 +42,9 @@ def solve_cell(cell, precision="mixed", econv=1e-8, options=None):
     grid, total = build_cell(cell)
     total.ewald.energy
     rho = DirectField(grid=grid, data=cell["rho_ini"])
-    opts = {"econv": econv, "precision": precision, "maxiter": 200}
+    # launch mode for the TN loop: a graph capture on one worker's stream
+    # would make the other workers' synchronous copies (ion set-up) fail
+    opts = {"econv": econv, "precision": precision, "maxiter": 200, "tn_graph": False}
     if options:
         opts.update(options)
     opt = Optimization(EnergyEvaluator=total, optimization_method="TN", optimization_options=opts)

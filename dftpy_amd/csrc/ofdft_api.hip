@@ -94,12 +94,18 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
       return fail(OFDFT_EINVAL, "grid sizes must be products of 2, 3, 5, 7, 11 and 13 (at most 12 passes)");
     }
   }
-  // LDS limits (64 KiB per workgroup)
+  // LDS of the runtime-schedule kernels: a row workgroup holds (3 PB + 1)
+  // rows of n2 points, a column workgroup 16 columns of n0 (or n1) points;
+  // both must fit gfx950's 160 KiB per workgroup.  FP64: n0, n1 <= 601 and
+  // n2 <= 1462; FP32: n0, n1 <= 1203 and n2 <= 2925.
   const size_t cs = 2 * P->rs;
-  if ((size_t)((precision == OFDFT_F64 ? kRowPB<double> : kRowPB<float>) * 3 + 1) * n2 * cs > 65536 || (size_t)(kColTile * (n0 + 1) + n0) * cs > 65536 ||
-      (size_t)(kColTile * (n1 + 1) + n1) * cs > 65536) {
+  const size_t rows_lds = (size_t)((precision == OFDFT_F64 ? kRowPB<double> : kRowPB<float>) * 3 + 1) * n2 * cs;
+  const size_t col0_lds = (size_t)(kColTile * (n0 + 1) + n0) * cs, col1_lds = (size_t)(kColTile * (n1 + 1) + n1) * cs;
+  if (rows_lds > kMaxLdsBytes || col0_lds > kMaxLdsBytes || col1_lds > kMaxLdsBytes) {
     delete P;
-    return fail(OFDFT_EINVAL, "grid dimension too large for the LDS tile sizes");
+    return fail(OFDFT_EINVAL, std::string("grid dimension too large for the LDS tile sizes (") +
+                                  (precision == OFDFT_F64 ? "FP64: n0, n1 <= 601, n2 <= 1462" :
+                                                            "FP32: n0, n1 <= 1203, n2 <= 2925") + ")");
   }
   std::memcpy(P->lat, lattice9, sizeof(double) * 9);
   // bg = 2 pi inv(L)^T (grid.py:409-410): rows of bg are reciprocal vectors
@@ -190,7 +196,30 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->hred) (void)hipHostFree(P->hred);
   if (P->tnst) (void)hipFree(P->tnst);
   if (P->htnst) (void)hipHostFree(P->htnst);
+  if (P->tn_exec) (void)hipGraphExecDestroy(P->tn_exec);
+  if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
+  for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
+  return 0;
+}
+
+int ofdft_plan_set_tiles(ofdft_plan P, int tile_axis, int tile_pencil) {
+  if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
+  auto ok = [&](int w) { return w == 8 || w == 16 || (w == 32 && P->prec == OFDFT_F32); };
+  if (!ok(tile_axis) || !ok(tile_pencil))
+    return fail(OFDFT_EINVAL, "tile widths must be 8 or 16 (FP64), 8, 16 or 32 (FP32)");
+  P->tile_axis = tile_axis;
+  P->tile_pencil = tile_pencil;
+  if (P->tn_exec) {  // a captured CG iteration holds the old launch shapes
+    (void)hipGraphExecDestroy(P->tn_exec);
+    P->tn_exec = nullptr;
+  }
+  return 0;
+}
+
+int ofdft_plan_set_graph(ofdft_plan P, int on) {
+  if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
+  P->use_graph = on ? 1 : 0;
   return 0;
 }
 

@@ -62,6 +62,7 @@ int fail(int code, const std::string& msg) {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kZero = 1e-30;  // constants.py:85
 constexpr int kRedThreads = 1024;
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS a gfx950 workgroup may allocate
 
 // ---------------------------------------------------------------------------
 // host-side pass schedule and twiddles
@@ -181,6 +182,17 @@ struct ofdft_plan_s {
   double prof_ms[8];
   long long prof_count[8];
   double prof_bytes[8];
+  // hipGraph of one device CG iteration of ofdft_tn_direction
+  // (ofdft_plan_set_graph): replayed once per queued iteration in place of
+  // its launches; rebuilt when the iteration's buffers change
+  // columns per workgroup of the register column kernels (axis-1 pass and
+  // axis-0 pencils): 8, 16 or 32 (FP32), 8 or 16 (FP64); chosen per grid
+  // length and precision by the autotuner (ofdft_plan_set_tiles)
+  int tile_axis = 16, tile_pencil = 16;
+  int use_graph = 0;
+  hipGraphExec_t tn_exec = nullptr;
+  const void* tn_key[8] = {};
+  hipStream_t cap_stream = nullptr;  // capture happens here (the null stream cannot capture)
 };
 
 namespace {
@@ -1448,6 +1460,22 @@ __global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __res
 }
 
 
+// LDS stride of one column's N1 x (N2 + 1) block in the register column
+// kernels.  gfx950's LDS serves 256 bytes per cycle from 64 four-byte banks:
+// 32 lanes of 8-byte (FP32 complex) or 16 lanes of 16-byte (FP64 complex)
+// accesses.  Thread (w, q) of a W-column tile writes element w*S + k1*(N2+1)
+// + q and reads w*S + q*(N2+1) + n2; with S = lanes / W (mod lanes) the
+// lanes served together (consecutive w, consecutive q) fall on distinct banks
+// in both transposes (N2 + 1 is odd for every split used).
+template <typename T, int N1, int N2, int W>
+constexpr int col_block_stride() {
+  constexpr int lanes = 256 / (2 * (int)sizeof(T));
+  constexpr int want = W >= lanes ? 1 : lanes / W;
+  int s = N1 * (N2 + 1);
+  while (s % lanes != want % lanes) ++s;
+  return s;
+}
+
 // Four-step register transform of W columns along a strided axis (one
 // workgroup, W*N2 threads): x -> X in place, natural order in and out.
 template <typename T, int N1, int N2, int W>
@@ -1455,7 +1483,8 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec,
                                                      long long ostride, int ncol, const cplx<T>* __restrict__ tw,
                                                      int inv, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
-  __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
+  constexpr int S = col_block_stride<T, N1, N2, W>();
+  __shared__ cplx<T> lds[W * S];
   const int t = threadIdx.x, w = t % W, q = t / W;
   cplx<T>* base = spec + (long long)blockIdx.z * Nh + (long long)blockIdx.y * ostride;
   const int c = blockIdx.x * W + w;
@@ -1473,11 +1502,11 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec,
     v[k1] = cmul(v[k1], wv);
   }
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) lds[(w * N1 + k1) * (N2 + 1) + q] = v[k1];
+  for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
   __syncthreads();
   if (q < N1) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
     dftreg<T, N2>(v, iv);
     if (ok) {
 #pragma unroll
@@ -1494,7 +1523,8 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
                                                        const T* __restrict__ wtk, const int* __restrict__ skip,
                                                        double* __restrict__ cgp) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
-  __shared__ cplx<T> lds[W * N1 * (N2 + 1)];
+  constexpr int S = col_block_stride<T, N1, N2, W>();
+  __shared__ cplx<T> lds[W * S];
   const int t = threadIdx.x, w = t % W, q = t / W;
   const int i1 = blockIdx.y;
   const int f = blockIdx.z;
@@ -1519,14 +1549,14 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
 #pragma unroll
   for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) lds[(w * N1 + k1) * (N2 + 1) + q] = v[k1];
+  for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
   __syncthreads();
   // sum over this thread's points of m |X|^2: the field's share of p.Ap in
   // the TN inner loop (Hermitian weight 2 off the c = 0 and Nyquist planes)
   double pap = 0.0;
   if (q < N1) {
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[(w * N1 + q) * (N2 + 1) + n2];
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
     dftreg<T, N2>(v, false);
     const T scale = T(ck.scale);
     if constexpr (sizeof(T) == 4) {
@@ -1573,11 +1603,11 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
     }
     // this thread re-reads nothing it overwrites: row (w, q) is private to it
 #pragma unroll
-    for (int n2 = 0; n2 < N2; ++n2) lds[(w * N1 + q) * (N2 + 1) + n2] = v[n2];
+    for (int n2 = 0; n2 < N2; ++n2) lds[w * S + q * (N2 + 1) + n2] = v[n2];
   }
   __syncthreads();
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[(w * N1 + k1) * (N2 + 1) + q];
+  for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[w * S + k1 * (N2 + 1) + q];
   dftreg<T, N1>(v, true);
   if (ok) {
 #pragma unroll
@@ -2203,6 +2233,14 @@ void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid
                      P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
+template <typename T, int N1, int N2, int W>
+void axis_reg(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, long long stride, long long ostride, unsigned outer,
+              bool inv) {
+  OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W>), dim3((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F),
+               dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nhp,
+               reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
+}
+
 template <typename T>
 int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   const int W = kColTileC;
@@ -2224,10 +2262,10 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   switch (n) {
 #define OFDFT_CASE(NN)                                                                                       \
     case NN:                                                                                                 \
-      OFDFT_LAUNCH((k_axis_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                             \
-                         dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), outer, (unsigned)F),    \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P, spec, P->Nhp, stride, ostride, P->nhp, \
-                         reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);               \
+      if (P->tile_axis == 8) axis_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, spec, axis, F, stride, ostride, outer, inv); \
+      else if (sizeof(T) == 4 && P->tile_axis == 32)                                                           \
+        axis_reg<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16)>(P, spec, axis, F, stride, ostride, outer, inv); \
+      else axis_reg<T, Split<NN>::N1, Split<NN>::N2, 16>(P, spec, axis, F, stride, ostride, outer, inv);     \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -2253,6 +2291,13 @@ void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t l
                      reinterpret_cast<const T*>(P->wtk), P->skip);
 }
 
+template <typename T, int N1, int N2, int W>
+void pencil_reg(ofdft_plan_s* P, int F, const GridGeom& g, const ConvKinds& ck, double* cgp) {
+  OFDFT_LAUNCH((k_pencil_reg<T, N1, N2, W>), dim3((unsigned)((P->nhp + W - 1) / W), (unsigned)P->n[1], (unsigned)F),
+               dim3(W * N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), P->Nhp, g,
+               reinterpret_cast<const cplx<T>*>(P->tw[0]), ck, reinterpret_cast<const T*>(P->wtk), P->skip, cgp);
+}
+
 template <typename T>
 int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullptr, const double* coef = nullptr) {
   const int W = kColTileC;
@@ -2272,12 +2317,10 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
   switch (P->n[0]) {
 #define OFDFT_CASE(NN)                                                                                   \
     case NN:                                                                                             \
-      OFDFT_LAUNCH((k_pencil_reg<T, Split<NN>::N1, Split<NN>::N2, RegTile<T>::W>),                       \
-                         dim3((unsigned)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W), (unsigned)P->n[1], \
-                              (unsigned)F),                                                              \
-                         dim3(RegTile<T>::W * Split<NN>::N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), \
-                         P->Nhp, g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,                      \
-                         reinterpret_cast<const T*>(P->wtk), P->skip, cgp);                              \
+      if (P->tile_pencil == 8) pencil_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, g, ck, cgp);        \
+      else if (sizeof(T) == 4 && P->tile_pencil == 32)                                                   \
+        pencil_reg<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16)>(P, F, g, ck, cgp);      \
+      else pencil_reg<T, Split<NN>::N1, Split<NN>::N2, 16>(P, F, g, ck, cgp);                           \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -2448,7 +2491,7 @@ int hvp_cg_impl(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T*
       coef[fp.fW] = fp.alpha / fp.beta;
     }
     if ((rc = launch_pencil<T>(P, F, kinds, P->partials, coef))) return rc;
-    nblk_pen = (long long)((P->nhp + RegTile<T>::W - 1) / RegTile<T>::W) * P->n[1] * F;
+    nblk_pen = (long long)((P->nhp + P->tile_pencil - 1) / P->tile_pencil) * P->n[1] * F;
     if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
   }
   hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
@@ -3921,6 +3964,55 @@ bool valid(ofdft_plan P) { return P != nullptr; }
 // ===========================================================================
 namespace {
 
+// One fused CG iteration (the HVP with the CG step folded in, then the TN
+// control) as a graph: captured on the plan stream at the first use with
+// these buffers, then launched once per iteration.  The captured kernels
+// read their control from the device TnState, so replays need no new
+// arguments.
+template <typename T>
+int tn_graph_iteration(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T* best, TnState* st,
+                       double* rr) {
+  const void* key[8] = {phi, p, W0, d, r, best, st, rr};
+  if (!P->tn_exec || std::memcmp(key, P->tn_key, sizeof(key)) != 0) {
+    if (P->tn_exec) {
+      (void)hipGraphExecDestroy(P->tn_exec);
+      P->tn_exec = nullptr;
+    }
+    if (!P->cap_stream) HIPCHK(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+    // the launchers enqueue on P->stream: point it at the capture stream
+    hipStream_t run = P->stream;
+    P->stream = P->cap_stream;
+    const hipError_t eb = hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal);
+    if (eb != hipSuccess) {
+      P->stream = run;
+      return fail(OFDFT_EHIP, std::string("graph capture begin: ") + hipGetErrorString(eb));
+    }
+    long long nbr = 0;
+    int rc = hvp_cg_impl<T>(P, phi, p, W0, d, r, best, (const TnState*)st, &nbr);
+    if (!rc) {
+      hipLaunchKernelGGL(k_tn_reduce_ctrl4, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials,
+                         nbr, (const double*)P->red, rr);
+      if (hipGetLastError() != hipSuccess) rc = fail(OFDFT_EHIP, "control launch during graph capture");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(P->stream, &g);
+    P->stream = run;
+    if (!rc && e != hipSuccess) rc = fail(OFDFT_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    if (!rc) {
+      const hipError_t ei = hipGraphInstantiate(&P->tn_exec, g, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        P->tn_exec = nullptr;
+        rc = fail(OFDFT_EHIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    std::memcpy(P->tn_key, key, sizeof(key));
+  }
+  HIPCHK(hipGraphLaunch(P->tn_exec, P->stream));
+  return 0;
+}
+
 template <typename T>
 int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, T* d, T* work, int maxfun, int chunk,
                       int* niter, int* status, double* pAp_last) {
@@ -3959,7 +4051,9 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
         ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
         OFDFT_LAUNCH((k_tn_next<T>), dim3(nb), dim3(256), 0, P, (const TnState*)st, (const T*)r, p, N);
       }
-      if (cg_fused) {
+      if (cg_fused && fused && P->use_graph && !P->prof_on) {
+        if ((rc = tn_graph_iteration<T>(P, phi, p, W0, d, r, best, st, rr))) break;
+      } else if (cg_fused) {
         long long nbr = 0;
         if ((rc = hvp_cg_impl<T>(P, phi, p, W0, d, r, best, (const TnState*)st, &nbr))) break;
         hipLaunchKernelGGL(k_tn_reduce_ctrl4, dim3(1), dim3(kRedThreads), 0, P->stream, st,

@@ -13,8 +13,9 @@ def ecut2spacing(ecut):
 
 def best_fft_size(N, max_prime=13, scale=0.99, even=True):
     """Smallest 2^a 3^b 5^c 7^d (x 11 or 13) size >= ceil(scale*N) - 1
-    (mp_serial.py:17-83).  Note: the device FFT supports radices 2, 3, 5;
-    use ``fft_friendly_size`` for sizes the GPU path accepts."""
+    (mp_serial.py:17-83).  Every such size is a product of 2, 3, 5, 7, 11
+    and 13, which the device transforms accept (up to the per-axis LDS
+    bound of the plan: FP64 601 on axes 0 and 1)."""
     a = int(np.ceil(np.log2(N))) + 1
     b = int(np.ceil(np.log(N) / np.log(3))) + 1
     c = int(np.ceil(np.log(N) / np.log(5))) + 1

@@ -24,6 +24,9 @@ Options beyond the reference's:
                  line search fails (cut after 8 trials)
   ls_first_step  'unit' (the reference code: first trial theta = 1) or
                  'theta' (first trial = the running theta)
+  tn_graph       True (default): the device TN loop replays one captured
+                 hipGraph per CG iteration instead of launching its kernels
+                 (same results; about 3 % faster at 192^3); False: launches
 """
 import math
 import time
@@ -167,6 +170,7 @@ class Optimization:
         defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
                     "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
                     "ncheck": 2, "h0": 1.0, "hvp": "exact", "precision": None, "mixed_switch": None,
+                    "tn_graph": True,
                     "ls_first_step": "unit"}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
@@ -446,6 +450,7 @@ class Optimization:
         if key not in self._works:
             self._works = {k: w for k, w in self._works.items() if k[0] != prec}
             self._works[key] = _Work(ev)
+        ev.P.set_graph(bool(self.optimization_options.get("tn_graph")))
         return self._works[key]
 
     def _residual(self, W, mu):

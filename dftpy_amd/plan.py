@@ -49,6 +49,11 @@ class Plan:
                                                self.prec_code, 1 if self.full else 0, ctypes.c_void_p(stream)))
         self.functional = None
         self.bound = False
+        # column tile widths measured fastest for these lengths (tuning.py)
+        from . import tuning
+        self.tiles = tuning.tiles_for(self.nr, precision)
+        if self.tiles != (tuning.DEFAULT, tuning.DEFAULT):
+            self.set_tiles(*self.tiles)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -315,6 +320,17 @@ class Plan:
         by = L.darr(n)
         L.check(self.lib.ofdft_plan_get_profile(self._h, ms, cnt, by))
         return {k: (ms[i], int(cnt[i]), by[i]) for i, k in enumerate(self.PROFILE_CLASSES)}
+
+    def set_tiles(self, tile_axis, tile_pencil):
+        """Columns per workgroup of the axis pass and of the pencils
+        (ofdft_plan_set_tiles)."""
+        L.check(self.lib.ofdft_plan_set_tiles(self._h, int(tile_axis), int(tile_pencil)))
+        self.tiles = (int(tile_axis), int(tile_pencil))
+
+    def set_graph(self, on=True):
+        """Run the device TN loop's CG iterations as replays of one captured
+        hipGraph (ofdft_plan_set_graph)."""
+        L.check(self.lib.ofdft_plan_set_graph(self._h, 1 if on else 0))
 
     def workspace_bytes(self):
         return int(self.lib.ofdft_plan_workspace_bytes(self._h))

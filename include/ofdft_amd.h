@@ -85,10 +85,28 @@ typedef struct {
  * full=0: rfftn/irfftn semantics (grid full=False, field.py:166-170).
  * full=1: the reference's full complex grid (grid.py:302 default), i.e.
  * Re(ifftn(M * fftn(f))); realised on the half spectrum with the
- * Hermitian-symmetrised multiplier (M(G_k) + M(G_-k))/2.                  */
+ * Hermitian-symmetrised multiplier (M(G_k) + M(G_-k))/2.
+ * Grid lengths: products of 2, 3, 5, 7, 11 and 13 (at most 12 passes), with
+ * the LDS bound of the transforms (160 KiB per workgroup on gfx950): FP64
+ * n0, n1 <= 601 and n2 <= 1462; FP32 n0, n1 <= 1203 and n2 <= 2925.
+ * Anything else returns OFDFT_EINVAL before any device call.              */
 int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lattice9, int precision, int full,
                       void* hip_stream);
 int ofdft_plan_destroy(ofdft_plan plan);
+/* Execution mode of ofdft_tn_direction's inner CG loop (the loop of
+ * get_direction_TN, optimization.py:92-148): on=1 captures one CG iteration
+ * (its kernel launches only) into a hipGraph at the first call with a given
+ * set of buffers and launches that graph once per queued iteration instead
+ * of the individual kernels; on=0 (default) launches the kernels.  Results
+ * are identical; profiling (ofdft_plan_set_profiling) uses the launches.  */
+int ofdft_plan_set_graph(ofdft_plan plan, int on);
+/* Columns per workgroup of the compile-time-length column kernels: the
+ * axis-1 pass and the axis-0 pencils (forward transform, multiplier,
+ * inverse).  8 or 16 (FP64), 8, 16 or 32 (FP32); default 16.  The values
+ * the project's autotuner measured fastest per (length, precision) are
+ * applied by dftpy_amd at plan creation.  Results do not depend on them.
+ * No counterpart in the reference (numpy.fft has no tiling).              */
+int ofdft_plan_set_tiles(ofdft_plan plan, int tile_axis, int tile_pencil);
 int ofdft_plan_set_stream(ofdft_plan plan, void* hip_stream);
 /* Device bytes of workspace the plan owns. */
 long long ofdft_plan_workspace_bytes(ofdft_plan plan);

@@ -118,3 +118,20 @@ def test_grid_sizing_matches_oracle():
     # known answers of the reference's tests/unit/test_math_utils.py:4-12
     assert M.ecut2spacing(1) == pytest.approx(2.221441469079183)
     assert M.ecut2spacing(2) == pytest.approx(1.5707963267948966)
+
+
+def test_plan_size_limits_raise_before_any_device_call():
+    """The per-axis LDS bound (160 KiB per workgroup): FP64 n0 <= 601,
+    FP32 n0 <= 1203, checked on the host before anything touches a GPU."""
+    import ctypes
+    from dftpy_amd import _lib as L
+    lib = L.load()
+    lat = (ctypes.c_double * 9)(*[10.0, 0, 0, 0, 10.0, 0, 0, 0, 10.0])
+    h = ctypes.c_void_p()
+    for prec, n in ((L.F64, 640), (L.F32, 1250)):
+        rc = lib.ofdft_plan_create(ctypes.byref(h), n, 32, 32, lat, prec, 0, None)
+        assert rc != 0
+        assert "LDS" in lib.ofdft_last_error().decode()
+    # a length with a prime factor above 13 is refused on the host too
+    rc = lib.ofdft_plan_create(ctypes.byref(h), 34, 32, 32, lat, L.F64, 0, None)
+    assert rc != 0 and "products of 2, 3, 5, 7, 11 and 13" in lib.ofdft_last_error().decode()

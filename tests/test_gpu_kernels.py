@@ -19,7 +19,9 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(32, 32, 32), (20, 30, 20), (36, 36, 72), (48, 40, 30), (96, 96, 96), (24, 50, 18), (15, 9, 20), (20, 18, 15),
           (160, 24, 192), (108, 90, 48), (90, 64, 108),
           # factors 7, 11, 13 (grids of the reference's max_prime = 13 sizing)
-          (20, 30, 22), (14, 22, 26), (7, 11, 13), (143, 12, 10), (28, 44, 52), (12, 10, 143)]
+          (20, 30, 22), (14, 22, 26), (7, 11, 13), (143, 12, 10), (28, 44, 52), (12, 10, 143),
+          # column workgroups above 64 KiB of LDS (FP64 n0, n1 > 240): up to the 160 KiB bound
+          (256, 250, 16), (24, 600, 18)]
 
 
 def _plan(nr, lattice, prec="fp64", full=False):
@@ -373,3 +375,42 @@ def test_orth_sums_match_numpy(prec):
     ref = (np.sum(p * phi), np.sum(phi * phi), np.sum(p * p), np.sum(V * phi * p), np.sum(V * phi * phi))
     for g, r in zip(got, ref):
         assert abs(g - r) <= 1e-12 * max(abs(r), 1.0)
+
+
+@pytest.mark.parametrize("n", [48, 96, 192])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_column_tile_widths_give_identical_transforms(n, prec):
+    """Every tile width the autotuner may pick (8, 16, 32 columns per
+    workgroup) computes each column with the same instructions: the
+    transforms and a full evaluation agree bit for bit, and match numpy."""
+    print("case tiles", n, prec, flush=True)
+    nr = (n, n // 2, n // 4 * 2)
+    P = _plan(nr, np.eye(3) * 10.0, prec)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(nr)
+    ft = torch.tensor(f, dtype=P.dtype, device=P.device)
+    ref = np.fft.rfftn(f)
+    out = {}
+    for w in ((8, 16, 32) if prec == "fp32" else (8, 16)):
+        P.set_tiles(w, w)
+        F = P.r2c(ft)
+        g = P.c2r(F.clone())
+        out[w] = (F.cpu().numpy(), g.cpu().numpy())
+    tol = 1e-12 if prec == "fp64" else 2e-6
+    for w, (F, g) in out.items():
+        assert np.array_equal(F, out[16][0]) and np.array_equal(g, out[16][1]), w
+    assert np.abs(out[16][0] - ref).max() <= tol * np.abs(ref).max()
+    # a fused evaluation (three convolution fields through the pencils)
+    P2 = _plan((n, n, n), np.eye(3) * (0.25 * n), prec)
+    P2.set_functional(tf=True, vw=True, wt=True, lda=True, hartree=True, rho0=0.02)
+    rho = torch.tensor(0.02 * (1.0 + 0.1 * rng.random((n, n, n))), dtype=P2.dtype, device=P2.device)
+    sums = {}
+    for w in ((8, 16, 32) if prec == "fp32" else (8, 16)):
+        P2.set_tiles(w, w)
+        V = P2.empty()
+        sums[w] = (P2.eval(rho, V), V.cpu().numpy())
+    for w in sums:
+        assert np.array_equal(sums[w][1], sums[16][1]), w
+        assert np.array_equal(sums[w][0][:6], sums[16][0][:6]), w
+    with pytest.raises(Exception):
+        P2.set_tiles(12, 16)

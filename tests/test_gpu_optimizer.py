@@ -154,3 +154,28 @@ def test_al4_32_mixed_precision_within_1e6_ev_per_atom(phase1):
     print("mixed", phase1, omx.energy, o64.energy, err, [l["prec"] for l in omx.log])
     assert omx.converged == 0
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_tn_graph_replays_give_the_same_solve(prec):
+    """The device TN loop with each CG iteration replayed from one captured
+    hipGraph (many replays per solve, with the optimizer's own launches and
+    host reads between them) reproduces the launch-mode solve exactly."""
+    print("case TN graph mode al108_96", prec, flush=True)
+    wl = W.build("al108_96")
+    grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
+    ions = D.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    pp = D.LocalPseudo(grid=grid, ions=ions, PP_list={"Al": D.empty_core(3.0, 1.12)})
+    ev = D.TotalFunctional(KEDF=D.KEDF("WT"), XC=D.XC("LDA"), HARTREE=D.Hartree(), PSEUDO=pp)
+    res = {}
+    for graph in (False, True):
+        opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
+                             optimization_options={"econv": 1e-9, "precision": prec, "tn_graph": graph,
+                                                   "mixed_phase1": True})
+        opt.optimize_rho(guess_rho=D.DirectField(grid=grid, data=wl["rho_ini"]))
+        res[graph] = opt
+        print(graph, opt.energy, [l["Nd"] for l in opt.log])
+        assert opt.converged == 0
+    assert res[True].energy == res[False].energy
+    assert [l["Nd"] for l in res[True].log] == [l["Nd"] for l in res[False].log]
+    assert [l["energy"] for l in res[True].log] == [l["energy"] for l in res[False].log]
