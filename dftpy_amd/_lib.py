@@ -116,6 +116,11 @@ _SIGS = {
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
     "ofdft_plan_set_graph": (_i, [_vp, _i]),
     "ofdft_plan_set_tiles": (_i, [_vp, _i, _i]),
+    "ofdft_batch_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i]),
+    "ofdft_batch_destroy": (_i, [_vp]),
+    "ofdft_batch_tn_direction": (_i, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i, ctypes.POINTER(_i),
+                                      ctypes.POINTER(_i), _dp]),
     "ofdft_plan_get_profile": (_i, [_vp, _dp, ctypes.POINTER(_ll), _dp]),
     "ofdft_last_error": (ctypes.c_char_p, []),
     "ofdft_version": (_i, []),

@@ -1,6 +1,11 @@
 """Many independent cells solved concurrently on one GPU.
 
-Each worker thread owns one HIP stream (torch.cuda.Stream, default
+Two forms.  ``solve_batch_persistent`` runs the cells in lockstep: every TN
+outer iteration runs the inner CG loops of all cells in ONE persistent
+cooperative launch (``ofdft_batch_tn_direction``: each stage of a CG
+iteration is a work queue over (cell, tile) items of every cell, stages
+separated by grid barriers), then each cell's line search and update.
+``solve_batch`` hands the cells to worker threads instead: each worker thread owns one HIP stream (torch.cuda.Stream, default
 priority) and solves its share of the cells with the ordinary
 ``Optimization``; every plan a worker creates is bound to that worker's
 stream, so the kernels of different cells overlap on the device while the
@@ -11,11 +16,14 @@ largest first (cost ~ n^3 log n) to balance the workers.
 This is the batched counterpart of looping ``optimize_rho`` over cells
 (the reference has no batching; each cell is one DFTpy run).
 """
+import ctypes
 import math
 import threading
 import time
 
 import torch
+
+from . import _lib as L
 
 from .field import DirectField
 from .functional import KEDF, XC, Hartree, TotalFunctional
@@ -56,6 +64,104 @@ def solve_cell(cell, precision="mixed", econv=1e-8, options=None):
     return dict(name=cell.get("name"), energy=opt.energy, converged=opt.converged, nat=len(cell["symbols"]),
                 nr=tuple(cell["nr"]), iterations=sum(1 for l in opt.log[1:] if l["prec"] != "switch->fp64"),
                 setup_s=t1 - t0, solve_s=t2 - t1)
+
+
+class TnBatch:
+    """ofdft_batch over the inner-loop plans of several cells (one
+    precision, one stream)."""
+
+    def __init__(self, plans):
+        self.lib = L.load()
+        self.plans = list(plans)
+        n = len(self.plans)
+        arr = (ctypes.c_void_p * n)(*[P._h.value for P in self.plans])
+        self._h = ctypes.c_void_p()
+        L.check(self.lib.ofdft_batch_create(ctypes.byref(self._h), arr, n))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self.lib.ofdft_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def direction(self, works, maxfun):
+        """get_direction_TN of every cell (work buffers of optimization._Work:
+        phi, res, W0 in, tn()["d"] out) in one persistent launch.  Returns
+        (niter, status, pAp_last) lists."""
+        n = len(works)
+        if n != len(self.plans):
+            raise ValueError("one work per plan of the batch")
+
+        def arr(ts):
+            return (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+
+        tn = [w.tn() for w in works]
+        niter, status, pap = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_double * n)()
+        L.check(self.lib.ofdft_batch_tn_direction(self._h, arr([w.phi for w in works]), arr([w.res for w in works]),
+                                                  arr([w.W0 for w in works]), arr([t["d"] for t in tn]),
+                                                  arr([t["work"] for t in tn]), int(maxfun), niter, status, pap))
+        return list(niter), list(status), list(pap)
+
+
+def solve_batch_persistent(cells, econv=1e-8, options=None):
+    """Solve all cells in lockstep, mixed precision (FP64 outer iterations,
+    FP32 inner CG with the exact Hessian product): per TN outer iteration,
+    one persistent launch runs the inner loops of every unconverged cell,
+    then each cell takes its line search and update (the reference's
+    optimize_rho per cell, optimization.py:283-525).  Returns the per-cell
+    results (input order) and the wall time."""
+    from .optimization import _finite
+    opts_common = {"econv": econv, "precision": "mixed", "mixed_phase1": False, "maxiter": 200, "hvp": "exact",
+                   "tn_graph": False}
+    if options:
+        opts_common.update(options)
+    built = []
+    for c in cells:
+        grid, total = build_cell(c)
+        total.ewald.energy
+        rho = DirectField(grid=grid, data=c["rho_ini"])
+        opt = Optimization(EnergyEvaluator=total, optimization_method="TN", optimization_options=dict(opts_common))
+        opt.verbose = False
+        built.append((c, opt, rho))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c, opt, rho in built:
+        opt._begin(guess_rho=rho)
+    maxiter = opts_common["maxiter"]
+    maxfun = opts_common.get("maxfun", 50)
+    active = list(range(len(built)))
+    batch, batch_key = None, None
+    it = 0
+    while active and it < maxiter - 1:
+        it += 1
+        for i in active:
+            st = built[i][1]._st
+            built[i][1]._tn_inner_inputs(st.W, st.Wi)
+        if tuple(active) != batch_key:
+            batch = TnBatch([built[i][1]._st.Wi.P for i in active])
+            batch_key = tuple(active)
+        niter, status, pap = batch.direction([built[i][1]._st.Wi for i in active], maxfun)
+        still = []
+        for k, i in enumerate(active):
+            opt = built[i][1]
+            st = opt._st
+            _finite(pap[k], "p.Ap in the truncated-Newton loop")
+            st.W.P.convert_from(st.Wi.tn()["d"], st.W.p)
+            opt.nhvp += niter[k]
+            if not opt._advance(st, it, niter[k]):
+                still.append(i)
+        active = still
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    results = []
+    for c, opt, rho in built:
+        opt._end()
+        results.append(dict(name=c.get("name"), energy=opt.energy, converged=opt.converged, nat=len(c["symbols"]),
+                            nr=tuple(c["nr"]), iterations=len(opt.log) - 1))
+    return results, wall
 
 
 def solve_batch(cells, precision="mixed", workers=4, econv=1e-8, options=None):

@@ -203,6 +203,53 @@ int ofdft_plan_destroy(ofdft_plan P) {
   return 0;
 }
 
+int ofdft_batch_create(ofdft_batch* out, const ofdft_plan* plans, int nb) {
+  if (!out || !plans) return fail(OFDFT_EINVAL, "null argument");
+  *out = nullptr;
+  if (nb < 1) return fail(OFDFT_EINVAL, "a batch needs at least one plan");
+  for (int c = 0; c < nb; ++c) {
+    const ofdft_plan_s* P = plans[c];
+    if (!P) return fail(OFDFT_EINVAL, "null plan in the batch");
+    if (P->prec != plans[0]->prec) return fail(OFDFT_EINVAL, "the plans of a batch must share one precision");
+    const int n = P->n[0];
+    if (P->n[1] != n || P->n[2] != n || !(n == 32 || n == 48 || n == 64 || n == 90 || n == 96 || n == 108 ||
+                                           n == 128 || n == 160 || n == 192))
+      return fail(OFDFT_EINVAL, "batched plans must be cubic with a compile-time length "
+                                "(32, 48, 64, 90, 96, 108, 128, 160, 192)");
+  }
+  auto* B = new (std::nothrow) ofdft_batch_s();
+  if (!B) return fail(OFDFT_ENOMEM, "host allocation failed");
+  B->plans.assign(plans, plans + nb);
+  B->prec = plans[0]->prec;
+  *out = B;
+  return 0;
+}
+
+int ofdft_batch_destroy(ofdft_batch B) {
+  if (!B) return 0;
+  if (B->d_cells) (void)hipFree(B->d_cells);
+  if (B->d_pref) (void)hipFree(B->d_pref);
+  delete B;
+  return 0;
+}
+
+int ofdft_batch_tn_direction(ofdft_batch B, const void* const* phi, const void* const* res, const void* const* W0,
+                             void* const* d, void* const* work, int maxfun, int* niter, int* status,
+                             double* pAp_last) {
+  if (!B || !phi || !res || !W0 || !d || !work || !niter || !status || !pAp_last)
+    return fail(OFDFT_EINVAL, "null argument");
+  if (B->prec == OFDFT_F64)
+    return ofdft_tu::tn_batch_run<double>(B, reinterpret_cast<const double* const*>(phi),
+                                          reinterpret_cast<const double* const*>(res),
+                                          reinterpret_cast<const double* const*>(W0),
+                                          reinterpret_cast<double* const*>(d), reinterpret_cast<double* const*>(work),
+                                          maxfun, niter, status, pAp_last);
+  return ofdft_tu::tn_batch_run<float>(B, reinterpret_cast<const float* const*>(phi),
+                                       reinterpret_cast<const float* const*>(res),
+                                       reinterpret_cast<const float* const*>(W0), reinterpret_cast<float* const*>(d),
+                                       reinterpret_cast<float* const*>(work), maxfun, niter, status, pAp_last);
+}
+
 int ofdft_plan_set_tiles(ofdft_plan P, int tile_axis, int tile_pencil) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
   auto ok = [&](int w) { return w == 8 || w == 16 || (w == 32 && P->prec == OFDFT_F32); };

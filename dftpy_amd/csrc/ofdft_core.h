@@ -195,6 +195,16 @@ struct ofdft_plan_s {
   hipStream_t cap_stream = nullptr;  // capture happens here (the null stream cannot capture)
 };
 
+// A group of plans whose TN inner loops run in one persistent launch
+// (ofdft_batch_create / ofdft_batch_tn_direction).
+struct ofdft_batch_s {
+  std::vector<ofdft_plan_s*> plans;
+  int prec = 0;
+  void* d_cells = nullptr;      // per-plan descriptors of the persistent kernel
+  long long* d_pref = nullptr;  // work-item prefix sums of its stages
+  size_t cells_cap = 0, pref_cap = 0;
+};
+
 namespace {
 
 constexpr int kMaxSum = 16;
@@ -1164,20 +1174,18 @@ constexpr int fixed_len() {
 
 // FC > 0: the field count F fixed at compile time (index arithmetic by F
 // becomes constant); FC = 0: F taken from Frt.
+// One workgroup's share of the forward row pass (row pairs bx*PB ..):
+// the body of k_rows_fwd, also run by the persistent batched TN kernel.
 template <typename T, class FFT, class Pre, int FC>
-__global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int nhp, int Frt,
-                                                      const cplx<T>* __restrict__ tw,
-                                                      const int* __restrict__ skip) {
-  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+__device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, cplx<T>* __restrict__ spec,
+                                               long long nrows, long long Nh, int nhp, int Frt,
+                                               const cplx<T>* __restrict__ tw, long long bx, cplx<T>* lds) {
   const int F = FC > 0 ? FC : Frt;
-  extern __shared__ unsigned char smem_raw[];
-  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPB<T>;
   const int n2 = fft.n();
   const int nh = n2 / 2 + 1;
   const int th = fft.th();
-  const long long pair0 = (long long)blockIdx.x * PB;
+  const long long pair0 = bx * PB;
   const int ld = n2;
   if constexpr (FFT::kFixed) {
     // every load of the thread's points is issued before the first use
@@ -1235,26 +1243,37 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
   }
 }
 
+template <typename T, class FFT, class Pre, int FC>
+__global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
+                                                      long long nrows, long long Nh, int nhp, int Frt,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  extern __shared__ unsigned char smem_raw[];
+  rows_fwd_block<T, FFT, Pre, FC>(pre, fft, spec, nrows, Nh, nhp, Frt, tw, blockIdx.x,
+                                  reinterpret_cast<cplx<T>*>(smem_raw));
+}
+
 // Axis-2 complex-to-real fused with the post functor.  Two half spectra A, B
 // are combined into Z = A + iB on the full axis (Hermitian extension, DC and
 // Nyquist imaginary parts dropped as irfft does); after the inverse FFT the
 // real part is row r, the imaginary part row r+1.  For compile-time lengths
 // the post functor's own inputs are loaded first, so that their latency
 // overlaps the spectrum loads and the transform.
+// One workgroup's share of the inverse row pass: the body of k_rows_inv,
+// also run by the persistent batched TN kernel; the workgroup's sums go to
+// partials[bx * NS ..] (none when partials is null).
 template <typename T, class FFT, class Post, int FC>
-__global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
-                                                      long long nrows, long long Nh, int nhp, int Frt,
-                                                      const cplx<T>* __restrict__ tw,
-                                                      double* __restrict__ partials, const int* __restrict__ skip) {
-  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+__device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft, const cplx<T>* __restrict__ spec,
+                                               long long nrows, long long Nh, int nhp, int Frt,
+                                               const cplx<T>* __restrict__ tw, double* __restrict__ partials,
+                                               long long bx, cplx<T>* lds) {
   const int F = FC > 0 ? FC : Frt;
-  extern __shared__ unsigned char smem_raw[];
-  cplx<T>* lds = reinterpret_cast<cplx<T>*>(smem_raw);
   constexpr int PB = kRowPB<T>;
   const int n2 = fft.n();
   const int nh = n2 / 2 + 1;
   const int th = fft.th();
-  const long long pair0 = (long long)blockIdx.x * PB;
+  const long long pair0 = bx * PB;
   const int ld = n2;
   const bool even = (n2 % 2) == 0;
   constexpr int NN = fixed_len<FFT>();
@@ -1329,7 +1348,18 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const 
       if (r1 < nrows) post(r1 * n2 + j, cb, acc);
     }
   }
-  if (partials) block_sums<Post::NS>(acc, partials + (long long)blockIdx.x * Post::NS);
+  if (partials) block_sums<Post::NS>(acc, partials + bx * Post::NS);
+}
+
+template <typename T, class FFT, class Post, int FC>
+__global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
+                                                      long long nrows, long long Nh, int nhp, int Frt,
+                                                      const cplx<T>* __restrict__ tw,
+                                                      double* __restrict__ partials, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  extern __shared__ unsigned char smem_raw[];
+  rows_inv_block<T, FFT, Post, FC>(post, fft, spec, nrows, Nh, nhp, Frt, tw, partials, blockIdx.x,
+                                   reinterpret_cast<cplx<T>*>(smem_raw));
 }
 
 // Strided complex transform along one axis: blockIdx.y selects the outer
@@ -1476,35 +1506,38 @@ constexpr int col_block_stride() {
   return s;
 }
 
-// Four-step register transform of W columns along a strided axis (one
-// workgroup, W*N2 threads): x -> X in place, natural order in and out.
+// One workgroup's share of the four-step axis transform (the body of
+// k_axis_reg, also run by the persistent batched TN kernel).
 template <typename T, int N1, int N2, int W>
-__global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
-                                                     long long ostride, int ncol, const cplx<T>* __restrict__ tw,
-                                                     int inv, const int* __restrict__ skip) {
-  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+__device__ __forceinline__ void axis_reg_block(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+                                               long long ostride, int ncol, const cplx<T>* __restrict__ tw, int inv,
+                                               int bx, int by, int bz, cplx<T>* lds) {
   constexpr int S = col_block_stride<T, N1, N2, W>();
-  __shared__ cplx<T> lds[W * S];
+  // threads beyond the W x N2 of the split idle (the persistent kernel runs
+  // every length with one workgroup size); they still reach the barrier
   const int t = threadIdx.x, w = t % W, q = t / W;
-  cplx<T>* base = spec + (long long)blockIdx.z * Nh + (long long)blockIdx.y * ostride;
-  const int c = blockIdx.x * W + w;
-  const bool ok = c < ncol;
+  const bool act = t < W * N2;
+  cplx<T>* base = spec + (long long)bz * Nh + (long long)by * ostride;
+  const int c = bx * W + w;
+  const bool ok = act && c < ncol;
   const bool iv = inv != 0;
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1)
     v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
-  dftreg<T, N1>(v, iv);
+  if (act) {
+    dftreg<T, N1>(v, iv);
 #pragma unroll
-  for (int k1 = 1; k1 < N1; ++k1) {
-    cplx<T> wv = tw[q * k1];
-    if (iv) wv.y = -wv.y;
-    v[k1] = cmul(v[k1], wv);
+    for (int k1 = 1; k1 < N1; ++k1) {
+      cplx<T> wv = tw[q * k1];
+      if (iv) wv.y = -wv.y;
+      v[k1] = cmul(v[k1], wv);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
   }
-#pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
   __syncthreads();
-  if (q < N1) {
+  if (act && q < N1) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
     dftreg<T, N2>(v, iv);
@@ -1515,24 +1548,36 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec,
   }
 }
 
-// Four-step axis-0 pencils: forward transform, multiplier, inverse transform
-// with two LDS transposes and no intermediate global traffic.
+// Four-step register transform of W columns along a strided axis (one
+// workgroup, W*N2 threads): x -> X in place, natural order in and out.
 template <typename T, int N1, int N2, int W>
-__global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
-                                                       const cplx<T>* __restrict__ tw, ConvKinds ck,
-                                                       const T* __restrict__ wtk, const int* __restrict__ skip,
-                                                       double* __restrict__ cgp) {
+__global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+                                                     long long ostride, int ncol, const cplx<T>* __restrict__ tw,
+                                                     int inv, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
+  axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+}
+
+// Four-step axis-0 pencils of one workgroup: forward transform, multiplier,
+// inverse transform with two LDS transposes and no intermediate global
+// traffic.  (by, bz) = (axis-1 index, field); cgp: the workgroup's share of
+// the spectral part of p.Ap goes to cgp[pslot] (TN inner loop).
+template <typename T, int N1, int N2, int W>
+__device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, long long Nh, const GridGeom& geo,
+                                                 const cplx<T>* __restrict__ tw, const ConvKinds& ck,
+                                                 const T* __restrict__ wtk, double* __restrict__ cgp,
+                                                 long long pslot, int bx, int by, int bz, cplx<T>* lds) {
   constexpr int S = col_block_stride<T, N1, N2, W>();
-  __shared__ cplx<T> lds[W * S];
   const int t = threadIdx.x, w = t % W, q = t / W;
-  const int i1 = blockIdx.y;
-  const int f = blockIdx.z;
+  const bool act = t < W * N2;
+  const int i1 = by;
+  const int f = bz;
   const long long stride = (long long)geo.n1 * geo.nhp;
   cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nhp;
-  const int c = blockIdx.x * W + w;
-  const bool ok = c < geo.nhp;     // padded columns are transformed (they stay zero)
-  const bool live = c < geo.nh;
+  const int c = bx * W + w;
+  const bool ok = act && c < geo.nhp;  // padded columns are transformed (they stay zero)
+  const bool live = act && c < geo.nh;
   const int kind = ck.kind[f];
   cplx<T> v[N2];
 #pragma unroll
@@ -1545,16 +1590,18 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2)
     wm[k2] = wt ? wtk[((long long)(q + N1 * k2) * geo.n1 + i1) * geo.nh + c] : T(0);
-  dftreg<T, N1>(v, false);
+  if (act) {
+    dftreg<T, N1>(v, false);
 #pragma unroll
-  for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
+    for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
+    for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
+  }
   __syncthreads();
   // sum over this thread's points of m |X|^2: the field's share of p.Ap in
   // the TN inner loop (Hermitian weight 2 off the c = 0 and Nyquist planes)
   double pap = 0.0;
-  if (q < N1) {
+  if (act && q < N1) {
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
     dftreg<T, N2>(v, false);
@@ -1606,18 +1653,32 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
     for (int n2 = 0; n2 < N2; ++n2) lds[w * S + q * (N2 + 1) + n2] = v[n2];
   }
   __syncthreads();
+  if (act) {
 #pragma unroll
-  for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[w * S + k1 * (N2 + 1) + q];
-  dftreg<T, N1>(v, true);
-  if (ok) {
+    for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[w * S + k1 * (N2 + 1) + q];
+    dftreg<T, N1>(v, true);
+    if (ok) {
 #pragma unroll
-    for (int n1 = 0; n1 < N1; ++n1) base[(long long)(N2 * n1 + q) * stride + c] = v[n1];
+      for (int n1 = 0; n1 < N1; ++n1) base[(long long)(N2 * n1 + q) * stride + c] = v[n1];
+    }
   }
   if (cgp) {
     const bool edge = c == 0 || ((geo.n2 & 1) == 0 && c == geo.n2 / 2);
     double acc[1] = {live ? pap * ck.coef[f] * (edge ? 1.0 : 2.0) : 0.0};
-    block_sums<1>(acc, cgp + ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
+    block_sums<1>(acc, cgp + pslot);
   }
+}
+
+template <typename T, int N1, int N2, int W>
+__global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
+                                                       const cplx<T>* __restrict__ tw, ConvKinds ck,
+                                                       const T* __restrict__ wtk, const int* __restrict__ skip,
+                                                       double* __restrict__ cgp) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
+  pencil_reg_block<T, N1, N2, W>(spec, Nh, geo, tw, ck, wtk, cgp,
+                                 ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, blockIdx.x,
+                                 blockIdx.y, blockIdx.z, lds);
 }
 
 
@@ -1778,8 +1839,7 @@ __global__ void k_tn_init(const T* __restrict__ res, const T* __restrict__ W0, T
   block_sums<2>(acc, partials + 2 * (long long)blockIdx.x);
 }
 
-__global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void tn_start_state(TnState* st, const double* __restrict__ rr, int maxit) {
   const double r0 = rr[0];
   st->r0 = r0;
   st->rconv = 0.1 * r0;
@@ -1797,6 +1857,11 @@ __global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit
   st->alias = 1;
   st->fin = 0;
   st->done = maxit > 0 ? 0 : 1;
+}
+
+__global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  tn_start_state(st, rr, maxit);
 }
 
 // Best = direction when the previous iteration asked for a copy (direction
@@ -1889,20 +1954,24 @@ __global__ void k_tn_reduce_ctrl(TnState* st, const double* __restrict__ partial
 
 // pAp = the pencil's spectral sums + sum W0 p^2 (kept in the state) into
 // red[0], before the inverse row pass that takes the CG step.
-__global__ void k_tn_pap(const TnState* st, const double* __restrict__ partials, long long nblk,
-                         double* __restrict__ red) {
-  if (st->done) return;
+__device__ __forceinline__ void tn_pap_block(const TnState* st, const double* __restrict__ partials, long long nblk,
+                                             double* __restrict__ red) {
   double tmp[1] = {thread_partials<1>(partials, nblk, 0)};
   block_sums<1>(tmp, red);
   if (threadIdx.x == 0) red[0] += st->wpp;
 }
 
+__global__ void k_tn_pap(const TnState* st, const double* __restrict__ partials, long long nblk,
+                         double* __restrict__ red) {
+  if (st->done) return;
+  tn_pap_block(st, partials, nblk, red);
+}
+
 // r1 and the three W0 sums of the inverse row pass, the tests of
 // optimization.py:116-145, and sum W0 p^2 of the next direction
 // p' = r + beta p from W0 r^2 + 2 beta W0 r p + beta^2 W0 p^2.
-__global__ void k_tn_reduce_ctrl4(TnState* st, const double* __restrict__ partials, long long nblk,
-                                  const double* __restrict__ red, double* __restrict__ rr) {
-  if (st->done) return;
+__device__ __forceinline__ void tn_ctrl4_block(TnState* st, const double* __restrict__ partials, long long nblk,
+                                               const double* __restrict__ red, double* __restrict__ rr) {
   // the four sums of a block are adjacent: one pass over the blocks with
   // two independent accumulator sets keeps two 32-byte loads in flight
   double tmp[4] = {0.0, 0.0, 0.0, 0.0}, tm2[4] = {0.0, 0.0, 0.0, 0.0};
@@ -1930,6 +1999,12 @@ __global__ void k_tn_reduce_ctrl4(TnState* st, const double* __restrict__ partia
       st->wpp = rr[1] + 2.0 * b * rr[2] + b * b * rr[3];
     }
   }
+}
+
+__global__ void k_tn_reduce_ctrl4(TnState* st, const double* __restrict__ partials, long long nblk,
+                                  const double* __restrict__ red, double* __restrict__ rr) {
+  if (st->done) return;
+  tn_ctrl4_block(st, partials, nblk, red, rr);
 }
 
 // p = r + beta p when the control asked for it: used only where no forward
@@ -4294,3 +4369,11 @@ int lwt(ofdft_plan_s* P, const ofdft_lwt_params& L, const double* kfl, int nsp, 
 #define OFDFT_TU_LWT(EXT, T)                                                                                     \
   EXT template int ofdft_tu::lwt<T>(ofdft_plan_s*, const ofdft_lwt_params&, const double*, int, const T*,         \
                                     const T*, const T*, double, double, T*, double*);
+
+namespace ofdft_tu {
+// the TN inner loop of every plan of a batch in one persistent cooperative
+// launch (defined in tn_batch.h, instantiated in ofdft_tu_batch_f*.hip)
+template <typename T>
+int tn_batch_run(ofdft_batch_s* B, const T* const* phi, const T* const* res, const T* const* W0, T* const* d,
+                 T* const* work, int maxfun, int* niter, int* status, double* pAp_last);
+}  // namespace ofdft_tu

@@ -162,6 +162,10 @@ class _Work:
         return self._tn
 
 
+class _State:
+    """Loop state of one optimize_rho (the locals of optimization.py:283-525)."""
+
+
 class Optimization:
     def __init__(self, optimization_method="CG-HS", optimization_options=None, EnergyEvaluator=None,
                  guess_rho=None):
@@ -247,13 +251,7 @@ class Optimization:
         precision) the inner CG runs on FP32 copies of phi, W0 and the
         residual; the direction is returned to W.p in W's precision."""
         if Wi is not None:
-            if W.ev.configured_rho0 is not None:
-                Wi.ev.ensure_kernel(W.ev.configured_rho0)  # same WT kernel as the outer evaluator
-            Wi.P.convert_from(W.phi, Wi.phi)
-            Wi.P.convert_from(W.res, Wi.res)
-            if Wi.W0 is None:
-                Wi.W0 = Wi.e()
-            Wi.P.convert_from(W.W0, Wi.W0)
+            self._tn_inner_inputs(W, Wi)
             nd = self._tn_loop(Wi, mu)
             W.P.convert_from(Wi.tn()["d"], W.p)
             return nd
@@ -275,6 +273,17 @@ class Optimization:
         nd = self._tn_loop(W, mu)
         W.p.copy_(W.tn()["d"])
         return nd
+
+    def _tn_inner_inputs(self, W, Wi):
+        """FP32 copies of phi, the residual and the Hessian diagonal W0 for
+        the mixed-precision inner CG, with the outer evaluator's WT kernel."""
+        if W.ev.configured_rho0 is not None:
+            Wi.ev.ensure_kernel(W.ev.configured_rho0)  # same WT kernel as the outer evaluator
+        Wi.P.convert_from(W.phi, Wi.phi)
+        Wi.P.convert_from(W.res, Wi.res)
+        if Wi.W0 is None:
+            Wi.W0 = Wi.e()
+        Wi.P.convert_from(W.W0, Wi.W0)
 
     def _tn_loop(self, W, mu):
         P = W.P
@@ -473,6 +482,20 @@ class Optimization:
 
     # ------------------------------------------------------------------
     def optimize_rho(self, guess_rho=None, guess_phi=None, lphi=False):
+        """optimize_rho (optimization.py:283-525): the outer loop on phi."""
+        self._begin(guess_rho, guess_phi, lphi)
+        st = self._st
+        for it in range(1, self.optimization_options["maxiter"]):
+            nd = self._direction(st)
+            if self._advance(st, it, nd):
+                break
+        else:
+            self._print("!WARN: Not converged, but reached max steps")
+        return self._end()
+
+    # the outer loop in three parts, so that a batch driver can interleave
+    # the iterations of many cells (batch.solve_batch_persistent)
+    def _begin(self, guess_rho=None, guess_phi=None, lphi=False):
         if lphi:
             raise AttributeError("lphi=True is not on the device path")
         if guess_rho is None and self.rho is None and guess_phi is None:
@@ -489,16 +512,17 @@ class Optimization:
             raise ValueError("precision must be fp64, fp32 or mixed")
         self._works = {}
         self._ii_energy = self._ii()
-        phase1 = prec == "mixed" and bool(o.get("mixed_phase1"))
-        cur = "fp32" if (prec == "fp32" or phase1) else "fp64"
-        W = self._work(cur)
+        st = _State()
+        st.phase1 = prec == "mixed" and bool(o.get("mixed_phase1"))
+        st.cur = "fp32" if (prec == "fp32" or st.phase1) else "fp64"
+        st.W = W = self._work(st.cur)
         # mixed precision: FP64 outer iterations with the TN inner CG in FP32
-        Wi = None
-        if prec == "mixed" and not phase1 and self.optimization_method == "TN" and o["hvp"] == "exact":
-            Wi = self._work("fp32")
-        for w in ([W, Wi] if Wi is not None else [W]):
+        st.Wi = None
+        if prec == "mixed" and not st.phase1 and self.optimization_method == "TN" and o["hvp"] == "exact":
+            st.Wi = self._work("fp32")
+        for w in ([W, st.Wi] if st.Wi is not None else [W]):
             w.P.bind_stream(True)
-        t0 = time.perf_counter()
+        st.t0 = time.perf_counter()
         with torch._C.DisableTorchFunctionSubclass():
             if guess_phi is None:
                 rho = self.rho.to(torch.float64)
@@ -510,124 +534,141 @@ class Optimization:
             W.phi.copy_(phi.to(W.P.dtype))
             # electron count fixed in FP64 from the initial phi (optimization.py:305-315)
             self._nelec = float((phi * phi).sum()) * W.P.dV
-        E, mu, resN, sums = self._start(W)
-        _finite(E, "energy")
-        _finite(resN, "residual norm")
-        hist = [E]
-        self.log = [dict(step=0, energy=E, dE=E, dP=resN, Nd=1, Nls=1, time=time.perf_counter() - t0, prec=cur)]
+        st.E, st.mu, st.resN, st.sums = self._start(W)
+        _finite(st.E, "energy")
+        _finite(st.resN, "residual norm")
+        st.hist = [st.E]
+        self.log = [dict(step=0, energy=st.E, dE=st.E, dP=st.resN, Nd=1, Nls=1, time=time.perf_counter() - st.t0,
+                         prec=st.cur)]
         self._print("{:8s}{:24s}{:16s}{:16s}{:8s}{:8s}{:16s}".format("Step", "Energy(a.u.)", "dE", "dP", "Nd", "Nls",
                                                                       "Time(s)"))
-        self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(0, E, E, resN, 1, 1,
+        self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(0, st.E, st.E, st.resN, 1, 1,
                                                                                    self.log[-1]["time"]))
-        theta = 0.1
-        converged = 2
+        st.theta = 0.1
+        st.converged = 2
         self._lbfgs = dict(s=[], y=[], rho=[])
         self._dir_prev = None
-        first = True
-        switch = o["mixed_switch"]
-        if switch is None:
-            switch = 1e-8 * abs(E)
-        for it in range(1, o["maxiter"]):
-            m = self.optimization_method
-            if m == "TN":
-                nd = self._direction_tn(W, mu, Wi if cur == "fp64" else None)
-            elif m == "LBFGS":
-                nd = self._direction_lbfgs(W)
-            elif m == "GD":
-                W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
-                nd = 1
-            else:
-                nd = self._direction_cg(W, first)
+        st.first = True
+        st.switch = o["mixed_switch"]
+        if st.switch is None:
+            st.switch = 1e-8 * abs(st.E)
+        self._st = st
+
+    def _direction(self, st):
+        """The search direction into st.W.p; returns Nd."""
+        m = self.optimization_method
+        W = st.W
+        if m == "TN":
+            return self._direction_tn(W, st.mu, st.Wi if st.cur == "fp64" else None)
+        if m == "LBFGS":
+            return self._direction_lbfgs(W)
+        if m == "GD":
+            W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
+            return 1
+        return self._direction_cg(W, st.first)
+
+    def _advance(self, st, it, nd):
+        """Orthonormalisation, line search and update of one outer iteration
+        (optimization.py:363-516) after the direction; True when the loop
+        stops (converged, or the line search failed)."""
+        o = self.optimization_options
+        m = self.optimization_method
+        W = st.W
+        theta0, vphip = self._orth_norm(W)
+        if vphip * 2.0 > 0:
+            self._print("!WARN: Change to steepest decent")
+            W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
             theta0, vphip = self._orth_norm(W)
-            if vphip * 2.0 > 0:
-                self._print("!WARN: Change to steepest decent")
-                W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
-                theta0, vphip = self._orth_norm(W)
-            theta = min(theta0, theta)
-            g0 = vphip * 2.0 * W.P.dV
-            # an FP32 line search that cannot converge is cut short: the
-            # outer loop then continues in FP64
-            stp, nls, cache = self._line_search(W, E, g0, theta, 8 if (phase1 and cur == "fp32") else 100)
-            failed = stp is None
-            if failed and not (phase1 and cur == "fp32"):
-                converged = 1
-                self._print("!!!ERROR : Line-Search Failed!!!")
-                self._print("!!!ERROR : Density Optimization NOT Converged  !!!")
-                break
-            if not failed:
-                theta = stp
-                t = cache["t"]
-                sums = cache["sums"]
-                old_phi = None
-                if m == "LBFGS":
-                    old_phi = W.phi.clone()
-                W.P.lincomb(math.cos(t), W.phi, math.sin(t), W.p, W.phi)
-                W.V, W.Vt = W.Vt, W.V
-                W.aux, W.auxt = W.auxt, W.aux
-                Enew = _finite(self._energy(W, sums), "energy")
-                mu = sums[L.S_VRHO] / sums[L.S_RHO]
-                W.res_prev.copy_(W.res)
-                resN = _finite(self._residual(W, mu), "residual norm")
-                if m == "LBFGS":
-                    self._lbfgs_update(W, old_phi)
-                if m.startswith("CG"):
-                    if self._dir_prev is None:
-                        self._dir_prev = W.e()
-                    self._dir_prev.copy_(W.p)
-                first = False
-                hist.append(Enew)
-                E = Enew
-                self.log.append(dict(step=it, energy=E, dE=hist[-1] - hist[-2], dP=resN, Nd=nd, Nls=nls,
-                                     time=time.perf_counter() - t0, prec=cur))
-                self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
-                    it, E, hist[-1] - hist[-2], resN, nd, nls, self.log[-1]["time"]))
-            # mixed precision: leave FP32 near its noise floor
-            if phase1 and cur == "fp32" and (failed or (len(hist) > 1 and abs(hist[-1] - hist[-2]) < switch)):
-                W32 = W
-                cur = "fp64"
-                W = self._work("fp64")
-                W.P.bind_stream(True)
-                Wi = self._work("fp32") if (self.optimization_method == "TN" and o["hvp"] == "exact") else None
-                W.P.convert_from(W32.phi, W.phi)
-                # FP64 electron-count constraint: FP32 rotations drift the norm
-                nphi = W.P.dot3(W.phi, W.phi)[1] * W.P.dV
-                W.P.lincomb(math.sqrt(self._nelec / nphi), W.phi, 0.0, W.phi, W.phi)
-                E32 = hist[-1]
-                E, mu, resN, sums = self._start(W)
-                _finite(E, "energy")
-                # the convergence test restarts on FP64 energies
-                hist = [E]
-                theta = 0.1
-                first = True
-                self._lbfgs = dict(s=[], y=[], rho=[])
-                self._dir_prev = None
-                self.log.append(dict(step=it, energy=E, dE=E - E32, dP=resN, Nd=0, Nls=0,
-                                     time=time.perf_counter() - t0, prec="switch->fp64"))
-                self._print("#### FP32 -> FP64 at step %d, E = %.12E" % (it, E))
-                continue
-            if self._check_converge(hist):
-                converged = 0
-                self._print("#### Density Optimization Converged ####")
-                break
-        else:
-            self._print("!WARN: Not converged, but reached max steps")
+        st.theta = min(theta0, st.theta)
+        g0 = vphip * 2.0 * W.P.dV
+        # an FP32 line search that cannot converge is cut short: the outer
+        # loop then continues in FP64
+        stp, nls, cache = self._line_search(W, st.E, g0, st.theta, 8 if (st.phase1 and st.cur == "fp32") else 100)
+        failed = stp is None
+        if failed and not (st.phase1 and st.cur == "fp32"):
+            st.converged = 1
+            self._print("!!!ERROR : Line-Search Failed!!!")
+            self._print("!!!ERROR : Density Optimization NOT Converged  !!!")
+            return True
+        if not failed:
+            st.theta = stp
+            t = cache["t"]
+            sums = cache["sums"]
+            old_phi = None
+            if m == "LBFGS":
+                old_phi = W.phi.clone()
+            W.P.lincomb(math.cos(t), W.phi, math.sin(t), W.p, W.phi)
+            W.V, W.Vt = W.Vt, W.V
+            W.aux, W.auxt = W.auxt, W.aux
+            Enew = _finite(self._energy(W, sums), "energy")
+            st.mu = sums[L.S_VRHO] / sums[L.S_RHO]
+            W.res_prev.copy_(W.res)
+            st.resN = _finite(self._residual(W, st.mu), "residual norm")
+            if m == "LBFGS":
+                self._lbfgs_update(W, old_phi)
+            if m.startswith("CG"):
+                if self._dir_prev is None:
+                    self._dir_prev = W.e()
+                self._dir_prev.copy_(W.p)
+            st.first = False
+            st.hist.append(Enew)
+            st.E = Enew
+            hist = st.hist
+            self.log.append(dict(step=it, energy=st.E, dE=hist[-1] - hist[-2], dP=st.resN, Nd=nd, Nls=nls,
+                                 time=time.perf_counter() - st.t0, prec=st.cur))
+            self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
+                it, st.E, hist[-1] - hist[-2], st.resN, nd, nls, self.log[-1]["time"]))
+        # mixed precision: leave FP32 near its noise floor
+        if st.phase1 and st.cur == "fp32" and (failed or (len(st.hist) > 1 and
+                                                          abs(st.hist[-1] - st.hist[-2]) < st.switch)):
+            W32 = W
+            st.cur = "fp64"
+            st.W = W = self._work("fp64")
+            W.P.bind_stream(True)
+            st.Wi = self._work("fp32") if (m == "TN" and o["hvp"] == "exact") else None
+            W.P.convert_from(W32.phi, W.phi)
+            # FP64 electron-count constraint: FP32 rotations drift the norm
+            nphi = W.P.dot3(W.phi, W.phi)[1] * W.P.dV
+            W.P.lincomb(math.sqrt(self._nelec / nphi), W.phi, 0.0, W.phi, W.phi)
+            E32 = st.hist[-1]
+            st.E, st.mu, st.resN, st.sums = self._start(W)
+            _finite(st.E, "energy")
+            # the convergence test restarts on FP64 energies
+            st.hist = [st.E]
+            st.theta = 0.1
+            st.first = True
+            self._lbfgs = dict(s=[], y=[], rho=[])
+            self._dir_prev = None
+            self.log.append(dict(step=it, energy=st.E, dE=st.E - E32, dP=st.resN, Nd=0, Nls=0,
+                                 time=time.perf_counter() - st.t0, prec="switch->fp64"))
+            self._print("#### FP32 -> FP64 at step %d, E = %.12E" % (it, st.E))
+            return False
+        if self._check_converge(st.hist):
+            st.converged = 0
+            self._print("#### Density Optimization Converged ####")
+            return True
+        return False
+
+    def _end(self):
+        st = self._st
         for w in self._works.values():
             w.P.bind_stream(False)
-        self.mu = mu
-        self._print("Chemical potential (a.u.):", mu)
-        self._print("Chemical potential (eV)  :", mu * HARTREE_EV)
+        self.mu = st.mu
+        self._print("Chemical potential (a.u.):", st.mu)
+        self._print("Chemical potential (eV)  :", st.mu * HARTREE_EV)
         grid = self.rho.grid
+        W = st.W
         with torch._C.DisableTorchFunctionSubclass():
             phi_out = W.phi.clone()
             rho_out = phi_out * phi_out
             Vout = W.V.clone()
         self.phi = DirectField(grid=grid, data=phi_out, dtype=phi_out.dtype)
         self.rho = DirectField(grid=grid, data=rho_out, dtype=rho_out.dtype)
-        self.functional = FunctionalOutput(name="TOTAL", energy=E,
+        self.functional = FunctionalOutput(name="TOTAL", energy=st.E,
                                            potential=DirectField(grid=grid, data=Vout, dtype=Vout.dtype))
-        self.energy = E
-        self.converged = converged
-        self.elapsed = time.perf_counter() - t0
+        self.energy = st.E
+        self.converged = st.converged
+        self.elapsed = time.perf_counter() - st.t0
         return self.rho
 
     def _lbfgs_update(self, W, old_phi):

@@ -107,6 +107,27 @@ int ofdft_plan_set_graph(ofdft_plan plan, int on);
  * applied by dftpy_amd at plan creation.  Results do not depend on them.
  * No counterpart in the reference (numpy.fft has no tiling).              */
 int ofdft_plan_set_tiles(ofdft_plan plan, int tile_axis, int tile_pencil);
+
+/* Batched TN inner loop over many independent cells (the C5 batching form;
+ * the reference solves each cell with its own optimize_rho,
+ * optimization.py:283-525, whose get_direction_TN :92-148 this runs for
+ * all cells at once).  A batch is a list of plans -- one per cell, each
+ * with its own grid length, lattice, functional, WT table and V_loc -- that
+ * share one precision and one stream; lengths must be cubic and in the
+ * compile-time set 32, 48, 64, 90, 96, 108, 128, 160, 192.
+ * ofdft_batch_tn_direction runs ofdft_tn_direction for every cell c
+ * (phi[c], res[c], W0[c], d[c], work[c]: the same device arrays) in ONE
+ * persistent cooperative launch: each stage of the CG iteration is a work
+ * queue over (cell, tile) items for all cells, stages separated by grid
+ * barriers, until every cell's loop has ended.  niter, status, pAp_last:
+ * host arrays of nb.  The batch keeps pointers to the plans: destroy the
+ * batch before them.  ofdft_batch_create validates the plans (host only).  */
+typedef struct ofdft_batch_s* ofdft_batch;
+int ofdft_batch_create(ofdft_batch* out, const ofdft_plan* plans, int nb);
+int ofdft_batch_destroy(ofdft_batch batch);
+int ofdft_batch_tn_direction(ofdft_batch batch, const void* const* phi, const void* const* res,
+                             const void* const* W0, void* const* d, void* const* work, int maxfun, int* niter,
+                             int* status, double* pAp_last);
 int ofdft_plan_set_stream(ofdft_plan plan, void* hip_stream);
 /* Device bytes of workspace the plan owns. */
 long long ofdft_plan_workspace_bytes(ofdft_plan plan);
