@@ -10,7 +10,16 @@ fields and transforms, FP64 reductions) until |dE| < 1e-8 |E|, then FP64
 outer iterations (FP64 energies, potential, line search, electron-count
 constraint) with the TN inner CG kept in FP32, to econv.  ``value`` is TN outer
 iterations per second over the timed solves.  The same solve in FP64 is run
-once after the timed region for the mixed/FP64 ratio and the energy error.
+once after the timed region for the mixed/FP64 ratio and the energy error;
+both energies are also compared with the oracle's converged minimum of the
+workload (tests/golden/oracle_converged.json).
+
+After that, outside the timed region: the batching comparison (64 uneven
+independent cells, persistent lockstep launch and concurrent streams, each
+cell's error against its own FP64 solve and, where committed, the oracle),
+and the CPU baseline: the first TN outer iteration of the oracle on the same
+workload with one FFT thread and with the usable cores, next to the full
+oracle solve timed in the build container.
 
 Prints one JSON line (rank 0 of one GPU).
 """
@@ -26,11 +35,15 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM traffic per launch of a kernel class, measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE on the named workload and scaled by what the same
-# counters read for a plain device copy (FETCH x 2.00, WRITE x 1.00); the
-# summary and commands are in profiles/axis_c2c_fp32_al864_192.md.
+# HBM traffic per launch of a kernel class: a calibrated constant, not
+# measured by this script.  It comes from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE runs of this script on the named workload, scaled by what the
+# same counters read for a plain device copy (FETCH x 2.00, WRITE x 1.00);
+# the summary, the commands and the source revision they ran on are in
+# profiles/axis_c2c_fp32_al864_192.md.
 MEASURED_TRAFFIC = {("al864_192", "fp32", "axis_c2c"): 198.3e6}
+TRAFFIC_SOURCE = ("calibrated constant: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, copy-kernel scaled "
+                  "(profiles/axis_c2c_fp32_al864_192.md)")
 
 
 def parse():
@@ -42,12 +55,15 @@ def parse():
     ap.add_argument("--econv", type=float, default=1e-8)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison solve")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-evals", type=int, default=20, help="oracle evaluations timed for the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="FFT threads of the all-cores CPU baseline sample (default: the usable cores, at most 16)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-phase1", dest="phase1", action="store_false",
                     help="mixed mode without the FP32 outer phase (FP64 outer loop from the start)")
     ap.add_argument("--batch", type=int, default=0, metavar="NCELLS",
-                    help="batch mode: solve NCELLS seeded independent Al/Mg cells concurrently and report solves/s")
+                    help="batch mode only: solve NCELLS seeded independent Al/Mg cells concurrently, report solves/s")
+    ap.add_argument("--batch-cells", type=int, default=64,
+                    help="cells of the batching comparison added to the main line (0: skip it)")
     ap.add_argument("--workers", type=int, default=8, help="concurrent streams in batch mode")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes 5 times (counter calibration) and exit")
@@ -92,68 +108,118 @@ def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact", tn_g
     return opt, niter
 
 
-def cpu_baseline(wl, nevals, niter_evals):
-    """Oracle (numpy FP64, single-threaded pocketfft) evaluations on the
-    same 192^3 workload; converted to TN iterations/s with the evaluations
-    per TN iteration of the GPU FP64 solve."""
-    import numpy as np
+def host_cores():
+    """(cores this process may run on, cores of the host)."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
+def cpu_baseline(wl, threads):
+    """The oracle (numpy FP64 restatement of the reference's optimize_rho, TN
+    with its forward-difference Hessian product) timed on the host cores:
+    the first TN outer iteration (initial evaluation, inner CG, line search)
+    from the same start on the same workload, ``threads`` FFT worker threads
+    (the pointwise numpy work is single-threaded either way).  Set-up (V_loc,
+    Ewald, the WT kernel table) is outside the sample."""
     import oracle as O
+    from dftpy_amd import workloads as W
+    O.set_fft_workers(threads)
     og = O.Grid(wl["cell"], wl["nr"], full=False)
     oi = O.Ions(wl["symbols"], wl["positions"], wl["cell"])
-    tabs = {s: O.empty_core_table(*__import__("dftpy_amd.workloads", fromlist=["x"]).EMPTY_CORE[s])
-            for s in wl["tables"]}
-    opp = O.LocalPseudo(og, oi, tabs, PME=True)
-    tot = O.TotalFunctional(og, kedf=wl["kedf"], kedf_options=wl["kedf_options"], pseudo=opp)
-    rho = wl["rho_ini"]
-    opp.vreal  # set-up outside the timed sample
-    tot.compute(rho)  # warm-up (kernel table)
+    opp = O.LocalPseudo(og, oi, {s: O.empty_core_table(*W.EMPTY_CORE[s]) for s in wl["tables"]}, PME=True)
+    tot = O.TotalFunctional(og, kedf=wl["kedf"], kedf_options=wl["kedf_options"], pseudo=opp,
+                            ewald=O.Ewald(og, oi, PME=True))
+    tot.ewald.energy
+    opp.vreal
+    tot.compute(wl["rho_ini"])  # builds the kernel table
+    opt = O.Optimization(lambda r, c, p: tot.compute(r, c, phi=p), og, "TN", {"econv": 1e-9, "maxiter": 2})
     t0 = time.perf_counter()
-    for _ in range(nevals):
-        tot.compute(rho)
+    opt.optimize_rho(wl["rho_ini"])
     dt = time.perf_counter() - t0
-    evals_per_s = nevals / dt
-    return dict(value=evals_per_s / niter_evals, unit="TN iterations/s", cores=1, kind="port",
-                sample="%d FP64 E+V evaluations of the oracle (numpy, 1 thread) on the same %s workload (%d^3, %s) in "
-                       "%.1f s = %.4f evals/s, divided by %.2f evaluations per TN iteration (GPU FP64 solve log)"
-                       % (nevals, wl["name"], wl["nr"][0], wl["kedf"], dt, evals_per_s, niter_evals))
+    it = len(opt.log) - 1
+    O.set_fft_workers(1)
+    return dict(value=it / dt, unit="TN iterations/s", cores=threads, kind="port",
+                sample="first TN outer iteration of the oracle (%d functional evaluations: Nd=%d, Nls=%d) on the "
+                       "same %s workload (%d^3, %s) from the same start, %d FFT thread(s): %.1f s"
+                       % (opt.nfeval, opt.log[1][4], opt.log[1][5], wl["name"], wl["nr"][0], wl["kedf"], threads, dt))
 
 
-def batch_main(args):
-    """Batching configuration: independent cubic Al/Mg cells (32-500 atoms,
-    grids 48^3-108^3 at 0.2 A), mixed precision, solved concurrently on
-    ``--workers`` streams; the error of each cell is measured against an FP64
-    solve of the same cell run on its own."""
+def committed_oracle_solve(name):
+    """The full oracle TN solve of the workload timed in the build container
+    (tests/golden/oracle_converged.json, written by
+    tests/golden/make_converged.py): energy, iterations, wall time, CPU."""
+    path = os.path.join(ROOT, "tests", "golden", "oracle_converged.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        g = json.load(fh).get(name)
+    if not g:
+        return None
+    return {"energy_Ha": g["energy_Ha"], "econv_Ha": g["econv_Ha"], "iterations": g["iterations"],
+            "evaluations": g["evaluations"], "wall_s": g["wall_s"], "value": g["tn_iterations_per_s"],
+            "unit": "TN iterations/s", "fft_threads": g["fft_threads"], "cpu": g["cpu"], "host_cores": g["cpu_count"]}
+
+
+def batch_compare(ncells, econv, workers):
+    """The batching configuration (``ncells`` independent cubic Al/Mg cells,
+    32-500 atoms, grids 48^3-108^3, mixed precision) solved both ways:
+    ``persistent`` (lockstep, one cooperative launch per TN outer iteration
+    running the inner CG loops of every cell) and ``streams`` (worker
+    threads, one HIP stream each).  Every cell's energy is compared with an
+    FP64 solve of that cell on its own, and the cells with a committed oracle
+    minimum (one per grid size) with it."""
     import numpy as np
-    import torch
     from dftpy_amd import batch as B
     from dftpy_amd import workloads as W
-    cells = W.batch_cells(args.batch)
-    # warm-up: one cell of every grid size, so that no first-launch cost is timed
+    cells = W.batch_cells(ncells)
     seen = {}
     for c in cells:
         seen.setdefault(c["nr"], c)
-    B.solve_batch(list(seen.values()), "mixed", workers=len(seen), econv=args.econv)
-    res, wall = B.solve_batch(cells, "mixed", workers=args.workers, econv=args.econv)
-    ref = []
+    warm = list(seen.values())
+    # warm-up: one cell of every grid size in each form (first launches, kernel tables)
+    B.solve_batch_persistent(warm, econv=econv)
+    B.solve_batch(warm, "mixed", workers=len(warm), econv=econv)
+    res_p, wall_p = B.solve_batch_persistent(cells, econv=econv)
+    res_s, wall_s = B.solve_batch(cells, "mixed", workers=workers, econv=econv)
     t0 = time.perf_counter()
-    for c in cells:
-        ref.append(B.solve_cell(c, "fp64", econv=args.econv))
+    ref = [B.solve_cell(c, "fp64", econv=econv) for c in cells]
     t64 = time.perf_counter() - t0
-    err = np.array([abs(a["energy"] - b["energy"]) * W.HARTREE / a["nat"] for a, b in zip(res, ref)])
+    gold = {}
+    gpath = os.path.join(ROOT, "tests", "golden", "oracle_converged.json")
+    if os.path.exists(gpath):
+        with open(gpath) as fh:
+            gold = json.load(fh)
+
+    def errors(res):
+        e64 = np.array([abs(a["energy"] - b["energy"]) * W.HARTREE / a["nat"] for a, b in zip(res, ref)])
+        eo = [abs(a["energy"] - gold[c["name"]]["energy_Ha"]) * W.HARTREE / a["nat"]
+              for a, c in zip(res, cells) if c["name"] in gold]
+        return {"vs_fp64_max": float(e64.max()), "vs_fp64_mean": float(e64.mean()),
+                "vs_oracle_max": float(max(eo)) if eo else None, "oracle_anchored_cells": len(eo),
+                "all_converged": all(r["converged"] == 0 for r in res)}
+
     sizes = {}
     for c in cells:
-        k = "%d atoms %d^3" % (len(c["symbols"]), c["nr"][0])
+        k = "%d^3" % c["nr"][0]
         sizes[k] = sizes.get(k, 0) + 1
-    out = {
-        "metric": "independent-cell TN solves/sec, mixed precision, concurrent streams",
-        "value": len(cells) / wall, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-        "wall_s": wall, "workers": args.workers, "cells": sizes, "econv_Ha": args.econv,
-        "all_converged": all(r["converged"] == 0 for r in res),
-        "tn_iterations_total": int(sum(r["iterations"] for r in res)),
-        "fp64_one_at_a_time": {"value": len(cells) / t64, "unit": "solves/s", "wall_s": t64},
-        "error_eV_per_atom": {"max": float(err.max()), "mean": float(err.mean())},
-        "data": "synthetic: workloads.batch_cells(%d, seed=11)" % len(cells),
-    }
+    return {"metric": "independent-cell TN solves/sec, mixed precision", "cells": len(cells), "grids": sizes,
+            "econv_Ha": econv, "data": "synthetic: workloads.batch_cells(%d, seed=11)" % len(cells),
+            "persistent": {"value": len(cells) / wall_p, "unit": "solves/s", "wall_s": wall_p,
+                           "tn_iterations_total": int(sum(r["iterations"] for r in res_p)), "error_eV_per_atom":
+                           errors(res_p)},
+            "streams": {"value": len(cells) / wall_s, "unit": "solves/s", "wall_s": wall_s, "workers": workers,
+                        "tn_iterations_total": int(sum(r["iterations"] for r in res_s)), "error_eV_per_atom":
+                        errors(res_s)},
+            "fp64_one_at_a_time": {"value": len(cells) / t64, "unit": "solves/s", "wall_s": t64}}
+
+
+def batch_main(args):
+    """Batching configuration only: prints its own JSON line."""
+    out = batch_compare(args.batch, args.econv, args.workers)
+    out.update({"value": out["persistent"]["value"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True})
     print(json.dumps(out), flush=True)
 
 
@@ -239,8 +305,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "%s (%s)" % (cls_d, prec_d), "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((args.workload, prec_d, cls_d)),
+                     "traffic_source": TRAFFIC_SOURCE if (args.workload, prec_d, cls_d) in MEASURED_TRAFFIC else None,
                      "launches": cnt_d, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "all_kernels_GBps": total_bytes / (total_dev_ms / 1e3) / 1e9 if total_dev_ms else None},
+        # algorithmic bytes of every kernel of one solve over the solve's wall time
+        "whole_solve": {"algorithmic_bytes": total_bytes, "seconds": elapsed / args.steps,
+                        "GBps": total_bytes / (elapsed / args.steps) / 1e9,
+                        "frac": total_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
+                        "device_busy_frac": total_dev_ms / 1e3 / (elapsed / args.steps)},
         # one warm-up solve, every class timed: [total ms, launches, us per launch, GB/s]
         "profile": {p: {k: [round(v[0], 3), v[1], round(v[0] / v[1] * 1e3, 1),
                             round(v[2] / v[1] / (v[0] / v[1] / 1e3) / 1e9, 0)] for k, v in pr.items() if v[1]}
@@ -260,6 +332,10 @@ def main():
     tg = time.perf_counter() - tg
     out["tn_graph"] = {"value": giters / tg, "unit": "TN iterations/s", "ms_per_step": tg / args.steps * 1e3,
                        "same_energy": bool(genergy == energies[-1])}
+    gold = committed_oracle_solve(args.workload)
+    if gold is not None:
+        out["oracle_energy_Ha"] = gold["energy_Ha"]
+        out["mixed_energy_error_vs_oracle_eV_per_atom"] = abs(energies[-1] - gold["energy_Ha"]) * W.HARTREE / nat
     if not args.no_fp64:
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -270,11 +346,12 @@ def main():
         err = abs(energies[-1] - o64.energy) * W.HARTREE / nat
         out["fp64_gpu"] = {"value": fp64_rate, "unit": "TN iterations/s", "iterations": n64, "seconds": t64,
                            "energy_Ha": o64.energy, "converged": o64.converged}
+        if gold is not None:
+            out["fp64_gpu"]["energy_error_vs_oracle_eV_per_atom"] = abs(o64.energy - gold["energy_Ha"]) * W.HARTREE / nat
         out["mixed_over_fp64"] = value / fp64_rate
         out["time_to_solution_ratio"] = t64 / (elapsed / args.steps)
         out["mixed_energy_error_eV_per_atom"] = err
         out["mixed_energy_Ha"] = energies[-1]
-        evals = sum(l["Nd"] + l["Nls"] for l in o64.log[1:]) / max(n64, 1)
         # the reference's own TN on the FP64 kernels: the Hessian product as
         # a forward difference of the potential (optimization.py:106-114)
         torch.cuda.synchronize()
@@ -287,10 +364,17 @@ def main():
                               "hvp": "forward difference of V (the reference's TN), FP64 kernels"}
         out["mixed_over_fp64_fd"] = value / (nfd / tfd)
         out["time_to_solution_ratio_fd"] = tfd / (elapsed / args.steps)
-    else:
-        evals = sum(l["Nd"] + l["Nls"] for l in logs[-1][1:]) / max(iters / args.steps, 1)
+    if args.batch_cells:
+        out["batch"] = batch_compare(args.batch_cells, args.econv, args.workers)
+    usable, total = host_cores()
+    out["host_cores"] = {"usable": usable, "total": total}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_evals, evals)
+        out["cpu_baseline"] = cpu_baseline(wl, 1)
+        nthr = args.cpu_threads or min(usable, 16)
+        if nthr > 1:
+            out["cpu_baseline_all_cores"] = cpu_baseline(wl, nthr)
+        if gold is not None:
+            out["cpu_full_solve_committed"] = gold
     print(json.dumps(out), flush=True)
 
 

@@ -4,7 +4,8 @@ Two forms.  ``solve_batch_persistent`` runs the cells in lockstep: every TN
 outer iteration runs the inner CG loops of all cells in ONE persistent
 cooperative launch (``ofdft_batch_tn_direction``: each stage of a CG
 iteration is a work queue over (cell, tile) items of every cell, stages
-separated by grid barriers), then each cell's line search and update.
+separated by grid barriers), then each cell's line search and update on
+worker threads and streams.
 ``solve_batch`` hands the cells to worker threads instead: each worker thread owns one HIP stream (torch.cuda.Stream, default
 priority) and solves its share of the cells with the ordinary
 ``Optimization``; every plan a worker creates is bound to that worker's
@@ -106,59 +107,107 @@ class TnBatch:
         return list(niter), list(status), list(pap)
 
 
-def solve_batch_persistent(cells, econv=1e-8, options=None):
+def solve_batch_persistent(cells, econv=1e-8, options=None, workers=8):
     """Solve all cells in lockstep, mixed precision (FP64 outer iterations,
     FP32 inner CG with the exact Hessian product): per TN outer iteration,
     one persistent launch runs the inner loops of every unconverged cell,
     then each cell takes its line search and update (the reference's
-    optimize_rho per cell, optimization.py:283-525).  Returns the per-cell
-    results (input order) and the wall time."""
+    optimize_rho per cell, optimization.py:283-525).
+
+    The per-cell outer work (line search, update, residual) runs on
+    ``workers`` host threads, each with its own HIP stream: the outer plans
+    of cell k are bound to stream k mod workers, the inner (FP32) plans of
+    every cell to the calling stream, on which the batched launch runs.  The
+    streams meet at a device synchronisation before each batched launch (the
+    launch reads the outer fields) and at the launch's own completion.
+    Returns the per-cell results (input order) and the wall time."""
+    from concurrent.futures import ThreadPoolExecutor
     from .optimization import _finite
+    if workers < 1:
+        raise ValueError("workers must be positive")
     opts_common = {"econv": econv, "precision": "mixed", "mixed_phase1": False, "maxiter": 200, "hvp": "exact",
                    "tn_graph": False}
     if options:
         opts_common.update(options)
-    built = []
-    for c in cells:
+    if opts_common["mixed_phase1"]:
+        raise ValueError("the lockstep batch runs FP64 outer iterations from the start (mixed_phase1=False)")
+    device = torch.cuda.current_device()
+    main = torch.cuda.current_stream()
+    nw = max(1, min(workers, len(cells)))
+    streams = [torch.cuda.Stream() for _ in range(nw)]
+    groups = [list(range(g, len(cells), nw)) for g in range(nw)]
+    built = [None] * len(cells)
+
+    def on_workers(fn, ids=None):
+        """fn(i) for every cell i (of ``ids``), cell i on its worker stream."""
+        want = None if ids is None else set(ids)
+
+        def run(g):
+            torch.cuda.set_device(device)
+            out = []
+            with torch.cuda.stream(streams[g]):
+                for i in groups[g]:
+                    if want is None or i in want:
+                        out.append((i, fn(i)))
+            return out
+
+        res = {}
+        for part in pool.map(run, range(nw)):
+            res.update(part)
+        return res
+
+    def build(i):
+        c = cells[i]
         grid, total = build_cell(c)
         total.ewald.energy
         rho = DirectField(grid=grid, data=c["rho_ini"])
         opt = Optimization(EnergyEvaluator=total, optimization_method="TN", optimization_options=dict(opts_common))
         opt.verbose = False
-        built.append((c, opt, rho))
+        built[i] = (c, opt, rho)
+
+    with ThreadPoolExecutor(nw) as pool:
+        on_workers(build)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        on_workers(lambda i: built[i][1]._begin(guess_rho=built[i][2]))
+        # the inner plans of every cell go on the batch (calling) stream
+        for c, opt, rho in built:
+            opt._st.Wi.P.bind_stream(True)
+        maxiter = opts_common["maxiter"]
+        maxfun = opts_common.get("maxfun", 50)
+        active = list(range(len(built)))
+        batch, batch_key = None, None
+        it = 0
+        while active and it < maxiter - 1:
+            it += 1
+            torch.cuda.synchronize()  # the outer fields of every cell are final
+            for i in active:
+                st = built[i][1]._st
+                built[i][1]._tn_inner_inputs(st.W, st.Wi)
+            if tuple(active) != batch_key:
+                batch = TnBatch([built[i][1]._st.Wi.P for i in active])
+                batch_key = tuple(active)
+            niter, status, pap = batch.direction([built[i][1]._st.Wi for i in active], maxfun)
+            nd = {}
+            for k, i in enumerate(active):
+                _finite(pap[k], "p.Ap in the truncated-Newton loop")
+                nd[i] = niter[k]
+
+            def step(i):
+                opt = built[i][1]
+                st = opt._st
+                st.W.P.convert_from(st.Wi.tn()["d"], st.W.p)
+                opt.nhvp += nd[i]
+                return opt._advance(st, it, nd[i])
+
+            done = on_workers(step, active)
+            active = [i for i in active if not done[i]]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        on_workers(lambda i: built[i][1]._end())
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for c, opt, rho in built:
-        opt._begin(guess_rho=rho)
-    maxiter = opts_common["maxiter"]
-    maxfun = opts_common.get("maxfun", 50)
-    active = list(range(len(built)))
-    batch, batch_key = None, None
-    it = 0
-    while active and it < maxiter - 1:
-        it += 1
-        for i in active:
-            st = built[i][1]._st
-            built[i][1]._tn_inner_inputs(st.W, st.Wi)
-        if tuple(active) != batch_key:
-            batch = TnBatch([built[i][1]._st.Wi.P for i in active])
-            batch_key = tuple(active)
-        niter, status, pap = batch.direction([built[i][1]._st.Wi for i in active], maxfun)
-        still = []
-        for k, i in enumerate(active):
-            opt = built[i][1]
-            st = opt._st
-            _finite(pap[k], "p.Ap in the truncated-Newton loop")
-            st.W.P.convert_from(st.Wi.tn()["d"], st.W.p)
-            opt.nhvp += niter[k]
-            if not opt._advance(st, it, niter[k]):
-                still.append(i)
-        active = still
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     results = []
     for c, opt, rho in built:
-        opt._end()
         results.append(dict(name=c.get("name"), energy=opt.energy, converged=opt.converged, nat=len(c["symbols"]),
                             nr=tuple(c["nr"]), iterations=len(opt.log) - 1))
     return results, wall

@@ -75,7 +75,7 @@ __device__ __forceinline__ int batch_cell(const long long* __restrict__ pref, in
 }
 
 template <typename T, int NN>
-__device__ __forceinline__ void batch_rows_fwd(const TnCell<T>& C, long long local, cplx<T>* lds) {
+__device__ __noinline__ void batch_rows_fwd(const TnCell<T>& C, long long local, cplx<T>* lds) {
   PreHvp<T> pre{C.phi, C.p, C.fp};
   PreHvpCG<T> pcg{pre, C.phi, C.p, C.r, C.st};
   using FFT = FFTFixed<NN, kRowPB<T> * 3, kBatchTH>;
@@ -86,7 +86,7 @@ __device__ __forceinline__ void batch_rows_fwd(const TnCell<T>& C, long long loc
 }
 
 template <typename T, int NN>
-__device__ __forceinline__ void batch_rows_inv(const TnCell<T>& C, long long local, cplx<T>* lds) {
+__device__ __noinline__ void batch_rows_inv(const TnCell<T>& C, long long local, cplx<T>* lds) {
   PostHvpFast<T> h{C.phi, C.p, C.W0, nullptr, C.fp};
   PostHvpCG<T> post{h, C.d, C.r, C.best, C.st, C.red};
   using FFT = FFTFixed<NN, kRowPB<T> * 3, kBatchTH>;
@@ -99,7 +99,7 @@ __device__ __forceinline__ void batch_rows_inv(const TnCell<T>& C, long long loc
 }
 
 template <typename T, int NN>
-__device__ __forceinline__ void batch_axis(const TnCell<T>& C, long long local, bool inv, cplx<T>* lds) {
+__device__ __noinline__ void batch_axis(const TnCell<T>& C, long long local, bool inv, cplx<T>* lds) {
   const int bx = (int)(local % C.tx);
   const long long rem = local / C.tx;
   const int by = (int)(rem % C.n), bz = (int)(rem / C.n);
@@ -108,7 +108,7 @@ __device__ __forceinline__ void batch_axis(const TnCell<T>& C, long long local, 
 }
 
 template <typename T, int NN>
-__device__ __forceinline__ void batch_pencil(const TnCell<T>& C, long long local, cplx<T>* lds) {
+__device__ __noinline__ void batch_pencil(const TnCell<T>& C, long long local, cplx<T>* lds) {
   const int bx = (int)(local % C.tx);
   const long long rem = local / C.tx;
   const int by = (int)(rem % C.n), bz = (int)(rem / C.n);
@@ -130,8 +130,14 @@ __device__ __forceinline__ void batch_pencil(const TnCell<T>& C, long long local
     default: break;                     \
   }
 
+// Waves per SIMD the register allocation is held to: the stage functions are
+// called (not inlined), so each keeps its own register set and the kernel
+// saves what is live across a call; FP32 fits 4 waves (127 VGPRs), FP64 2.
 template <typename T>
-__global__ __launch_bounds__(kBatchTH) void k_tn_batch(const TnCell<T>* __restrict__ cells, int nb,
+constexpr int kBatchWaves = sizeof(T) == 4 ? 4 : 2;
+
+template <typename T>
+__global__ __launch_bounds__(kBatchTH, kBatchWaves<T>) void k_tn_batch(const TnCell<T>* __restrict__ cells, int nb,
                                                        const long long* __restrict__ pref, int maxfun) {
   namespace cgr = cooperative_groups;
   cgr::grid_group grid = cgr::this_grid();
