@@ -69,15 +69,18 @@ def solve_cell(cell, precision="mixed", econv=1e-8, options=None):
 
 class TnBatch:
     """ofdft_batch over the inner-loop plans of several cells (one
-    precision, one stream)."""
+    precision, one stream).  graph=True: the graph form (per-length stage
+    launches, one captured hipGraph per CG iteration of all cells);
+    graph=False: the persistent form (one cooperative launch)."""
 
-    def __init__(self, plans):
+    def __init__(self, plans, graph=True):
         self.lib = L.load()
         self.plans = list(plans)
         n = len(self.plans)
         arr = (ctypes.c_void_p * n)(*[P._h.value for P in self.plans])
         self._h = ctypes.c_void_p()
         L.check(self.lib.ofdft_batch_create(ctypes.byref(self._h), arr, n))
+        L.check(self.lib.ofdft_batch_set_graph(self._h, 1 if graph else 0))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -90,16 +93,17 @@ class TnBatch:
 
     def direction(self, works, maxfun):
         """get_direction_TN of every cell (work buffers of optimization._Work:
-        phi, res, W0 in, tn()["d"] out) in one persistent launch.  Returns
-        (niter, status, pAp_last) lists."""
+        phi, res, W0 in, tn()["d"] out) in one batched call; a None work
+        leaves its cell out.  Returns (niter, status, pAp_last) lists."""
         n = len(works)
         if n != len(self.plans):
             raise ValueError("one work per plan of the batch")
 
         def arr(ts):
-            return (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+            return (ctypes.c_void_p * n)(*[t.data_ptr() if t is not None else None for t in ts])
 
-        tn = [w.tn() for w in works]
+        tn = [w.tn() if w is not None else {"d": None, "work": None} for w in works]
+        works = [w if w is not None else _Absent for w in works]
         niter, status, pap = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_double * n)()
         L.check(self.lib.ofdft_batch_tn_direction(self._h, arr([w.phi for w in works]), arr([w.res for w in works]),
                                                   arr([w.W0 for w in works]), arr([t["d"] for t in tn]),
@@ -107,12 +111,18 @@ class TnBatch:
         return list(niter), list(status), list(pap)
 
 
-def solve_batch_persistent(cells, econv=1e-8, options=None, workers=8):
+class _Absent:
+    """Stands for a cell left out of a batched call (null pointers)."""
+    phi = res = W0 = None
+
+
+def solve_batch_persistent(cells, econv=1e-8, options=None, workers=8, graph=True):
     """Solve all cells in lockstep, mixed precision (FP64 outer iterations,
     FP32 inner CG with the exact Hessian product): per TN outer iteration,
-    one persistent launch runs the inner loops of every unconverged cell,
-    then each cell takes its line search and update (the reference's
-    optimize_rho per cell, optimization.py:283-525).
+    one batched call runs the inner loops of every unconverged cell (graph
+    form by default, the single cooperative launch with graph=False), then
+    each cell takes its line search and update (the reference's optimize_rho
+    per cell, optimization.py:283-525).
 
     The per-cell outer work (line search, update, residual) runs on
     ``workers`` host threads, each with its own HIP stream: the outer plans
@@ -176,7 +186,8 @@ def solve_batch_persistent(cells, econv=1e-8, options=None, workers=8):
         maxiter = opts_common["maxiter"]
         maxfun = opts_common.get("maxfun", 50)
         active = list(range(len(built)))
-        batch, batch_key = None, None
+        # one batch over every cell; converged cells are left out of later calls
+        batch = TnBatch([c[1]._st.Wi.P for c in built], graph=graph)
         it = 0
         while active and it < maxiter - 1:
             it += 1
@@ -184,14 +195,13 @@ def solve_batch_persistent(cells, econv=1e-8, options=None, workers=8):
             for i in active:
                 st = built[i][1]._st
                 built[i][1]._tn_inner_inputs(st.W, st.Wi)
-            if tuple(active) != batch_key:
-                batch = TnBatch([built[i][1]._st.Wi.P for i in active])
-                batch_key = tuple(active)
-            niter, status, pap = batch.direction([built[i][1]._st.Wi for i in active], maxfun)
+            live = set(active)
+            niter, status, pap = batch.direction(
+                [c[1]._st.Wi if i in live else None for i, c in enumerate(built)], maxfun)
             nd = {}
-            for k, i in enumerate(active):
-                _finite(pap[k], "p.Ap in the truncated-Newton loop")
-                nd[i] = niter[k]
+            for i in active:
+                _finite(pap[i], "p.Ap in the truncated-Newton loop")
+                nd[i] = niter[i]
 
             def step(i):
                 opt = built[i][1]

@@ -227,9 +227,20 @@ int ofdft_batch_create(ofdft_batch* out, const ofdft_plan* plans, int nb) {
 
 int ofdft_batch_destroy(ofdft_batch B) {
   if (!B) return 0;
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
   if (B->d_cells) (void)hipFree(B->d_cells);
   if (B->d_pref) (void)hipFree(B->d_pref);
+  if (B->d_idx) (void)hipFree(B->d_idx);
+  if (B->h_states) (void)hipHostFree(B->h_states);
+  if (B->h_live) (void)hipHostFree(B->h_live);
   delete B;
+  return 0;
+}
+
+int ofdft_batch_set_graph(ofdft_batch B, int on) {
+  if (!B) return fail(OFDFT_EINVAL, "null batch");
+  B->use_graph = on ? 1 : 0;
   return 0;
 }
 

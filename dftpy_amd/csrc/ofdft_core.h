@@ -200,9 +200,20 @@ struct ofdft_plan_s {
 struct ofdft_batch_s {
   std::vector<ofdft_plan_s*> plans;
   int prec = 0;
-  void* d_cells = nullptr;      // per-plan descriptors of the persistent kernel
-  long long* d_pref = nullptr;  // work-item prefix sums of its stages
-  size_t cells_cap = 0, pref_cap = 0;
+  int use_graph = 1;            // 1: graph form (per-length stage launches), 0: one cooperative launch
+  void* d_cells = nullptr;      // per-plan descriptors
+  long long* d_pref = nullptr;  // work-item prefix sums of the stages
+  int* d_idx = nullptr;         // graph form: the cells of each length class
+  size_t cells_cap = 0, pref_cap = 0, idx_cap = 0;
+  void* h_states = nullptr;     // pinned, mapped: every cell's TnState after a call
+  void* d_states = nullptr;
+  size_t states_cap = 0;
+  int* h_live = nullptr;        // pinned, mapped: cells still iterating (graph form)
+  int* d_live = nullptr;
+  hipStream_t cap_stream = nullptr;  // graph form: capture stream
+  hipGraphExec_t exec = nullptr;     // graph form: one CG iteration of every cell
+  std::vector<long long> key;        // launch shapes and buffers the graph was captured with
+  int last_iters = 0;                // longest loop of the previous call (first chunk of the next)
 };
 
 namespace {

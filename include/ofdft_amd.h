@@ -116,15 +116,26 @@ int ofdft_plan_set_tiles(ofdft_plan plan, int tile_axis, int tile_pencil);
  * share one precision and one stream; lengths must be cubic and in the
  * compile-time set 32, 48, 64, 90, 96, 108, 128, 160, 192.
  * ofdft_batch_tn_direction runs ofdft_tn_direction for every cell c
- * (phi[c], res[c], W0[c], d[c], work[c]: the same device arrays) in ONE
- * persistent cooperative launch: each stage of the CG iteration is a work
- * queue over (cell, tile) items for all cells, stages separated by grid
- * barriers, until every cell's loop has ended.  niter, status, pAp_last:
- * host arrays of nb.  The batch keeps pointers to the plans: destroy the
- * batch before them.  ofdft_batch_create validates the plans (host only).  */
+ * (phi[c], res[c], W0[c], d[c], work[c]: the same device arrays) for all
+ * cells at once, in the form ofdft_batch_set_graph selects, until every
+ * cell's loop has ended.  A null phi[c] leaves cell c out of the call (its
+ * other pointers may be null too; it reports niter 0): a caller keeps one
+ * batch while cells converge.  niter, status, pAp_last: host arrays of nb.
+ * The batch keeps pointers to the plans: destroy the batch before them.
+ * ofdft_batch_create validates the plans (host only).  */
 typedef struct ofdft_batch_s* ofdft_batch;
 int ofdft_batch_create(ofdft_batch* out, const ofdft_plan* plans, int nb);
 int ofdft_batch_destroy(ofdft_batch batch);
+/* How ofdft_batch_tn_direction runs (no reference counterpart: a launch
+ * form of the same computation).  on = 1 (the default): the graph form --
+ * the cells are grouped by length, each stage of a CG iteration is one
+ * launch per length with the one-cell kernels' workgroup shapes and tile
+ * widths, and one CG iteration of every cell is a captured hipGraph
+ * replayed until every loop has ended (the host checks every few
+ * iterations).  on = 0: the persistent form -- ONE cooperative launch whose
+ * stages are work queues over (cell, tile) items separated by grid
+ * barriers.  Same results either way (up to the order of FP64 sums).  */
+int ofdft_batch_set_graph(ofdft_batch batch, int on);
 int ofdft_batch_tn_direction(ofdft_batch batch, const void* const* phi, const void* const* res,
                              const void* const* W0, void* const* d, void* const* work, int maxfun, int* niter,
                              int* status, double* pAp_last);
