@@ -122,5 +122,12 @@ econv = 1e-8
         rho.mul_(12.0 / (float(rho.sum()) * grid.dV))
     opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN", optimization_options={"econv": 4e-8})
     opt.optimize_rho(guess_rho=rho)
-    print("INI %.12f  API %.12f" % (E, opt.energy))
-    assert abs(E - opt.energy) < 1e-9
+    # the job reports the energy of a fresh evaluation at the final density
+    # (interface.py:245-317).  LMGPA's kf node range only grows with the
+    # densities evaluated (lwt.py:36-80), so after a failed line search
+    # (whose trial rotations reach far) that evaluation can differ from the
+    # optimizer's last energy; the API side takes the same evaluation.
+    E_api = ev.compute(opt.rho).energy
+    print("INI %.12f  API %.12f (optimizer %.12f, converged %d)" % (E, E_api, opt.energy, opt.converged))
+    assert abs(E - E_api) < 1e-9
+    assert opt.converged in (0, 1)

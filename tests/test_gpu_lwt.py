@@ -147,16 +147,10 @@ def test_lwt_stress_is_not_implemented():
         tot.get_stress(D.DirectField(grid=grid, data=_smooth(32, wl["cell"], 12.0)))
 
 
-@pytest.mark.parametrize("name,opts", [("LWT", dict()), ("LWT", dict(interp="hermite")),
-                                       ("LMGP", dict(maxpoints=100, interp="hermite"))])
-def test_lwt_tn_solve(name, opts):
-    """KEDF LWT / LMGP + LDA + Hartree + PME local PP on the 4-atom cell: a
-    TN solve (forward-difference product, as the reference) against the
-    oracle's TN with the same outcome.  With linear interpolation the
-    potential is not the exact gradient of the energy and both line
-    searches fail at the same step; with Hermite interpolation both
-    converge."""
-    print("case lwt TN", name, flush=True)
+KF_FIXED = dict(kfmin=0.3, kfmax=5.0)
+
+
+def _lwt_pair(name, opts):
     wl = W.build("al4_32")
     grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
     ions = D.Ions(wl["symbols"], wl["positions"], wl["cell"])
@@ -174,10 +168,50 @@ def test_lwt_tn_solve(name, opts):
     oopt.optimize_rho(wl["rho_ini"])
     print("GPU %.12f  oracle %.12f  (%d / %d iterations, status %d / %d)"
           % (opt.energy, oopt.energy, len(opt.log), len(oopt.log), opt.converged, oopt.converged))
+    return opt, oopt
+
+
+@pytest.mark.parametrize("name,opts", [("LWT", dict(KF_FIXED)), ("LWT", dict(KF_FIXED, interp="hermite")),
+                                       ("LMGP", dict(KF_FIXED, maxpoints=100, interp="hermite"))])
+def test_lwt_tn_solve(name, opts):
+    """KEDF LWT / LMGP + LDA + Hartree + PME local PP on the 4-atom cell: a
+    TN solve (forward-difference product, as the reference) against the
+    oracle's TN, with the kf node range given (kfmin, kfmax: lwt.py:36-40),
+    so that the functional is the same function at every call.  With
+    linear interpolation the potential is not the exact gradient of the
+    energy and both line searches fail (converged = 1) at the same point;
+    with Hermite interpolation both converge.  These outcomes are stable
+    under rounding: the oracle gives the same status with its potential
+    multiplied by (1 + 1e-15 N(0,1)) (linear: 6e-9 Ha apart) or 1e-14 N(0,1)
+    (Hermite: 4e-13 Ha, LMGP 8e-12 Ha apart)."""
+    print("case lwt TN", name, opts, flush=True)
+    opt, oopt = _lwt_pair(name, opts)
     assert opt.converged == oopt.converged
     if opts.get("interp") == "hermite":
         assert opt.converged == 0
-    # both stop on two energy changes below econv (or on the same failed line
-    # search); with the forward-difference product the stopping points differ
-    # by a few econv
-    assert abs(opt.energy - oopt.energy) < 3e-8
+        # both stop on two energy changes below econv; with the
+        # forward-difference product the stopping points differ by a few econv
+        assert abs(opt.energy - oopt.energy) < 3e-8
+    else:
+        assert opt.converged == 1
+        assert abs(opt.energy - oopt.energy) < 1e-7
+
+
+def test_lwt_tn_solve_growing_kf_range():
+    """The same solve with the kf range that grows with the densities seen
+    (guess_kf_bound, lwt.py:36-80).  The range widens by whole 0.5 steps of
+    kfmax (and factors of kfmin) as soon as one evaluated density reaches
+    past it, including the forward-difference points phi + 1e-9 p, so which
+    evaluation widens it turns on last-digit rounding of max(rho), and from
+    there the trajectories of any two implementations part (here after the
+    second step).  The bar: the first two outer steps equal the oracle's
+    (1e-8 Ha, same CG and line-search counts), and the solve ends converged
+    or at a failed line search, the reference's two outcomes."""
+    print("case lwt TN growing kf range", flush=True)
+    opt, oopt = _lwt_pair("LWT", {})
+    for k in (1, 2):
+        g, o = opt.log[k], oopt.log[k]
+        print(k, g["energy"], o[1], g["Nd"], o[4], g["Nls"], o[5])
+        assert abs(g["energy"] - o[1]) < 1e-8
+        assert (g["Nd"], g["Nls"]) == (o[4], o[5])
+    assert opt.converged in (0, 1) and oopt.converged in (0, 1)
