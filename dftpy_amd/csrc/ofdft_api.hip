@@ -84,6 +84,10 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
   P->N = (long long)n0 * n1 * n2;
   P->Nh = (long long)n0 * n1 * P->nh;
   P->Nhp = (long long)n0 * n1 * P->nhp;
+  {
+    const long long cs = precision == OFDFT_F64 ? 16 : 8;
+    P->idx32 = (3 * P->Nhp * cs < (1LL << 31) && P->N * cs < (1LL << 31)) ? 1 : 0;
+  }
   P->prec = precision;
   P->full = full ? 1 : 0;
   P->rs = precision == OFDFT_F64 ? sizeof(double) : sizeof(float);

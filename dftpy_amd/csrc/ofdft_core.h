@@ -135,6 +135,7 @@ struct ofdft_plan_s {
   int nh;            // n2/2 + 1 logical half-spectrum length
   int nhp;           // row stride of the internal spectrum (nh rounded up to a multiple of 16)
   long long N, Nh, Nhp;
+  int idx32;  // every field and the 3-field spectrum fit 2^31 bytes: 32-bit element offsets
   int prec;
   int full;   // reference full=True semantics (Hermitian-symmetrised multipliers)
   size_t rs;  // bytes per real
@@ -346,6 +347,24 @@ __device__ void block_sums(double (&acc)[NS], double* __restrict__ out) {
   }
 }
 
+// Element i of a field.  With a 32-bit index the byte offset is formed in
+// 32 bits, so that the access takes the scalar-base + 32-bit vector-offset
+// form of global memory instructions instead of a 64-bit address per
+// lane; used where the plan guarantees every field and the spectrum fit in
+// 2^31 bytes (ofdft_plan_s::idx32).
+template <typename T>
+__device__ __forceinline__ T& at(T* p, long long i) {
+  return p[i];
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* p, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T& at(const T* p, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
+}
+
 // Select one of three per-point values without a dynamically indexed array.
 template <typename T>
 __device__ __forceinline__ T pick3(const T* c, int f) {
@@ -369,6 +388,22 @@ struct FnParams {
   int fH, fV, fW;  // spectrum slot of each convolution field (-1 if absent)
   int F;
 };
+
+// Spectrum slots and the beta = 5/6 flag of a Hessian-product functor.
+// LAY = 1: the fixed layout of the full functional (Hartree, vW and WT
+// fields in slots 0, 1, 2, beta = 5/6), so that every slot test and
+// register select folds at compile time; LAY = 0: read from FnParams.
+template <int LAY>
+struct FieldSlots {
+  __device__ __forceinline__ static int fH(const FnParams& p) { return LAY ? 0 : p.fH; }
+  __device__ __forceinline__ static int fV(const FnParams& p) { return LAY ? 1 : p.fV; }
+  __device__ __forceinline__ static int fW(const FnParams& p) { return LAY ? 2 : p.fW; }
+  __device__ __forceinline__ static bool b56(const FnParams& p) { return LAY ? true : p.beta56 != 0; }
+};
+
+inline bool fixed_layout(const FnParams& p) {
+  return p.F == 3 && p.fH == 0 && p.fV == 1 && p.fW == 2 && p.beta56;
+}
 
 // LDA exchange-correlation energy density per electron and potential:
 // xc/semilocal_xc.py:287-320.
@@ -501,9 +536,10 @@ struct PreEval {
   struct In {
     T a, b;
   };
-  __device__ __forceinline__ In load(long long i) const {
-    if (rho) return {rho[i], T(0)};
-    return {phi[i], p ? p[i] : T(0)};
+  template <class I>
+  __device__ __forceinline__ In load(I i) const {
+    if (rho) return {at(rho, i), T(0)};
+    return {at(phi, i), p ? at(p, i) : T(0)};
   }
   __device__ __forceinline__ T density(const In& x) const {
     if (rho) return x.a;
@@ -511,7 +547,8 @@ struct PreEval {
     return ph * ph;
   }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
-  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  template <class I>
+  __device__ __forceinline__ void apply(const In& x, T* out, I) const { apply(x, out); }
   __device__ __forceinline__ void apply(const In& x, T* out) const {
     const T r = density(x);
     if (fp.fH >= 0) put3(out, fp.fH, r);
@@ -527,8 +564,9 @@ struct PreEval {
 
 // HVP inputs for direction p at phi: drho = 2 phi p (Hartree), ds = drho/(2s)
 // (vW), beta*rho^(beta-1)*drho (WT).
-template <typename T>
+template <typename T, int LAY = 0>
 struct PreHvp {
+  using S = FieldSlots<LAY>;
   static constexpr int kProf = PC_ROWS_FWD_HVP;
   const T* __restrict__ phi;
   const T* __restrict__ p;
@@ -537,9 +575,11 @@ struct PreHvp {
   struct In {
     T ph, pp;
   };
-  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i]}; }
+  template <class I>
+  __device__ __forceinline__ In load(I i) const { return {at(phi, i), at(p, i)}; }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out); }
-  __device__ __forceinline__ void apply(const In& x, T* out, long long) const { apply(x, out); }
+  template <class I>
+  __device__ __forceinline__ void apply(const In& x, T* out, I) const { apply(x, out); }
   __device__ __forceinline__ void apply(const In& x, T* out) const {
     const T ph = x.ph, pp = x.pp;
     const T r = ph * ph;
@@ -547,20 +587,21 @@ struct PreHvp {
     if constexpr (sizeof(T) == 4) {
       // the same fields with sqrt(rho) = |phi| cancelled: drho / (2 sqrt(rho))
       // = sign(phi) p, beta rho^(beta-1) drho = 2 beta sign(phi) |phi|^(2/3) p
-      if (fp.fH >= 0) put3(out, fp.fH, dr);
-      if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? (ph > T(0) ? pp : -pp) : T(0));
-      if (fp.fW >= 0) {
-        const T w = fp.beta56 ? T(2 * fp.beta) * sgn_pow23(ph) * pp : T(fp.beta) * dev_pow(r, T(fp.beta) - T(1)) * dr;
-        put3(out, fp.fW, r < T(kZero) ? T(0) : w);
+      if (S::fH(fp) >= 0) put3(out, S::fH(fp), dr);
+      if (S::fV(fp) >= 0) put3(out, S::fV(fp), r > T(0) ? (ph > T(0) ? pp : -pp) : T(0));
+      if (S::fW(fp) >= 0) {
+        const T w = S::b56(fp) ? T(2 * fp.beta) * sgn_pow23(ph) * pp
+                               : T(fp.beta) * dev_pow(r, T(fp.beta) - T(1)) * dr;
+        put3(out, S::fW(fp), r < T(kZero) ? T(0) : w);
       }
       return;
     }
     const T s = dev_sqrt(r);
-    if (fp.fH >= 0) put3(out, fp.fH, dr);
-    if (fp.fV >= 0) put3(out, fp.fV, r > T(0) ? dev_div(dr, T(2) * s) : T(0));
-    if (fp.fW >= 0) {
-      const T rb1 = fp.beta56 ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));  // rho^(beta-1)
-      put3(out, fp.fW, r < T(kZero) ? T(0) : T(fp.beta) * rb1 * dr);
+    if (S::fH(fp) >= 0) put3(out, S::fH(fp), dr);
+    if (S::fV(fp) >= 0) put3(out, S::fV(fp), r > T(0) ? dev_div(dr, T(2) * s) : T(0));
+    if (S::fW(fp) >= 0) {
+      const T rb1 = S::b56(fp) ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));  // rho^(beta-1)
+      put3(out, S::fW(fp), r < T(kZero) ? T(0) : T(fp.beta) * rb1 * dr);
     }
   }
 };
@@ -973,9 +1014,11 @@ struct PostEval {
     typename PreEval<T>::In d;
     T vl;
   };
-  __device__ __forceinline__ In load(long long i) const { return {pre.load(i), pre.fp.pseudo ? vloc[i] : T(0)}; }
+  template <class I>
+  __device__ __forceinline__ In load(I i) const { return {pre.load(i), pre.fp.pseudo ? at(vloc, i) : T(0)}; }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
-  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+  template <class I>
+  __device__ __forceinline__ void apply(I i, const In& in, const T* c, double* acc) const {
     const FnParams& fp = pre.fp;
     T ph = T(0), dph = T(0);
     T r;
@@ -1003,7 +1046,7 @@ struct PostEval {
       const T vraw = dev_div(T(0.5) * pick3(c, fp.fV), s);
       if (r > T(0)) acc[OFDFT_E_VW] += (double)(r * vraw);
       v += vraw * T(fp.y);
-      if (aux) aux[i] = vraw;
+      if (aux) at(aux, i) = vraw;
     }
     if (fp.fW >= 0) {
       // WT: 2 alpha rho^beta / rho * conv, energy density V rho/(2 alpha) (wt.py:12-28, :125)
@@ -1012,7 +1055,7 @@ struct PostEval {
       const T vw = T(2 * fp.alpha) * rbm1 * pick3(c, fp.fW);
       acc[OFDFT_E_NL] += (double)(vw * r / T(2 * fp.alpha));
       v += vw;
-      if (aux) aux[N + i] = pick3(c, fp.fW);
+      if (aux) at(aux + N, i) = pick3(c, fp.fW);
     }
     if (fp.lda) {
       T ex, vx;
@@ -1033,7 +1076,7 @@ struct PostEval {
     acc[OFDFT_S_VRHO] += (double)(v * r);
     acc[OFDFT_S_RHO] += (double)r;
     if (!pre.rho) acc[OFDFT_S_DTHETA] += (double)(v * ph * dph);
-    if (V) V[i] = v;
+    if (V) at(V, i) = v;
   }
 };
 
@@ -1086,8 +1129,9 @@ struct PostHvp {
 // Ap = W0 p + phi * (H[drho] + y/2 GG[ds]/s + 2 alpha rho^(beta-1) K[beta rho^(beta-1) drho]),
 // with W0 = V - mu + 2 rho a(rho) precomputed by k_hvp_prepare (a: the local
 // second derivative of TF, LDA, vW and WT); acc[0] = sum p*Ap.
-template <typename T>
+template <typename T, int LAY = 0>
 struct PostHvpFast {
+  using S = FieldSlots<LAY>;
   static constexpr int NS = 1;
   static constexpr int kProf = PC_ROWS_INV_HVP;
   const T* __restrict__ phi;
@@ -1099,7 +1143,8 @@ struct PostHvpFast {
   struct In {
     T ph, pp, w0;
   };
-  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i], W0[i]}; }
+  template <class I>
+  __device__ __forceinline__ In load(I i) const { return {at(phi, i), at(p, i), at(W0, i)}; }
   __device__ __forceinline__ void operator()(long long i, const T* c, double* acc) const { apply(i, load(i), c, acc); }
   // (H p)(i) from the point's phi, p, W0 and the convolutions c
   __device__ __forceinline__ T ap_of(const In& in, const T* c) const {
@@ -1109,32 +1154,33 @@ struct PostHvpFast {
       // phi * dv with sqrt(rho) = |phi| cancelled: phi (y/2) c_vW / sqrt(rho)
       // = (y/2) sign(phi) c_vW, phi 2 alpha rho^(beta-1) = 2 alpha sign(phi) |phi|^(2/3)
       T ap = in.w0 * pp;
-      if (fp.fH >= 0) ap += ph * pick3(c, fp.fH);
-      if (fp.fV >= 0 && r > T(0)) {
-        const T cv = T(0.5 * fp.y) * pick3(c, fp.fV);
+      if (S::fH(fp) >= 0) ap += ph * pick3(c, S::fH(fp));
+      if (S::fV(fp) >= 0 && r > T(0)) {
+        const T cv = T(0.5 * fp.y) * pick3(c, S::fV(fp));
         ap += ph > T(0) ? cv : -cv;
       }
-      if (fp.fW >= 0 && r >= T(kZero)) {
-        const T g = fp.beta56 ? sgn_pow23(ph) : ph * dev_pow(r, T(fp.beta) - T(1));
-        ap += T(2 * fp.alpha) * g * pick3(c, fp.fW);
+      if (S::fW(fp) >= 0 && r >= T(kZero)) {
+        const T g = S::b56(fp) ? sgn_pow23(ph) : ph * dev_pow(r, T(fp.beta) - T(1));
+        ap += T(2 * fp.alpha) * g * pick3(c, S::fW(fp));
       }
       return ap;
     }
     T dv = T(0);
-    if (fp.fH >= 0) dv += pick3(c, fp.fH);
-    if (fp.fV >= 0 || fp.fW >= 0) {
+    if (S::fH(fp) >= 0) dv += pick3(c, S::fH(fp));
+    if (S::fV(fp) >= 0 || S::fW(fp) >= 0) {
       const T s = dev_sqrt(r);
-      if (fp.fV >= 0 && r > T(0)) dv += dev_div(T(fp.y) * T(0.5) * pick3(c, fp.fV), s);
-      if (fp.fW >= 0 && r >= T(kZero)) {
-        const T rb1 = fp.beta56 ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));
-        dv += T(2 * fp.alpha) * rb1 * pick3(c, fp.fW);
+      if (S::fV(fp) >= 0 && r > T(0)) dv += dev_div(T(fp.y) * T(0.5) * pick3(c, S::fV(fp)), s);
+      if (S::fW(fp) >= 0 && r >= T(kZero)) {
+        const T rb1 = S::b56(fp) ? dev_div(dev_cbrt(r), s) : dev_pow(r, T(fp.beta) - T(1));
+        dv += T(2 * fp.alpha) * rb1 * pick3(c, S::fW(fp));
       }
     }
     return in.w0 * pp + ph * dv;
   }
-  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+  template <class I>
+  __device__ __forceinline__ void apply(I i, const In& in, const T* c, double* acc) const {
     const T ap = ap_of(in, c);
-    Ap[i] = ap;
+    at(Ap, i) = ap;
     acc[0] += (double)(in.pp * ap);
   }
 };
@@ -1199,29 +1245,31 @@ __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, c
   const long long pair0 = bx * PB;
   const int ld = n2;
   if constexpr (FFT::kFixed) {
-    // every load of the thread's points is issued before the first use
+    // every load of the thread's points is issued before the first use;
+    // compile-time lengths run on plans whose fields fit 32-bit offsets
     constexpr int NN = FFT::n();
     constexpr int IT = (PB * NN + FFT::TH - 1) / FFT::TH;
+    const unsigned nr = (unsigned)nrows, p0 = (unsigned)pair0;
     typename Pre::In xa[IT], xb[IT];
 #pragma unroll
     for (int u = 0; u < IT; ++u) {
       const int t = threadIdx.x + u * FFT::TH;
       const int lp = t / NN, j = t - lp * NN;
-      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
       xa[u] = {};
       xb[u] = {};
-      if (t < PB * NN && r0 < nrows) xa[u] = pre.load(r0 * NN + j);
-      if (t < PB * NN && r1 < nrows) xb[u] = pre.load(r1 * NN + j);
+      if (t < PB * NN && r0 < nr) xa[u] = pre.load(r0 * NN + j);
+      if (t < PB * NN && r1 < nr) xb[u] = pre.load(r1 * NN + j);
     }
 #pragma unroll
     for (int u = 0; u < IT; ++u) {
       const int t = threadIdx.x + u * FFT::TH;
       if (t < PB * NN) {
         const int lp = t / NN, j = t - lp * NN;
-        const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+        const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
         T a[3] = {T(0), T(0), T(0)}, b[3] = {T(0), T(0), T(0)};
-        if (r0 < nrows) pre.apply(xa[u], a, r0 * NN + j);
-        if (r1 < nrows) pre.apply(xb[u], b, r1 * NN + j);
+        if (r0 < nr) pre.apply(xa[u], a, r0 * NN + j);
+        if (r1 < nr) pre.apply(xb[u], b, r1 * NN + j);
 #pragma unroll
         for (int f = 0; f < 3; ++f)
           if (f < F) lds[(lp * F + f) * ld + j] = cplx<T>{a[f], b[f]};
@@ -1246,11 +1294,20 @@ __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, c
   for (int t = threadIdx.x; t < PB * F * nh; t += th) {
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
-    const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
     const cplx<T> zk = lds[tr * ld + k];
     const cplx<T> zm = lds[tr * ld + (k == 0 ? 0 : n2 - k)];
-    if (r0 < nrows) spec[f * Nh + r0 * nhp + k] = cplx<T>{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
-    if (r1 < nrows) spec[f * Nh + r1 * nhp + k] = cplx<T>{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+    const cplx<T> A{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+    const cplx<T> B{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+    if constexpr (FFT::kFixed) {
+      const unsigned r0 = 2u * ((unsigned)pair0 + lp), r1 = r0 + 1;
+      const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
+      if (r0 < (unsigned)nrows) at(spec, o) = A;
+      if (r1 < (unsigned)nrows) at(spec, o + (unsigned)nhp) = B;
+    } else {
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      if (r0 < nrows) spec[f * Nh + r0 * nhp + k] = A;
+      if (r1 < nrows) spec[f * Nh + r1 * nhp + k] = B;
+    }
   }
 }
 
@@ -1290,24 +1347,34 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
   constexpr int NN = fixed_len<FFT>();
   constexpr int IT = FFT::kFixed ? (PB * NN + FFT::TH - 1) / FFT::TH : 1;
   typename Post::In xa[IT], xb[IT];
+  const unsigned nr = (unsigned)nrows, p0 = (unsigned)pair0;
   if constexpr (FFT::kFixed) {
+    // compile-time lengths run on plans whose fields fit 32-bit offsets
 #pragma unroll
     for (int u = 0; u < IT; ++u) {
       const int t = threadIdx.x + u * FFT::TH;
       const int lp = t / NN, j = t - lp * NN;
-      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
       xa[u] = {};
       xb[u] = {};
-      if (t < PB * NN && r0 < nrows) xa[u] = post.load(r0 * NN + j);
-      if (t < PB * NN && r1 < nrows) xb[u] = post.load(r1 * NN + j);
+      if (t < PB * NN && r0 < nr) xa[u] = post.load(r0 * NN + j);
+      if (t < PB * NN && r1 < nr) xb[u] = post.load(r1 * NN + j);
     }
   }
   for (int t = threadIdx.x; t < PB * F * nh; t += th) {
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
-    const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
-    cplx<T> A = r0 < nrows ? spec[f * Nh + r0 * nhp + k] : cplx<T>{T(0), T(0)};
-    cplx<T> B = r1 < nrows ? spec[f * Nh + r1 * nhp + k] : cplx<T>{T(0), T(0)};
+    cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
+    if constexpr (FFT::kFixed) {
+      const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
+      const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
+      if (r0 < nr) A = at(spec, o);
+      if (r1 < nr) B = at(spec, o + (unsigned)nhp);
+    } else {
+      const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+      if (r0 < nrows) A = spec[f * Nh + r0 * nhp + k];
+      if (r1 < nrows) B = spec[f * Nh + r1 * nhp + k];
+    }
     if (k == 0 || (even && k == n2 / 2)) {
       A.y = T(0);
       B.y = T(0);
@@ -1328,7 +1395,7 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
       const int t = threadIdx.x + u * FFT::TH;
       if (t < PB * NN) {
         const int lp = t / NN, j = t - lp * NN;
-        const long long r0 = 2 * (pair0 + lp), r1 = r0 + 1;
+        const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
         T ca[3] = {T(0), T(0), T(0)}, cb[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -1338,8 +1405,8 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
             cb[f] = z.y;
           }
         }
-        if (r0 < nrows) post.apply(r0 * NN + j, xa[u], ca, acc);
-        if (r1 < nrows) post.apply(r1 * NN + j, xb[u], cb, acc);
+        if (r0 < nr) post.apply(r0 * NN + j, xa[u], ca, acc);
+        if (r1 < nr) post.apply(r1 * NN + j, xb[u], cb, acc);
       }
     }
   } else {
@@ -1535,7 +1602,7 @@ __device__ __forceinline__ void axis_reg_block(cplx<T>* __restrict__ spec, long 
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1)
-    v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
+    v[n1] = ok ? at(base, (unsigned)(N2 * n1 + q) * (unsigned)stride + c) : cplx<T>{T(0), T(0)};
   if (act) {
     dftreg<T, N1>(v, iv);
 #pragma unroll
@@ -1554,7 +1621,7 @@ __device__ __forceinline__ void axis_reg_block(cplx<T>* __restrict__ spec, long 
     dftreg<T, N2>(v, iv);
     if (ok) {
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) base[(long long)(q + N1 * k2) * stride + c] = v[k2];
+      for (int k2 = 0; k2 < N2; ++k2) at(base, (unsigned)(q + N1 * k2) * (unsigned)stride + c) = v[k2];
     }
   }
 }
@@ -1587,20 +1654,22 @@ __device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, lon
   const long long stride = (long long)geo.n1 * geo.nhp;
   cplx<T>* base = spec + (long long)f * Nh + (long long)i1 * geo.nhp;
   const int c = bx * W + w;
-  const bool ok = act && c < geo.nhp;  // padded columns are transformed (they stay zero)
+  // padded columns (c >= nh) are neither read nor written: no kernel reads
+  // them, and their transform would only move zeros
+  const bool ok = act && c < geo.nh;
   const bool live = act && c < geo.nh;
   const int kind = ck.kind[f];
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1)
-    v[n1] = ok ? base[(long long)(N2 * n1 + q) * stride + c] : cplx<T>{T(0), T(0)};
+    v[n1] = ok ? at(base, (unsigned)(N2 * n1 + q) * (unsigned)stride + c) : cplx<T>{T(0), T(0)};
   // WT multipliers of this thread's points, loaded with the spectrum so that
   // their latency is not exposed between the two transforms
   T wm[N2];
   const bool wt = kind == OFDFT_CONV_WT && q < N1 && live;
 #pragma unroll
   for (int k2 = 0; k2 < N2; ++k2)
-    wm[k2] = wt ? wtk[((long long)(q + N1 * k2) * geo.n1 + i1) * geo.nh + c] : T(0);
+    wm[k2] = wt ? at(wtk, ((unsigned)(q + N1 * k2) * (unsigned)geo.n1 + i1) * (unsigned)geo.nh + c) : T(0);
   if (act) {
     dftreg<T, N1>(v, false);
 #pragma unroll
@@ -1670,7 +1739,7 @@ __device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, lon
     dftreg<T, N1>(v, true);
     if (ok) {
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) base[(long long)(N2 * n1 + q) * stride + c] = v[n1];
+      for (int n1 = 0; n1 < N1; ++n1) at(base, (unsigned)(N2 * n1 + q) * (unsigned)stride + c) = v[n1];
     }
   }
   if (cgp) {
@@ -2043,10 +2112,10 @@ __global__ void k_tn_finish(const TnState* __restrict__ st, T* __restrict__ d, c
 // iteration folded in: p = r + beta p (optimization.py:144-145) when the
 // control asked for it, written back for the step kernel and the next
 // update; each point is loaded by exactly one thread of the row pass.
-template <typename T>
+template <typename T, int LAY = 0>
 struct PreHvpCG {
   static constexpr int kProf = PC_ROWS_FWD_HVP;
-  PreHvp<T> h;  // its transform of (phi, p); its own pointers are not read
+  PreHvp<T, LAY> h;  // its transform of (phi, p); its own pointers are not read
   const T* __restrict__ phi;
   T* __restrict__ p;  // the direction, updated in place
   const T* __restrict__ r;
@@ -2057,14 +2126,16 @@ struct PreHvpCG {
   struct In {
     T ph, pp, rr;
   };
-  __device__ __forceinline__ In load(long long i) const { return {phi[i], p[i], r[i]}; }
-  __device__ __forceinline__ void apply(const In& x, T* out, long long i) const {
+  template <class I>
+  __device__ __forceinline__ In load(I i) const { return {at(phi, i), at(p, i), at(r, i)}; }
+  template <class I>
+  __device__ __forceinline__ void apply(const In& x, T* out, I i) const {
     T pp = x.pp;
     if (st->act & kActNext) {
       pp = x.rr + T(st->beta) * pp;
-      p[i] = pp;
+      at(p, i) = pp;
     }
-    h.apply(typename PreHvp<T>::In{x.ph, pp}, out);
+    h.apply(typename PreHvp<T, LAY>::In{x.ph, pp}, out);
   }
   __device__ __forceinline__ void operator()(long long i, T* out) const { apply(load(i), out, i); }
 };
@@ -2076,11 +2147,11 @@ struct PreHvpCG {
 // never stored.  acc: r^2 and, for the pointwise part of the next pAp,
 // W0 r^2, W0 r p, W0 p^2.  Nothing changes when pAp < 0 (the reference
 // leaves the loop before the update).
-template <typename T>
+template <typename T, int LAY = 0>
 struct PostHvpCG {
   static constexpr int NS = 4;
   static constexpr int kProf = PC_ROWS_INV_HVP;
-  PostHvpFast<T> h;  // phi, p, W0 and the functional; its Ap is not written
+  PostHvpFast<T, LAY> h;  // phi, p, W0 and the functional; its Ap is not written
   T* __restrict__ d;
   T* __restrict__ r;
   T* __restrict__ best;
@@ -2088,19 +2159,21 @@ struct PostHvpCG {
   const double* __restrict__ red;
   __host__ __device__ int nio() const { return 7; }
   struct In {
-    typename PostHvpFast<T>::In x;
+    typename PostHvpFast<T, LAY>::In x;
     T dd, rr;
   };
-  __device__ __forceinline__ In load(long long i) const { return {h.load(i), d[i], r[i]}; }
-  __device__ __forceinline__ void apply(long long i, const In& in, const T* c, double* acc) const {
+  template <class I>
+  __device__ __forceinline__ In load(I i) const { return {h.load(i), at(d, i), at(r, i)}; }
+  template <class I>
+  __device__ __forceinline__ void apply(I i, const In& in, const T* c, double* acc) const {
     const double pAp = red[0];
     if (pAp < 0.0) return;
     const T ap = h.ap_of(in.x, c);
     const T a = T(st->r0 / pAp);
-    if (st->act & kActKeep) best[i] = in.dd;
-    d[i] = in.dd + a * in.x.pp;
+    if (st->act & kActKeep) at(best, i) = in.dd;
+    at(d, i) = in.dd + a * in.x.pp;
     const T rn = in.rr - a * ap;
-    r[i] = rn;
+    at(r, i) = rn;
     const double rv = (double)rn, pv = (double)in.x.pp, w = (double)in.x.w0;
     acc[0] += rv * rv;
     acc[1] += w * rv * rv;
@@ -2201,6 +2274,10 @@ constexpr bool pencil_fixed(int n) {
 #undef OFDFT_IS
 }
 
+// the register pencil (and with it the spectral p.Ap sums of the fused TN
+// step) runs on compile-time lengths of plans with 32-bit offsets
+inline bool pencil_reg_ok(const ofdft_plan_s* P) { return P->idx32 && pencil_fixed(P->n[0]); }
+
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // columns per workgroup of the register column kernels
 template <typename T>
@@ -2241,7 +2318,9 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   dim3 grid((unsigned)((npairs + PB - 1) / PB));
   ProfScope ps(P, Pre::kProf, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
-  switch (P->n[2]) {
+  // compile-time lengths address with 32-bit offsets: plans too large for
+  // them take the runtime-schedule kernels
+  switch (P->idx32 ? P->n[2] : 0) {
 #define OFDFT_CASE(NN) \
     case NN: go_rows_fwd<T>(P, pre, RowsFFT<T, NN>{}, F, grid, rows_th<NN>(), lds); break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
@@ -2287,7 +2366,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   double* part = reduce ? P->partials : nullptr;
   {
     ProfScope ps(P, Post::kProf, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
-    switch (P->n[2]) {
+    switch (P->idx32 ? P->n[2] : 0) {
 #define OFDFT_CASE(NN) \
       case NN: go_rows_inv<T>(P, post, RowsFFT<T, NN>{}, F, src, grid, rows_th<NN>(), lds, part); break;
       OFDFT_FIXED_SIZES(OFDFT_CASE)
@@ -2323,7 +2402,7 @@ template <typename T, int N1, int N2, int W>
 void axis_reg(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, long long stride, long long ostride, unsigned outer,
               bool inv) {
   OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W>), dim3((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F),
-               dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nhp,
+               dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
                reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
@@ -2345,7 +2424,8 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   }
   dim3 grid((unsigned)((P->nh + W - 1) / W), outer, (unsigned)F);
   ProfScope ps(P, PC_AXIS, (long long)P->rs * 4LL * F * P->Nh);
-  switch (n) {
+  // the register kernels address with 32-bit offsets (ofdft_plan_s::idx32)
+  switch (P->idx32 ? n : 0) {
 #define OFDFT_CASE(NN)                                                                                       \
     case NN:                                                                                                 \
       if (P->tile_axis == 8) axis_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, spec, axis, F, stride, ostride, outer, inv); \
@@ -2393,14 +2473,14 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
   for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
   ck.scale = 1.0 / (double)P->N;
   for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
-  if (cgp && !pencil_fixed(P->n[0])) return fail(OFDFT_EINVAL, "p.Ap sums need a compile-time pencil length");
+  if (cgp && !pencil_reg_ok(P)) return fail(OFDFT_EINVAL, "p.Ap sums need a compile-time pencil length");
   for (int f = 0; f < F; ++f)
     if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
   GridGeom g = geom(P);
   long long nwt = 0;
   for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
   ProfScope ps(P, PC_PENCIL, (long long)P->rs * (4LL * F + nwt) * P->Nh);
-  switch (P->n[0]) {
+  switch (P->idx32 ? P->n[0] : 0) {
 #define OFDFT_CASE(NN)                                                                                   \
     case NN:                                                                                             \
       if (P->tile_pencil == 8) pencil_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, g, ck, cgp);        \
@@ -2508,18 +2588,16 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
 // The exact HVP; with `cg`, the forward row pass first applies the pending
 // CG update p = r + beta p of the device-side TN loop (p is then updated in
 // place; with no convolution fields the caller does that update itself).
-template <typename T>
-int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap, double* pAp,
-                  const T* cg_r = nullptr, const TnState* cg_st = nullptr) {
-  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
-  FnParams fp = make_fn(P->fn);
+template <typename T, int LAY>
+int hvp_fast_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, const T* p, const T* W0, T* Ap, double* pAp,
+                 const T* cg_r, const TnState* cg_st) {
   const int F = fp.F;
   auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
   int rc;
   if (F > 0) {
-    PreHvp<T> pre{phi, p, fp};
+    PreHvp<T, LAY> pre{phi, p, fp};
     if (cg_st) {
-      PreHvpCG<T> pcg{pre, phi, const_cast<T*>(p), cg_r, cg_st};
+      PreHvpCG<T, LAY> pcg{pre, phi, const_cast<T*>(p), cg_r, cg_st};
       if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
     } else if ((rc = launch_rows_fwd<T>(P, pre, F))) {
       return rc;
@@ -2532,7 +2610,7 @@ int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap,
     if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
     if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
   }
-  PostHvpFast<T> post{phi, p, W0, Ap, fp};
+  PostHvpFast<T, LAY> post{phi, p, W0, Ap, fp};
   if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, pAp != nullptr))) return rc;
   if (pAp) {
     double t;
@@ -2542,21 +2620,29 @@ int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap,
   return 0;
 }
 
+template <typename T>
+int hvp_fast_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* W0, T* Ap, double* pAp,
+                  const T* cg_r = nullptr, const TnState* cg_st = nullptr) {
+  if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
+  const FnParams fp = make_fn(P->fn);
+  return fixed_layout(fp) ? hvp_fast_lay<T, 1>(P, fp, phi, p, W0, Ap, pAp, cg_r, cg_st)
+                          : hvp_fast_lay<T, 0>(P, fp, phi, p, W0, Ap, pAp, cg_r, cg_st);
+}
+
 // One inner CG step of the device TN loop: the HVP's transforms with the
 // spectral part of pAp summed in the pencil, pAp completed with sum W0 p^2
 // (k_tn_pap), and the step taken in the inverse row pass (PostHvpCG).
 // *nblk_rows: the partial-sum blocks that pass leaves for k_tn_reduce_ctrl4.
-template <typename T>
-int hvp_cg_impl(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T* best, const TnState* st,
-                long long* nblk_rows) {
-  FnParams fp = make_fn(P->fn);
+template <typename T, int LAY>
+int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T* W0, T* d, T* r, T* best,
+               const TnState* st, long long* nblk_rows) {
   const int F = fp.F;
   auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
   int rc;
   long long nblk_pen = 0;
   if (F > 0) {
-    PreHvp<T> pre{phi, p, fp};
-    PreHvpCG<T> pcg{pre, phi, p, r, st};
+    PreHvp<T, LAY> pre{phi, p, fp};
+    PreHvpCG<T, LAY> pcg{pre, phi, p, r, st};
     if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
     if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
     int kinds[3] = {0, 0, 0};
@@ -2583,12 +2669,20 @@ int hvp_cg_impl(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T*
   hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
                      P->red);
   LAUNCHCHK();
-  PostHvpFast<T> h{phi, p, W0, nullptr, fp};
-  PostHvpCG<T> post{h, d, r, best, st, (const double*)P->red};
+  PostHvpFast<T, LAY> h{phi, p, W0, nullptr, fp};
+  PostHvpCG<T, LAY> post{h, d, r, best, st, (const double*)P->red};
   if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
   const long long npairs = ((long long)P->n[0] * P->n[1] + 1) / 2;
   *nblk_rows = (npairs + kRowPB<T> - 1) / kRowPB<T>;
   return 0;
+}
+
+template <typename T>
+int hvp_cg_impl(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T* r, T* best, const TnState* st,
+                long long* nblk_rows) {
+  const FnParams fp = make_fn(P->fn);
+  return fixed_layout(fp) ? hvp_cg_lay<T, 1>(P, fp, phi, p, W0, d, r, best, st, nblk_rows)
+                          : hvp_cg_lay<T, 0>(P, fp, phi, p, W0, d, r, best, st, nblk_rows);
 }
 
 template <typename T>
@@ -4128,7 +4222,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   P->skip = &st->done;
   const bool fused = make_fn(P->fn).F > 0;  // a forward row pass carries the update of p
   // the CG step rides on the inverse row pass when the pencil can sum p.Ap
-  const bool cg_fused = !fused || pencil_fixed(P->n[0]);
+  const bool cg_fused = !fused || pencil_reg_ok(P);
   int rc = 0, queued = 0, k = std::max(1, chunk);
   while (queued < maxfun) {
     for (int j = 0; j < k && queued < maxfun; ++j, ++queued) {

@@ -113,7 +113,7 @@ __device__ __noinline__ void batch_axis(const TnCell<T>& C, long long local, boo
   const int bx = (int)(local % C.tx);
   const long long rem = local / C.tx;
   const int by = (int)(rem % C.n), bz = (int)(rem / C.n);
-  axis_reg_block<T, Split<NN>::N1, Split<NN>::N2, kBatchTile>(C.spec, C.Nhp, C.nhp, (long long)C.n * C.nhp, C.nhp,
+  axis_reg_block<T, Split<NN>::N1, Split<NN>::N2, kBatchTile>(C.spec, C.Nhp, C.nhp, (long long)C.n * C.nhp, C.nh,
                                                               C.tw1, inv ? 1 : 0, bx, by, bz, lds);
 }
 
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(W * N2) void k_bt_axis(const TnCell<T>* __restrict_
   const TnCell<T>& C = cells[idx[k]];
   if (C.st->done || f >= C.F) return;
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  axis_reg_block<T, N1, N2, W>(C.spec, C.Nhp, C.nhp, (long long)C.n * C.nhp, C.nhp, C.tw1, inv, blockIdx.x,
+  axis_reg_block<T, N1, N2, W>(C.spec, C.Nhp, C.nhp, (long long)C.n * C.nhp, C.nh, C.tw1, inv, blockIdx.x,
                                blockIdx.y, f, lds);
 }
 
