@@ -2047,6 +2047,28 @@ __global__ void k_tn_pap(const TnState* st, const double* __restrict__ partials,
   tn_pap_block(st, partials, nblk, red);
 }
 
+// k_tn_pap's arguments, for the inverse axis pass that absorbs it.
+struct PapArgs {
+  const TnState* st;
+  const double* part;
+  long long nblk;
+  double* red;
+};
+
+// The inverse axis-1 pass of the TN step with k_tn_pap folded in: the
+// pencil's partial sums are complete when this kernel starts, so its first
+// workgroup also completes p.Ap (one launch and one dispatch gap fewer per
+// CG iteration; the inverse row pass that reads p.Ap runs after it).
+template <typename T, int N1, int N2, int W>
+__global__ __launch_bounds__(W * N2) void k_axis_reg_pap(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+                                                         long long ostride, int ncol, const cplx<T>* __restrict__ tw,
+                                                         int inv, const int* __restrict__ skip, PapArgs pa) {
+  if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
+  __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
+  axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+  if ((blockIdx.x | blockIdx.y | blockIdx.z) == 0 && !pa.st->done) tn_pap_block(pa.st, pa.part, pa.nblk, pa.red);
+}
+
 // r1 and the three W0 sums of the inverse row pass, the tests of
 // optimization.py:116-145, and sum W0 p^2 of the next direction
 // p' = r + beta p from W0 r^2 + 2 beta W0 r p + beta^2 W0 p^2.
@@ -2400,14 +2422,21 @@ void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid
 
 template <typename T, int N1, int N2, int W>
 void axis_reg(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, long long stride, long long ostride, unsigned outer,
-              bool inv) {
-  OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W>), dim3((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F),
-               dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
+              bool inv, const PapArgs* pa) {
+  const dim3 grid((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F);
+  if (pa) {
+    OFDFT_LAUNCH((k_axis_reg_pap<T, N1, N2, W>), grid, dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
+                 reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip, *pa);
+    return;
+  }
+  OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W>), grid, dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
                reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
+// pa: also complete p.Ap of the TN step (k_tn_pap), folded into the
+// register kernel or launched after the runtime-schedule one.
 template <typename T>
-int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
+int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv, const PapArgs* pa = nullptr) {
   const int W = kColTileC;
   const int n = P->n[axis];
   const size_t lds = (size_t)(W * (n + 1) + n) * sizeof(cplx<T>);
@@ -2428,10 +2457,10 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
   switch (P->idx32 ? n : 0) {
 #define OFDFT_CASE(NN)                                                                                       \
     case NN:                                                                                                 \
-      if (P->tile_axis == 8) axis_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, spec, axis, F, stride, ostride, outer, inv); \
+      if (P->tile_axis == 8) axis_reg<T, Split<NN>::N1, Split<NN>::N2, 8>(P, spec, axis, F, stride, ostride, outer, inv, pa); \
       else if (sizeof(T) == 4 && P->tile_axis == 32)                                                           \
-        axis_reg<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16)>(P, spec, axis, F, stride, ostride, outer, inv); \
-      else axis_reg<T, Split<NN>::N1, Split<NN>::N2, 16>(P, spec, axis, F, stride, ostride, outer, inv);     \
+        axis_reg<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16)>(P, spec, axis, F, stride, ostride, outer, inv, pa); \
+      else axis_reg<T, Split<NN>::N1, Split<NN>::N2, 16>(P, spec, axis, F, stride, ostride, outer, inv, pa);     \
       break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
@@ -2443,6 +2472,10 @@ int launch_axis(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, bool inv) {
       else if (L.kb == 1) go_axis<T>(P, FFTRuntime<1>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
       else if (L.kb == 2) go_axis<T>(P, FFTRuntime<2>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
       else go_axis<T>(P, FFTRuntime<4>{pl}, spec, axis, grid, L.threads, lds, stride, ostride, inv);
+      if (pa) {
+        LAUNCHCHK();
+        hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, pa->st, pa->part, pa->nblk, pa->red);
+      }
     }
   }
   LAUNCHCHK();
@@ -2664,11 +2697,14 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
     }
     if ((rc = launch_pencil<T>(P, F, kinds, P->partials, coef))) return rc;
     nblk_pen = (long long)((P->nhp + P->tile_pencil - 1) / P->tile_pencil) * P->n[1] * F;
-    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+    // p.Ap is completed by the inverse axis pass (k_tn_pap folded in)
+    const PapArgs pa{st, (const double*)P->partials, nblk_pen, (double*)P->red};
+    if ((rc = launch_axis<T>(P, spec, 1, F, true, &pa))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
+                       P->red);
+    LAUNCHCHK();
   }
-  hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
-                     P->red);
-  LAUNCHCHK();
   PostHvpFast<T, LAY> h{phi, p, W0, nullptr, fp};
   PostHvpCG<T, LAY> post{h, d, r, best, st, (const double*)P->red};
   if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
