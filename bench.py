@@ -36,6 +36,10 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# the timed region times one launch in PROFILE_STRIDE of the dominant class;
+# odd, so that classes whose launches alternate between two kernels (the
+# forward and inverse axis passes) are sampled alike
+PROFILE_STRIDE = 5
 
 # HBM traffic per launch of a kernel class: a calibrated constant, not
 # measured by this script.  It comes from rocprofv3 --pmc FETCH_SIZE and
@@ -268,9 +272,12 @@ def main():
     # dominant kernel class: largest total device time
     prec_d, cls_d = max(((pk, ck) for pk, pr in breakdown.items() for ck, v in pr.items() if v[1]),
                         key=lambda t: breakdown[t[0]][t[1]][0])
-    # timed region: events around the dominant class only (two per launch)
+    # timed region: events around the dominant class only (two per timed
+    # launch), one launch in every PROFILE_STRIDE of that class: the event
+    # pair adds dispatch overhead (about 10 us per timed launch), so timing
+    # a uniform sample keeps most of it out of the timed solves
     for pk, P in plans.items():
-        P.set_profiling(pk == prec_d, classes=[cls_d])
+        P.set_profiling(pk == prec_d, classes=[cls_d], stride=PROFILE_STRIDE)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     iters, energies, logs = 0, [], []
@@ -315,7 +322,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MEASURED_TRAFFIC.get((args.workload, prec_d, cls_d)),
                      "traffic_source": TRAFFIC_SOURCE if (args.workload, prec_d, cls_d) in MEASURED_TRAFFIC else None,
-                     "launches": cnt_d, "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "launches": cnt_d, "sampled": "1 launch in %d of the class, events on the plan stream" % PROFILE_STRIDE,
+                     "avg_us": avg_s * 1e6, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "all_kernels_GBps": total_bytes / (total_dev_ms / 1e3) / 1e9 if total_dev_ms else None},
         # algorithmic bytes of every kernel of one solve over the solve's wall time
         "whole_solve": {"algorithmic_bytes": total_bytes, "seconds": elapsed / args.steps,

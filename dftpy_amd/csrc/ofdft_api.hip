@@ -303,7 +303,9 @@ long long ofdft_plan_workspace_bytes(ofdft_plan P) {
 int ofdft_plan_set_profiling(ofdft_plan P, int on) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
   P->prof_on = on & ((1 << PC_NCLS) - 1);
+  P->prof_stride = 1 + ((on >> 8) & 0xff);
   for (int i = 0; i < 8; ++i) {
+    P->prof_seq[i] = 0;
     P->prof_ms[i] = 0.0;
     P->prof_count[i] = 0;
     P->prof_bytes[i] = 0.0;

@@ -172,8 +172,11 @@ struct ofdft_plan_s {
   ofdft_lwt_table lwt_tab[2];
   double* d_lwttab;
   void* d_lwt;
-  // per-kernel-class event timing (ofdft_plan_set_profiling)
+  // per-kernel-class event timing (ofdft_plan_set_profiling): classes in
+  // prof_on, one launch in every prof_stride of a class timed
   int prof_on;
+  int prof_stride = 1;
+  long long prof_seq[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;  // reused across profile windows
   size_t ev_used = 0;
   struct ProfRec { int cls; long long bytes; int e0, e1, tag; };
@@ -242,6 +245,7 @@ struct ProfScope {
   int e0 = -1;
   ProfScope(ofdft_plan_s* p, int c, long long b) : P(p), cls(c), bytes(b) {
     if (!(P->prof_on & (1 << c))) return;
+    if (P->prof_seq[c]++ % P->prof_stride != 0) return;  // a sampled launch of the class
     while (P->ev_pool.size() < P->ev_used + 2) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;

@@ -299,9 +299,10 @@ class Plan:
 
     PROFILE_CLASSES = ("rows_fwd", "rows_fwd_hvp", "axis_c2c", "pencil_conv", "rows_inv", "rows_inv_hvp", "vector")
 
-    def set_profiling(self, on=True, classes=None):
+    def set_profiling(self, on=True, classes=None, stride=1):
         """Time kernel launches with HIP events: all classes, or only the
-        named ones (PROFILE_CLASSES); resets the counters."""
+        named ones (PROFILE_CLASSES), one launch in every ``stride`` of
+        each (1..256); resets the counters."""
         if not on:
             mask = 0
         elif classes is None:
@@ -310,6 +311,10 @@ class Plan:
             mask = 0
             for c in classes:
                 mask |= 1 << self.PROFILE_CLASSES.index(c)
+        if not 1 <= int(stride) <= 256:
+            raise ValueError("stride must be in 1..256")
+        if mask:
+            mask |= (int(stride) - 1) << 8
         L.check(self.lib.ofdft_plan_set_profiling(self._h, mask))
 
     def get_profile(self):

@@ -155,7 +155,10 @@ long long ofdft_plan_workspace_bytes(ofdft_plan plan);
  * bytes (each launch's compulsory reads + writes) summed over launches.
  * set_profiling resets the counters; `mask` selects the classes timed
  * (bit c for class c, 0 = off, 127 = all), so that a caller can time one
- * kernel class without adding events around every other launch.        */
+ * kernel class without adding events around every other launch; bits
+ * 8-15 hold k - 1 to time one launch in every k of each selected class
+ * (the first, the (k+1)-th, ...), which keeps the timing's own dispatch
+ * overhead off most launches.                                          */
 #define OFDFT_NPROF 7
 int ofdft_plan_set_profiling(ofdft_plan plan, int mask);
 int ofdft_plan_get_profile(ofdft_plan plan, double* ms, long long* count, double* bytes);
