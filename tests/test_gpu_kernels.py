@@ -414,3 +414,27 @@ def test_column_tile_widths_give_identical_transforms(n, prec):
         assert np.array_equal(sums[w][0][:6], sums[16][0][:6]), w
     with pytest.raises(Exception):
         P2.set_tiles(12, 16)
+
+
+@pytest.mark.parametrize("stride", [1, 3])
+def test_profiling_times_one_launch_in_stride(stride):
+    """ofdft_plan_set_profiling: per-class launch counts, time and bytes;
+    with a stride k only launches 0, k, 2k, ... of each class are timed."""
+    print("case profiling stride", stride, flush=True)
+    P = _plan((32, 32, 32), np.eye(3) * 10.0, "fp32")
+    f = torch.rand((32, 32, 32), dtype=P.dtype, device=P.device)
+    P.set_profiling(True, stride=stride)
+    for _ in range(6):
+        P.r2c(f)
+    prof = P.get_profile()
+    P.set_profiling(False)
+    # each transform: one row pass, two axis passes
+    n_rows, n_axis = 6, 12
+    assert prof["rows_fwd"][1] == -(-n_rows // stride)
+    assert prof["axis_c2c"][1] == -(-n_axis // stride)
+    for k in ("rows_fwd", "axis_c2c"):
+        ms, cnt, by = prof[k]
+        assert ms > 0 and by > 0
+    assert all(v[1] == 0 for k, v in prof.items() if k not in ("rows_fwd", "axis_c2c"))
+    with pytest.raises(ValueError):
+        P.set_profiling(True, stride=0)
