@@ -102,7 +102,17 @@ STRESS_EV_A3 = HARTREE_EV / BOHR ** 3  # Ha/bohr^3 -> eV/Angstrom^3 (constants.p
 
 
 class DFTpyCalculator(Calculator):
-    """DFTpy calculator for ase (api/api4ase.py:11-119)."""
+    """An ase calculator over the GPU density optimization.  Its interface
+    is the reference's (api/api4ase.py:11-119): the constructor arguments,
+    the ase properties energy / forces / stress and their units.
+
+    ``dftpy_results`` holds the last job's results (density, evaluator,
+    energies; forces and stress once asked for).  A change of the atoms
+    reruns the density optimization: with ``config`` as an INI job, which
+    reuses the last density and the pseudopotential tables while the cell
+    is unchanged and MATH.reuse is set; with ``evaluator`` from the last
+    density, carried over to a new cell at the same grid shape and
+    renormalised to the ions' charge."""
 
     implemented_properties = ["energy", "forces", "stress"]
 
@@ -116,100 +126,82 @@ class DFTpyCalculator(Calculator):
         self.zero_stress = zero_stress
         self.dftpy_results = {"density": rho}
 
+    # -- ase ---------------------------------------------------------------
     def calculate(self, atoms=None, properties=("energy",), system_changes=all_changes):
-        """api4ase.py:25-50."""
         Calculator.calculate(self, atoms, properties, system_changes)
-        if len(system_changes) > 0:
+        if system_changes:
             self.run(system_changes, properties=properties)
         self.energy = self.dftpy_results["energypotential"]["TOTAL"].energy * HARTREE_EV
         self.results["energy"] = self.energy
         if "forces" in properties:
-            if "TOTAL" not in self.dftpy_results["forces"]:
-                self.calc_forces()
-            self.forces = self.dftpy_results["forces"]["TOTAL"] * FORCE_EV_A
+            self.forces = self._lazy("forces") * FORCE_EV_A
             self.results["forces"] = self.forces
         if "stress" in properties:
-            voigt = np.zeros(6)
-            if not self.zero_stress:
-                if "TOTAL" not in self.dftpy_results["stress"]:
-                    self.calc_stress()
-                s = self.dftpy_results["stress"]["TOTAL"]
-                voigt[:3] = np.diag(s)
-                voigt[3], voigt[4], voigt[5] = s[1, 2], s[0, 2], s[0, 1]
-            self.stress = voigt * STRESS_EV_A3
+            self.stress = (np.zeros(6) if self.zero_stress else self._voigt(self._lazy("stress"))) * STRESS_EV_A3
             self.results["stress"] = self.stress
 
-    def run(self, system_changes=all_changes, properties=("energy",)):
-        """api4ase.py:52-60."""
-        if self.config is not None:
-            self.run_config(system_changes=system_changes, properties=properties)
-        elif self.evaluator is not None:
-            self.run_normal(system_changes=system_changes, properties=properties)
-        else:
-            raise ValueError("!!!Error: Please give a 'config' or 'evaluator' for the calculator")
+    @staticmethod
+    def _voigt(s):
+        """ase's Voigt order xx, yy, zz, yz, xz, xy of a symmetric 3x3 stress."""
+        return np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
 
-    def run_config(self, system_changes=all_changes, properties=("energy",), **kwargs):
-        """The INI job for the current atoms (api4ase.py:62-74): the
-        density and pseudopotential of the last run are reused when
-        MATH.reuse is set and the cell is unchanged."""
-        pseudo = self.dftpy_results.get("pseudo", None)
-        rho = self.dftpy_results.get("density", None)
+    def _lazy(self, kind):
+        """The TOTAL forces or stress (Ha/bohr, Ha/bohr^3) of the last
+        density, evaluated on first request after a job."""
+        have = self.dftpy_results.get(kind) or {}
+        if "TOTAL" not in have:
+            res = evaluator2results(self.dftpy_results.get("evaluator"), rho=self.dftpy_results.get("density"),
+                                    calcType={"F" if kind == "forces" else "S"})
+            self.dftpy_results[kind] = res.get(kind)
+        return self.dftpy_results[kind]["TOTAL"]
+
+    # -- density optimization ----------------------------------------------
+    def run(self, system_changes=all_changes, properties=("energy",)):
+        """The density optimization for the current atoms."""
+        if self.config is not None:
+            self._job_config(system_changes)
+        elif self.evaluator is not None:
+            self._job_evaluator(system_changes, properties)
+        else:
+            raise ValueError("DFTpyCalculator has nothing to compute with: give it a config (INI options) "
+                             "or an evaluator (TotalFunctional)")
+
+    def _job_config(self, system_changes):
         ions = Ions.from_ase(self.atoms)
-        grid = None
-        if "cell" not in system_changes and rho is not None:
-            grid = rho.grid
+        rho = self.dftpy_results.get("density")
+        # the last grid (and with it the pseudopotential tables) only while
+        # the cell is unchanged; the last density as the start with MATH.reuse
+        grid = rho.grid if (rho is not None and "cell" not in system_changes) else None
+        pseudo = self.dftpy_results.get("pseudo") if grid is not None else None
+        if pseudo is not None:
+            pseudo.restart(ions=ions, grid=grid)
         if not self.config["MATH"]["reuse"]:
             rho = None
-        if pseudo is not None:
-            if "cell" in system_changes or grid is None:
-                pseudo = None
-            else:
-                pseudo.restart(ions=ions, grid=grid)
         config, others = ConfigParser(self.config, ions=ions, rhoini=rho, pseudo=pseudo, grid=grid)
         self.dftpy_results = OptimizeDensityConf(config, **others)
 
-    def run_normal(self, system_changes=all_changes, properties=("energy",), **kwargs):
-        """Optimize from the last density with the given evaluator
-        (api4ase.py:76-107). A new cell keeps the grid shape; the density
-        values are carried over and renormalised to the ions' charge."""
-        rho = self.dftpy_results.get("density", None)
+    def _job_evaluator(self, system_changes, properties):
         ions = Ions.from_ase(self.atoms)
+        rho = self.dftpy_results.get("density")
         if rho is None:
-            raise ValueError("!!!Error: Please give a initial density for the calculator")
+            raise ValueError("DFTpyCalculator(evaluator=...) needs the initial density: pass rho=DirectField")
+        new_cell = "cell" in system_changes
         grid = rho.grid
-        if "cell" in system_changes:
+        if new_cell:
             grid = DirectGrid(lattice=ions.cell, nr=grid.nrR, full=grid.full, device=grid.get_device())
-            with torch._C.DisableTorchFunctionSubclass():
-                rho = DirectField(grid=grid, data=rho.detach().clone())
-        for key, func in self.evaluator.funcDict.items():
+        # the local pseudopotential follows the ions (and sets their charges)
+        for func in self.evaluator.funcDict.values():
             if func.type == "PSEUDO":
                 func.restart(ions=ions, grid=grid)
-        if "cell" in system_changes:
-            ntot = ions.get_ncharges()
+        if new_cell:
             with torch._C.DisableTorchFunctionSubclass():
-                rho.mul_(ntot / (float(rho.sum()) * grid.dV))
+                data = rho.detach().clone()
+                data.mul_(ions.get_ncharges() / (float(data.sum()) * grid.dV))
+            rho = DirectField(grid=grid, data=data)
         if self.optimizer is None:
             self.optimizer = Optimization(EnergyEvaluator=self.evaluator)
         rho = self.optimizer.optimize_rho(guess_rho=rho)
-        calc = {"E"}
-        if "forces" in properties:
-            calc.add("F")
-        if "potential" in properties:
-            calc.add("V")
+        wanted = {"E"} | {c for c, p in (("F", "forces"), ("V", "potential")) if p in properties}
         if "stress" in properties and not self.zero_stress:
-            calc.add("S")
-        self.dftpy_results = evaluator2results(self.evaluator, rho=rho, calcType=calc, ions=ions, **kwargs)
-
-    def calc_forces(self):
-        """api4ase.py:109-113."""
-        evaluator = self.dftpy_results.get("evaluator", None)
-        rho = self.dftpy_results.get("density", None)
-        results = evaluator2results(evaluator, rho=rho, calcType={"F"})
-        self.dftpy_results["forces"] = results.get("forces", None)
-
-    def calc_stress(self):
-        """api4ase.py:115-119."""
-        evaluator = self.dftpy_results.get("evaluator", None)
-        rho = self.dftpy_results.get("density", None)
-        results = evaluator2results(evaluator, rho=rho, calcType={"S"})
-        self.dftpy_results["stress"] = results.get("stress", None)
+            wanted.add("S")
+        self.dftpy_results = evaluator2results(self.evaluator, rho=rho, calcType=wanted, ions=ions)

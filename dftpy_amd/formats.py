@@ -58,7 +58,7 @@ def read_xsf(infile, kind="all", full=False, pbc=True, units="angstrom", data_ty
     with open(infile) as fr:
         fh = _lines(fr)
         if next(fh).upper() != "CRYSTAL":
-            raise AttributeError("Only support crystal xsf.")
+            raise AttributeError("%s: the XSF reader takes periodic (CRYSTAL) files only" % infile)
         lattice = np.zeros((3, 3))
         line = next(fh).upper()
         if line.startswith("PRIMVEC"):
@@ -112,12 +112,12 @@ def read_xsf(infile, kind="all", full=False, pbc=True, units="angstrom", data_ty
             if line.strip().upper().startswith("END_BLOCK_DATAGRID"):
                 break
     if not blocks:
-        raise AttributeError("!!!ERROR : XSF file have some problem")
+        raise AttributeError("%s: no DATAGRID block found" % infile)
     shape = blocks[0].shape
     out = []
     for b in blocks:
         if b.shape != shape:
-            raise AttributeError("All DATAGRID should have same shape.")
+            raise AttributeError("%s: the DATAGRID blocks differ in shape" % infile)
         nr = np.array(b.shape)
         if pbc:
             nr = np.where(nr > 1, nr - 1, nr)

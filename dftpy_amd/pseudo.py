@@ -42,25 +42,26 @@ class RadialPseudo:
 
 
 def read_recpot(fname):
-    """CASTEP recpot reader (pseudo/recpot.py:11-41)."""
+    """CASTEP recpot table v(q) (the format pseudo/recpot.py:11-41 reads):
+    after the "END COMMENT" line come a version line, q_max in 1/Angstrom,
+    then v(q) in eV Angstrom^3 on an even grid from 0 to q_max, closed by a
+    terminator line ("1000", or any one-character line).  Returned in
+    atomic units, with the valence charge read off the Coulomb tail
+    v ~ -4 pi Z / q^2 between the first two points (as the reference
+    rounds it)."""
     with open(fname) as fh:
-        lines = fh.readlines()
-    ibegin, iend = 0, None
-    for i, line in enumerate(lines):
-        if "END COMMENT" in line:
-            ibegin = i + 3
-        elif ibegin > 1 and (line.strip() == "1000" or len(line.strip()) == 1):
-            iend = i
-            break
-    if iend is None:
-        raise AttributeError("Error : Check the PP file : {}".format(fname))
-    body = " ".join(l.strip() for l in lines[ibegin:iend])
-    gmax = float(lines[ibegin - 1].strip()) * BOHR
-    v = np.array(body.split()).astype(np.float64) / HARTREE / BOHR ** 3
-    r = np.linspace(0, gmax, num=len(v))
-    zval = round((v[0] - v[1]) * (r[1] ** 2) / (4.0 * np.pi))
-    return RadialPseudo(r, v, zval)
-
+        lines = [l.strip() for l in fh]
+    try:
+        start = next(i for i, l in enumerate(lines) if "END COMMENT" in l) + 2
+        stop = next(i for i in range(start + 1, len(lines)) if lines[i] == "1000" or len(lines[i]) == 1)
+    except StopIteration:
+        raise AttributeError("recpot file %s: no END COMMENT header or no terminator line after the "
+                             "v(q) table" % fname) from None
+    qmax = float(lines[start]) * BOHR
+    v = np.array(" ".join(lines[start + 1:stop]).split(), dtype=np.float64) / HARTREE / BOHR ** 3
+    q = np.linspace(0.0, qmax, num=len(v))
+    zval = round((v[0] - v[1]) * q[1] ** 2 / (4.0 * np.pi))
+    return RadialPseudo(q, v, zval)
 
 
 def sym6(v):
