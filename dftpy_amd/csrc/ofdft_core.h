@@ -1573,18 +1573,24 @@ __global__ __launch_bounds__(FFT::TH) void k_pencil_conv(FFT fft, cplx<T>* __res
 
 
 // LDS stride of one column's N1 x (N2 + 1) block in the register column
-// kernels.  gfx950's LDS serves 256 bytes per cycle from 64 four-byte banks:
-// 32 lanes of 8-byte (FP32 complex) or 16 lanes of 16-byte (FP64 complex)
-// accesses.  Thread (w, q) of a W-column tile writes element w*S + k1*(N2+1)
-// + q and reads w*S + q*(N2+1) + n2; with S = lanes / W (mod lanes) the
-// lanes served together (consecutive w, consecutive q) fall on distinct banks
-// in both transposes (N2 + 1 is odd for every split used).
+// kernels.  Thread (w, q) of a W-column tile writes element w*S + k1*(N2+1)
+// + q and reads w*S + q*(N2+1) + n2.  The stride is the smallest S >=
+// N1*(N2+1) in the residue class that a gfx950 sweep over every split, tile
+// width and S (SQ_LDS_BANK_CONFLICT per LDS instruction of both accesses,
+// 32 consecutive S each) found conflict-free or least conflicting:
+//   8-byte elements (FP32 complex): W >= 16 odd S (about 1 conflict cycle per
+//   instruction against 7-13 for even S), W = 8 S = 2 mod 4 (about 1
+//   against 7-13 for odd S or S = 0 mod 4);
+//   16-byte elements (FP64 complex): odd S (W = 8: 6-12 against 11-19; W =
+//   16: 9-12, the least of any S).
 template <typename T, int N1, int N2, int W>
 constexpr int col_block_stride() {
-  constexpr int lanes = 256 / (2 * (int)sizeof(T));
-  constexpr int want = W >= lanes ? 1 : lanes / W;
   int s = N1 * (N2 + 1);
-  while (s % lanes != want % lanes) ++s;
+  if (sizeof(T) == 4 && W == 8) {
+    while (s % 4 != 2) ++s;
+  } else {
+    while (s % 2 != 1) ++s;
+  }
   return s;
 }
 
