@@ -13,7 +13,11 @@ iterations per second over the timed solves.  The same solve in FP64 is run
 after the timed region (one untimed warm-up, then one timed solve) for the
 mixed/FP64 ratio and the energy error;
 both energies are also compared with the oracle's converged minimum of the
-workload (tests/golden/oracle_converged.json).
+workload (tests/golden/oracle_converged.json).  Also outside the timed
+region of ``value``: the same mixed solves with the inner CG's spectra
+stored as bf16 between the transform passes (``cg_bf16_spectra``: its own
+rate, energy error and ratio to FP64; every transform still computes in
+FP32), and with the CG iterations replayed from a hipGraph (``tn_graph``).
 
 After that, outside the timed region: the batching comparison (64 uneven
 independent cells, lockstep in the graph and persistent forms and
@@ -102,12 +106,13 @@ def build_system(name, D, W):
     return wl, grid, ions, ev
 
 
-def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact", tn_graph=False):
+def solve(D, ev, rho, precision, econv, verbose, phase1=False, hvp="exact", tn_graph=False, cg_spectra="plan"):
     """One TN solve from ``rho`` (a DirectField already resident on the GPU;
     optimize_rho does not modify its guess)."""
     opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
                          optimization_options={"econv": econv, "precision": precision, "maxiter": 200,
-                                               "mixed_phase1": phase1, "hvp": hvp, "tn_graph": tn_graph})
+                                               "mixed_phase1": phase1, "hvp": hvp, "tn_graph": tn_graph,
+                                               "cg_spectra": cg_spectra})
     opt.verbose = verbose
     opt.optimize_rho(guess_rho=rho)
     niter = sum(1 for l in opt.log[1:] if l["prec"] != "switch->fp64")
@@ -349,10 +354,30 @@ def main():
     tg = time.perf_counter() - tg
     out["tn_graph"] = {"value": giters / tg, "unit": "TN iterations/s", "ms_per_step": tg / args.steps * 1e3,
                        "same_energy": bool(genergy == energies[-1])}
+    # the same solves with the inner CG's spectra stored as bf16 between the
+    # transform passes (cg_spectra="bf16"; every transform, multiplier and
+    # pointwise step still in FP32), reported beside the FP32-spectra value
+    solve(D, ev, rho_ini, "mixed", args.econv, False, args.phase1, cg_spectra="bf16")  # first launches untimed
+    bf_tiles = plans["fp32"].cg_spectra_tiles
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    biters, benergy = 0, None
+    for _ in range(args.steps):
+        ob, nb = solve(D, ev, rho_ini, "mixed", args.econv, False, args.phase1, cg_spectra="bf16")
+        biters += nb
+        benergy = ob.energy
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - tb
+    out["cg_bf16_spectra"] = {"value": biters / tb, "unit": "TN iterations/s", "ms_per_step": tb / args.steps * 1e3,
+                              "tn_iterations_per_solve": biters / args.steps, "energy_Ha": benergy,
+                              "converged": ob.converged, "tiles_axis_pencil": list(bf_tiles),
+                              "storage": "bf16 spectra between the passes of the inner CG's Hessian products; "
+                                         "FP32 transforms, FP64 sums, FP64 outer refinement"}
     gold = committed_oracle_solve(args.workload)
     if gold is not None:
         out["oracle_energy_Ha"] = gold["energy_Ha"]
         out["mixed_energy_error_vs_oracle_eV_per_atom"] = abs(energies[-1] - gold["energy_Ha"]) * W.HARTREE / nat
+        out["cg_bf16_spectra"]["energy_error_vs_oracle_eV_per_atom"] = abs(benergy - gold["energy_Ha"]) * W.HARTREE / nat
     if not args.no_fp64:
         # untimed warm-up of the FP64 solves: their TN-loop kernels are not
         # used by the mixed solve, and first launches are not timed
@@ -370,6 +395,8 @@ def main():
         if gold is not None:
             out["fp64_gpu"]["energy_error_vs_oracle_eV_per_atom"] = abs(o64.energy - gold["energy_Ha"]) * W.HARTREE / nat
         out["mixed_over_fp64"] = value / fp64_rate
+        out["cg_bf16_spectra"]["over_fp64"] = out["cg_bf16_spectra"]["value"] / fp64_rate
+        out["cg_bf16_spectra"]["energy_error_vs_fp64_eV_per_atom"] = abs(benergy - o64.energy) * W.HARTREE / nat
         out["time_to_solution_ratio"] = t64 / (elapsed / args.steps)
         out["mixed_energy_error_eV_per_atom"] = err
         out["mixed_energy_Ha"] = energies[-1]

@@ -116,6 +116,8 @@ _SIGS = {
     "ofdft_plan_set_profiling": (_i, [_vp, _i]),
     "ofdft_plan_set_graph": (_i, [_vp, _i]),
     "ofdft_plan_set_tiles": (_i, [_vp, _i, _i]),
+    "ofdft_plan_set_cg_spectra": (_i, [_vp, _i, _i, _i]),
+    "ofdft_plan_cg_spectra": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "ofdft_batch_create": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i]),
     "ofdft_batch_destroy": (_i, [_vp]),
     "ofdft_batch_set_graph": (_i, [_vp, _i]),

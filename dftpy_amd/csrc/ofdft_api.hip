@@ -279,6 +279,33 @@ int ofdft_plan_set_tiles(ofdft_plan P, int tile_axis, int tile_pencil) {
   return 0;
 }
 
+int ofdft_plan_set_cg_spectra(ofdft_plan P, int mode, int tile_axis, int tile_pencil) {
+  if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
+  if (mode != OFDFT_CG_SPECTRA_PLAN && mode != OFDFT_CG_SPECTRA_BF16)
+    return fail(OFDFT_EINVAL, "CG spectra mode must be OFDFT_CG_SPECTRA_PLAN or OFDFT_CG_SPECTRA_BF16");
+  auto ok = [](int w) { return w == 0 || w == 8 || w == 16 || w == 32; };
+  if (!ok(tile_axis) || !ok(tile_pencil))
+    return fail(OFDFT_EINVAL, "bf16 tile widths must be 0 (automatic), 8, 16 or 32");
+  const bool change = mode != P->cg_spectra || tile_axis != P->tile_bf_axis || tile_pencil != P->tile_bf_pencil;
+  P->cg_spectra = mode;
+  P->tile_bf_axis = tile_axis;
+  P->tile_bf_pencil = tile_pencil;
+  if (change && P->tn_exec) {  // a captured CG iteration holds the old kernels
+    (void)hipGraphExecDestroy(P->tn_exec);
+    P->tn_exec = nullptr;
+  }
+  return 0;
+}
+
+int ofdft_plan_cg_spectra(ofdft_plan P, int* tile_axis, int* tile_pencil) {
+  if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
+  int a = 0, p = 0;
+  const bool bf = cg_bf16_tiles(P, &a, &p);
+  if (tile_axis) *tile_axis = a;
+  if (tile_pencil) *tile_pencil = p;
+  return bf ? OFDFT_CG_SPECTRA_BF16 : OFDFT_CG_SPECTRA_PLAN;
+}
+
 int ofdft_plan_set_graph(ofdft_plan P, int on) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
   P->use_graph = on ? 1 : 0;

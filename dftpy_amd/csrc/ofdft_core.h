@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ofdft_amd.h"
@@ -193,6 +194,12 @@ struct ofdft_plan_s {
   // axis-0 pencils): 8, 16 or 32 (FP32), 8 or 16 (FP64); chosen per grid
   // length and precision by the autotuner (ofdft_plan_set_tiles)
   int tile_axis = 16, tile_pencil = 16;
+  // spectra of the TN inner loop's Hessian products (ofdft_plan_set_cg_spectra):
+  // 0 plan precision, 1 bf16 (FP32 plans of compile-time lengths, see
+  // cg_bf16_tiles), with tile_bf_* columns per workgroup of the axis pass
+  // and of the pencils (0: automatic)
+  int cg_spectra = 0;
+  int tile_bf_axis = 0, tile_bf_pencil = 0;
   int use_graph = 0;
   hipGraphExec_t tn_exec = nullptr;
   const void* tn_key[8] = {};
@@ -367,6 +374,20 @@ __device__ __forceinline__ T& at(T* p, unsigned i) {
 template <typename T>
 __device__ __forceinline__ const T& at(const T* p, unsigned i) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + i * (unsigned)sizeof(T));
+}
+
+// bf16 storage of spectra (the TN inner loop's Hessian products on FP32
+// plans, when selected): one complex value in 32 bits, the real part in the
+// low half and the imaginary part in the high half, each rounded to nearest
+// even from FP32 by one v_cvt_pk_bf16_f32; the transforms compute in FP32.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pack_bf16(cplx<float> v) {
+  const f32x2_t f = {v.x, v.y};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f, bf16x2_t));
+}
+__device__ __forceinline__ cplx<float> unpack_bf16(unsigned u) {
+  return cplx<float>{__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
 }
 
 // Select one of three per-point values without a dynamically indexed array.
@@ -1237,7 +1258,11 @@ constexpr int fixed_len() {
 // becomes constant); FC = 0: F taken from Frt.
 // One workgroup's share of the forward row pass (row pairs bx*PB ..):
 // the body of k_rows_fwd, also run by the persistent batched TN kernel.
-template <typename T, class FFT, class Pre, int FC>
+// SL = 1 (FP32, compile-time even length): the spectrum is stored in bf16,
+// columns 0 .. n2/2-1 of field f and row r at word f*Nh + r*nhp + k (the
+// caller passes Nh = n0*n1*n2/2 and nhp = n2/2), the Nyquist column apart
+// at word 3*Nh + f*nrows + r.
+template <typename T, class FFT, class Pre, int FC, int SL = 0>
 __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, cplx<T>* __restrict__ spec,
                                                long long nrows, long long Nh, int nhp, int Frt,
                                                const cplx<T>* __restrict__ tw, long long bx, cplx<T>* lds) {
@@ -1302,7 +1327,16 @@ __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, c
     const cplx<T> zm = lds[tr * ld + (k == 0 ? 0 : n2 - k)];
     const cplx<T> A{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
     const cplx<T> B{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-    if constexpr (FFT::kFixed) {
+    if constexpr (SL == 1) {
+      unsigned* sb = reinterpret_cast<unsigned*>(spec);
+      const unsigned r0 = 2u * ((unsigned)pair0 + lp), r1 = r0 + 1;
+      const unsigned nb = (unsigned)nhp;
+      const unsigned o = (unsigned)k < nb ? (unsigned)f * (unsigned)Nh + r0 * nb + k
+                                          : 3u * (unsigned)Nh + (unsigned)f * (unsigned)nrows + r0;
+      const unsigned o1 = (unsigned)k < nb ? o + nb : o + 1u;
+      if (r0 < (unsigned)nrows) at(sb, o) = pack_bf16(A);
+      if (r1 < (unsigned)nrows) at(sb, o1) = pack_bf16(B);
+    } else if constexpr (FFT::kFixed) {
       const unsigned r0 = 2u * ((unsigned)pair0 + lp), r1 = r0 + 1;
       const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
       if (r0 < (unsigned)nrows) at(spec, o) = A;
@@ -1315,15 +1349,15 @@ __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, c
   }
 }
 
-template <typename T, class FFT, class Pre, int FC>
+template <typename T, class FFT, class Pre, int FC, int SL = 0>
 __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>* __restrict__ spec,
                                                       long long nrows, long long Nh, int nhp, int Frt,
                                                       const cplx<T>* __restrict__ tw,
                                                       const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
-  rows_fwd_block<T, FFT, Pre, FC>(pre, fft, spec, nrows, Nh, nhp, Frt, tw, blockIdx.x,
-                                  reinterpret_cast<cplx<T>*>(smem_raw));
+  rows_fwd_block<T, FFT, Pre, FC, SL>(pre, fft, spec, nrows, Nh, nhp, Frt, tw, blockIdx.x,
+                                      reinterpret_cast<cplx<T>*>(smem_raw));
 }
 
 // Axis-2 complex-to-real fused with the post functor.  Two half spectra A, B
@@ -1335,7 +1369,8 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
 // One workgroup's share of the inverse row pass: the body of k_rows_inv,
 // also run by the persistent batched TN kernel; the workgroup's sums go to
 // partials[bx * NS ..] (none when partials is null).
-template <typename T, class FFT, class Post, int FC>
+// SL = 1: the bf16 spectrum layout of rows_fwd_block's SL = 1.
+template <typename T, class FFT, class Post, int FC, int SL = 0>
 __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft, const cplx<T>* __restrict__ spec,
                                                long long nrows, long long Nh, int nhp, int Frt,
                                                const cplx<T>* __restrict__ tw, double* __restrict__ partials,
@@ -1369,7 +1404,16 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
     cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
-    if constexpr (FFT::kFixed) {
+    if constexpr (SL == 1) {
+      const unsigned* sb = reinterpret_cast<const unsigned*>(spec);
+      const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
+      const unsigned nb = (unsigned)nhp;
+      const unsigned o = (unsigned)k < nb ? (unsigned)f * (unsigned)Nh + r0 * nb + k
+                                          : 3u * (unsigned)Nh + (unsigned)f * nr + r0;
+      const unsigned o1 = (unsigned)k < nb ? o + nb : o + 1u;
+      if (r0 < nr) A = unpack_bf16(at(sb, o));
+      if (r1 < nr) B = unpack_bf16(at(sb, o1));
+    } else if constexpr (FFT::kFixed) {
       const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
       const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
       if (r0 < nr) A = at(spec, o);
@@ -1433,15 +1477,15 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
   if (partials) block_sums<Post::NS>(acc, partials + bx * Post::NS);
 }
 
-template <typename T, class FFT, class Post, int FC>
+template <typename T, class FFT, class Post, int FC, int SL = 0>
 __global__ __launch_bounds__(FFT::TH) void k_rows_inv(Post post, FFT fft, const cplx<T>* __restrict__ spec,
                                                       long long nrows, long long Nh, int nhp, int Frt,
                                                       const cplx<T>* __restrict__ tw,
                                                       double* __restrict__ partials, const int* __restrict__ skip) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   extern __shared__ unsigned char smem_raw[];
-  rows_inv_block<T, FFT, Post, FC>(post, fft, spec, nrows, Nh, nhp, Frt, tw, partials, blockIdx.x,
-                                   reinterpret_cast<cplx<T>*>(smem_raw));
+  rows_inv_block<T, FFT, Post, FC, SL>(post, fft, spec, nrows, Nh, nhp, Frt, tw, partials, blockIdx.x,
+                                       reinterpret_cast<cplx<T>*>(smem_raw));
 }
 
 // Strided complex transform along one axis: blockIdx.y selects the outer
@@ -1636,15 +1680,124 @@ __device__ __forceinline__ void axis_reg_block(cplx<T>* __restrict__ spec, long 
   }
 }
 
+// The axis-1 transform of axis_reg_block over bf16 spectra (SL = 1 of
+// rows_fwd_block): Nh = n0*n1*nb words per field, stride = nb, ostride =
+// n1*nb, ncol = nb = n2/2 (a multiple of W), nouter = n0.  Tiles bx < nb/W
+// take W columns of plane by; the last tile slot, bx = nb/W, takes the
+// Nyquist columns of the W planes by*W .. by*W+W-1, stored contiguously
+// along the axis at word 3*Nh + (bz*n0 + plane)*N (workgroups with by*W >=
+// n0 have none).  The packed words are loaded first and unpacked after, so
+// that every load of the thread is in flight at once.
+template <typename T, int N1, int N2, int W>
+__device__ __forceinline__ void axis_bf_block(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+                                              long long ostride, int ncol, const cplx<T>* __restrict__ tw, int inv,
+                                              int bx, int by, int bz, cplx<T>* lds, int nouter) {
+  constexpr int S = col_block_stride<T, N1, N2, W>();
+  constexpr unsigned NA = (unsigned)(N1 * N2);
+  unsigned* sp = reinterpret_cast<unsigned*>(spec);
+  const int t = threadIdx.x, w = t % W, q = t / W;
+  const bool act = t < W * N2;
+  const bool iv = inv != 0;
+  const unsigned n0 = (unsigned)nouter;
+  const bool nyq = bx * W >= ncol;  // uniform over the workgroup
+  if (nyq && (unsigned)by * W >= n0) return;  // the whole workgroup: no plane left
+  const unsigned pl = (unsigned)(by * W + w);
+  const bool ok = act && (!nyq || pl < n0);
+  const unsigned off = nyq ? 3u * (unsigned)Nh + ((unsigned)bz * n0 + pl) * NA
+                           : (unsigned)bz * (unsigned)Nh + (unsigned)by * (unsigned)ostride + (unsigned)(bx * W + w);
+  const unsigned js = nyq ? 1u : (unsigned)stride;
+  unsigned raw[N1];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : 0u;
+  cplx<T> v[N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) v[n1] = unpack_bf16(raw[n1]);
+  if (act) {
+    dftreg<T, N1>(v, iv);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) {
+      cplx<T> wv = tw[q * k1];
+      if (iv) wv.y = -wv.y;
+      v[k1] = cmul(v[k1], wv);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
+  }
+  __syncthreads();
+  if (act && q < N1) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
+    dftreg<T, N2>(v, iv);
+    if (ok) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) at(sp, off + (unsigned)(q + N1 * k2) * js) = pack_bf16(v[k2]);
+    }
+  }
+}
+
 // Four-step register transform of W columns along a strided axis (one
 // workgroup, W*N2 threads): x -> X in place, natural order in and out.
-template <typename T, int N1, int N2, int W>
+// SL = 1: bf16 spectra (axis_bf_block); nouter: the number of planes.
+template <typename T, int N1, int N2, int W, int SL = 0>
 __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
                                                      long long ostride, int ncol, const cplx<T>* __restrict__ tw,
-                                                     int inv, const int* __restrict__ skip) {
+                                                     int inv, const int* __restrict__ skip, int nouter) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+  if constexpr (SL == 1)
+    axis_bf_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds,
+                                nouter);
+  else
+    axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+}
+
+// The multiplier step of an axis-0 pencil: thread (w, q) holds the points
+// k0 = q + N1*k2 (k2 < N2) of column (i1, c) after the forward transform;
+// each becomes M_f(G)/N times itself (live == false: zero), and the return
+// value is this thread's sum of M |X|^2 (its share of p.Ap in the TN loop).
+// wm: the WT multipliers of the points, loaded by the caller (wt: WT field).
+template <typename T, int N1, int N2>
+__device__ __forceinline__ double pencil_multiply(cplx<T>* v, const GridGeom& geo, int kind, double cscale,
+                                                  const T* __restrict__ wtk, const T* wm, bool wt, bool live, int i1,
+                                                  int c, int q) {
+  const T scale = T(cscale);
+  double pap = 0.0;
+  if constexpr (sizeof(T) == 4) {
+    if (live && !wt && !geo.full && (kind == OFDFT_CONV_HARTREE || kind == OFDFT_CONV_GG)) {
+      // FP32 Hartree / vW multipliers along the pencil: the (i1, c) part of
+      // G is fixed per thread, only the axis-0 frequency changes
+      const float m1 = (float)fftfreq_i(i1, geo.n1), m2 = (float)c;
+      const float px = m1 * geo.bgf[3] + m2 * geo.bgf[6];
+      const float py = m1 * geo.bgf[4] + m2 * geo.bgf[7];
+      const float pz = m1 * geo.bgf[5] + m2 * geo.bgf[8];
+      const bool hartree = kind == OFDFT_CONV_HARTREE;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const float m0 = (float)fftfreq_i(q + N1 * k2, N1 * N2);
+        const float gx = m0 * geo.bgf[0] + px, gy = m0 * geo.bgf[1] + py, gz = m0 * geo.bgf[2] + pz;
+        const float g2 = gx * gx + gy * gy + gz * gz;
+        float m = hartree ? (g2 > 0.f ? float(4.0 * kPi) * __builtin_amdgcn_rcpf(g2) : 0.f) : g2;
+        m *= scale;
+        pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
+        v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+      }
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+        pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
+        v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
+      pap += (double)m * ((double)(v[k2].x * v[k2].x) + (double)(v[k2].y * v[k2].y));
+      v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
+    }
+  }
+  return pap;
 }
 
 // Four-step axis-0 pencils of one workgroup: forward transform, multiplier,
@@ -1695,42 +1848,7 @@ __device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, lon
 #pragma unroll
     for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
     dftreg<T, N2>(v, false);
-    const T scale = T(ck.scale);
-    if constexpr (sizeof(T) == 4) {
-      if (live && !wt && !geo.full && (kind == OFDFT_CONV_HARTREE || kind == OFDFT_CONV_GG)) {
-        // FP32 Hartree / vW multipliers along the pencil: the (i1, c) part of
-        // G is fixed per thread, only the axis-0 frequency changes
-        const float m1 = (float)fftfreq_i(i1, geo.n1), m2 = (float)c;
-        const float px = m1 * geo.bgf[3] + m2 * geo.bgf[6];
-        const float py = m1 * geo.bgf[4] + m2 * geo.bgf[7];
-        const float pz = m1 * geo.bgf[5] + m2 * geo.bgf[8];
-        const bool hartree = kind == OFDFT_CONV_HARTREE;
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-          const float m0 = (float)fftfreq_i(q + N1 * k2, N1 * N2);
-          const float gx = m0 * geo.bgf[0] + px, gy = m0 * geo.bgf[1] + py, gz = m0 * geo.bgf[2] + pz;
-          const float g2 = gx * gx + gy * gy + gz * gz;
-          float m = hartree ? (g2 > 0.f ? float(4.0 * kPi) * __builtin_amdgcn_rcpf(g2) : 0.f) : g2;
-          m *= scale;
-          pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
-          v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
-        }
-      } else {
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-          const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
-          pap += (double)m * ((double)v[k2].x * v[k2].x + (double)v[k2].y * v[k2].y);
-          v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
-        }
-      }
-    } else {
-#pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) {
-        const T m = !live ? T(0) : (wt ? wm[k2] * scale : conv_mult<T>(geo, kind, q + N1 * k2, i1, c, scale, wtk));
-        pap += (double)m * ((double)(v[k2].x * v[k2].x) + (double)(v[k2].y * v[k2].y));
-        v[k2] = cplx<T>{v[k2].x * m, v[k2].y * m};
-      }
-    }
+    pap = pencil_multiply<T, N1, N2>(v, geo, kind, ck.scale, wtk, wm, wt, live, i1, c, q);
     dftreg<T, N2>(v, true);
 #pragma unroll
     for (int n2 = 1; n2 < N2; ++n2) {
@@ -1759,13 +1877,106 @@ __device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, lon
   }
 }
 
+// The axis-0 pencils of pencil_reg_block over bf16 spectra (SL = 1 of
+// rows_fwd_block; nb = n2/2 a multiple of W, NB = n0*n1*nb words per
+// field).  Tiles bx < nb/W take columns bx*W + w at axis-1 index by; the
+// last tile slot, bx = nb/W, takes the Nyquist columns of axis-1 indices
+// by*W .. by*W+W-1 (word 3*NB + bz*n0*n1 + k0*n1 + i1; workgroups with
+// by*W >= n1 have none and leave a zero p.Ap partial sum).  The packed words
+// are loaded first and unpacked after.
 template <typename T, int N1, int N2, int W>
+__device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, const GridGeom& geo,
+                                                const cplx<T>* __restrict__ tw, const ConvKinds& ck,
+                                                const T* __restrict__ wtk, double* __restrict__ cgp, long long pslot,
+                                                int bx, int by, int bz, cplx<T>* lds) {
+  constexpr int S = col_block_stride<T, N1, N2, W>();
+  unsigned* sp = reinterpret_cast<unsigned*>(spec);
+  const int t = threadIdx.x, w = t % W, q = t / W;
+  const bool act = t < W * N2;
+  const int f = bz;
+  const int kind = ck.kind[f];
+  const int nb = geo.nh - 1;
+  const bool nyq = bx * W >= nb;  // uniform over the workgroup
+  if (nyq && by * W >= geo.n1) {  // the whole workgroup: no pencil left
+    if (cgp && threadIdx.x == 0) cgp[pslot] = 0.0;
+    return;
+  }
+  const unsigned n01 = (unsigned)geo.n0 * (unsigned)geo.n1;
+  const unsigned NB = n01 * (unsigned)nb;
+  const int i1 = nyq ? by * W + w : by;
+  const int c = nyq ? nb : bx * W + w;
+  const bool ok = act && i1 < geo.n1;
+  const unsigned off = nyq ? 3u * NB + (unsigned)f * n01 + (unsigned)i1
+                           : (unsigned)f * NB + (unsigned)i1 * (unsigned)nb + (unsigned)c;
+  const unsigned js = nyq ? (unsigned)geo.n1 : (unsigned)geo.n1 * (unsigned)nb;  // k0 stride
+  unsigned raw[N1];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : 0u;
+  T wm[N2];
+  const bool wt = kind == OFDFT_CONV_WT && q < N1 && ok;
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2)
+    wm[k2] = wt ? at(wtk, ((unsigned)(q + N1 * k2) * (unsigned)geo.n1 + i1) * (unsigned)geo.nh + c) : T(0);
+  cplx<T> v[N2];
+#pragma unroll
+  for (int n1 = 0; n1 < N1; ++n1) v[n1] = unpack_bf16(raw[n1]);
+  if (act) {
+    dftreg<T, N1>(v, false);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) v[k1] = cmul(v[k1], tw[q * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) lds[w * S + k1 * (N2 + 1) + q] = v[k1];
+  }
+  __syncthreads();
+  double pap = 0.0;
+  if (act && q < N1) {
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) v[n2] = lds[w * S + q * (N2 + 1) + n2];
+    dftreg<T, N2>(v, false);
+    pap = pencil_multiply<T, N1, N2>(v, geo, kind, ck.scale, wtk, wm, wt, ok, i1, c, q);
+    dftreg<T, N2>(v, true);
+#pragma unroll
+    for (int n2 = 1; n2 < N2; ++n2) {
+      cplx<T> wv = tw[n2 * q];
+      wv.y = -wv.y;
+      v[n2] = cmul(v[n2], wv);
+    }
+    // this thread re-reads nothing it overwrites: row (w, q) is private to it
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) lds[w * S + q * (N2 + 1) + n2] = v[n2];
+  }
+  __syncthreads();
+  if (act) {
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) v[k1] = lds[w * S + k1 * (N2 + 1) + q];
+    dftreg<T, N1>(v, true);
+    if (ok) {
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) at(sp, off + (unsigned)(N2 * n1 + q) * js) = pack_bf16(v[n1]);
+    }
+  }
+  if (cgp) {
+    // Hermitian weight 2 off the c = 0 and Nyquist planes
+    const bool edge = nyq || c == 0;
+    double acc[1] = {ok ? pap * ck.coef[f] * (edge ? 1.0 : 2.0) : 0.0};
+    block_sums<1>(acc, cgp + pslot);
+  }
+}
+
+// SL = 1: bf16 spectra (pencil_bf_block).
+template <typename T, int N1, int N2, int W, int SL = 0>
 __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
                                                        const cplx<T>* __restrict__ tw, ConvKinds ck,
                                                        const T* __restrict__ wtk, const int* __restrict__ skip,
                                                        double* __restrict__ cgp) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
+  if constexpr (SL == 1) {
+    pencil_bf_block<T, N1, N2, W>(spec, geo, tw, ck, wtk, cgp,
+                                  ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, blockIdx.x,
+                                  blockIdx.y, blockIdx.z, lds);
+    return;
+  }
   pencil_reg_block<T, N1, N2, W>(spec, Nh, geo, tw, ck, wtk, cgp,
                                  ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, blockIdx.x,
                                  blockIdx.y, blockIdx.z, lds);
@@ -2069,13 +2280,17 @@ struct PapArgs {
 // pencil's partial sums are complete when this kernel starts, so its first
 // workgroup also completes p.Ap (one launch and one dispatch gap fewer per
 // CG iteration; the inverse row pass that reads p.Ap runs after it).
-template <typename T, int N1, int N2, int W>
+template <typename T, int N1, int N2, int W, int SL = 0>
 __global__ __launch_bounds__(W * N2) void k_axis_reg_pap(cplx<T>* __restrict__ spec, long long Nh, long long stride,
                                                          long long ostride, int ncol, const cplx<T>* __restrict__ tw,
-                                                         int inv, const int* __restrict__ skip, PapArgs pa) {
+                                                         int inv, const int* __restrict__ skip, PapArgs pa, int nouter) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
+  if constexpr (SL == 1)
+    axis_bf_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds,
+                                nouter);
+  else
+    axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
   if ((blockIdx.x | blockIdx.y | blockIdx.z) == 0 && !pa.st->done) tn_pap_block(pa.st, pa.part, pa.nblk, pa.red);
 }
 
@@ -2310,6 +2525,27 @@ constexpr bool pencil_fixed(int n) {
 // step) runs on compile-time lengths of plans with 32-bit offsets
 inline bool pencil_reg_ok(const ofdft_plan_s* P) { return P->idx32 && pencil_fixed(P->n[0]); }
 
+// Columns per workgroup of the bf16 spectrum passes of the TN inner loop
+// (*wa: axis pass, *wp: pencils); false when the plan keeps plan-precision
+// spectra.  bf16 needs the selection, an FP32 plan with 32-bit offsets,
+// compile-time lengths on all three axes and an even n2 whose half n2/2 is a
+// multiple of both tile widths (automatic: the widest of 32, 16, 8 that
+// divides it).
+inline bool cg_bf16_tiles(const ofdft_plan_s* P, int* wa = nullptr, int* wp = nullptr) {
+  if (P->cg_spectra != 1 || P->prec != OFDFT_F32 || !P->idx32) return false;
+  if (!pencil_fixed(P->n[0]) || !pencil_fixed(P->n[1]) || !pencil_fixed(P->n[2]) || (P->n[2] & 1)) return false;
+  const int nb = P->n[2] / 2;
+  int widest = 0;
+  for (int w = 32; w >= 8 && !widest; w /= 2)
+    if (nb % w == 0) widest = w;
+  const int a = P->tile_bf_axis ? P->tile_bf_axis : widest;
+  const int p = P->tile_bf_pencil ? P->tile_bf_pencil : widest;
+  if (!a || !p || nb % a || nb % p) return false;
+  if (wa) *wa = a;
+  if (wp) *wp = p;
+  return true;
+}
+
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
 // columns per workgroup of the register column kernels
 template <typename T>
@@ -2329,35 +2565,54 @@ using RowsFFT = FFTFixed<N, kRowPB<T> * 3, rows_th<N>()>;
 template <int N>
 using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
-template <typename T, class Pre, class FFT>
-void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
-  if (F == 3) {
-    OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 3>), grid, dim3(threads), lds, P, pre, fft,
-                 reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
-                 reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
-    return;
-  }
-  OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 0>), grid, dim3(threads), lds, P, pre, fft,
-                     reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F,
-                     reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
+// Spectrum layout of a row pass: SL = 0 the plan-precision spectrum (field
+// stride Nhp, row stride nhp); SL = 1 bf16 (field stride n0*n1*n2/2 words,
+// row stride n2/2, the Nyquist column apart).
+template <int SL>
+long long spec_field_stride(const ofdft_plan_s* P) {
+  return SL == 1 ? (long long)P->n[0] * P->n[1] * (P->n[2] / 2) : P->Nhp;
+}
+template <int SL>
+int spec_row_stride(const ofdft_plan_s* P) {
+  return SL == 1 ? P->n[2] / 2 : P->nhp;
+}
+// bytes of F spectra in a layout (read or written once)
+template <int SL>
+long long spec_bytes(const ofdft_plan_s* P, int F) {
+  return (SL == 1 ? 4LL : 2LL * (long long)P->rs) * F * P->Nh;
 }
 
-template <typename T, class Pre>
+template <typename T, int SL = 0, class Pre, class FFT>
+void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 grid, int threads, size_t lds) {
+  if (F == 3) {
+    OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 3, SL>), grid, dim3(threads), lds, P, pre, fft,
+                 reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], spec_field_stride<SL>(P),
+                 spec_row_stride<SL>(P), F, reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
+    return;
+  }
+  OFDFT_LAUNCH((k_rows_fwd<T, FFT, Pre, 0, SL>), grid, dim3(threads), lds, P, pre, fft,
+               reinterpret_cast<cplx<T>*>(P->spec), (long long)P->n[0] * P->n[1], spec_field_stride<SL>(P),
+               spec_row_stride<SL>(P), F, reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
+}
+
+// SL = 1 (bf16 spectra) runs on compile-time lengths only (cg_bf16_tiles)
+template <typename T, class Pre, int SL = 0>
 int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const long long nrows = (long long)P->n[0] * P->n[1];
   const long long npairs = (nrows + 1) / 2;
   const int PB = kRowPB<T>;
   const size_t lds = (size_t)(PB * F + 1) * P->n[2] * sizeof(cplx<T>);
   dim3 grid((unsigned)((npairs + PB - 1) / PB));
-  ProfScope ps(P, Pre::kProf, (long long)P->rs * (pre.nin() * P->N + 2LL * F * P->Nh));
+  ProfScope ps(P, Pre::kProf, (long long)P->rs * pre.nin() * P->N + spec_bytes<SL>(P, F));
   // compile-time lengths address with 32-bit offsets: plans too large for
   // them take the runtime-schedule kernels
   switch (P->idx32 ? P->n[2] : 0) {
 #define OFDFT_CASE(NN) \
-    case NN: go_rows_fwd<T>(P, pre, RowsFFT<T, NN>{}, F, grid, rows_th<NN>(), lds); break;
+    case NN: go_rows_fwd<T, SL>(P, pre, RowsFFT<T, NN>{}, F, grid, rows_th<NN>(), lds); break;
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
     default: {
+      if (SL == 1) return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time row length");
       FFTLaunch L = fft_launch(P->pl[2], PB * F);
       // radix 11 / 13 schedules: one instantiation, up to four slots per thread
       if (has_large(P->pl[2])) go_rows_fwd<T>(P, pre, FFTRuntime<4, true>{P->pl[2]}, F, grid, L.threads, lds);
@@ -2370,22 +2625,23 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   return 0;
 }
 
-template <typename T, class Post, class FFT>
+template <typename T, int SL = 0, class Post, class FFT>
 void go_rows_inv(ofdft_plan_s* P, const Post& post, const FFT& fft, int F, const cplx<T>* src, dim3 grid,
                  int threads, size_t lds, double* part) {
   if (F == 3) {
-    OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 3>), grid, dim3(threads), lds, P, post, fft, src,
-                 (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part,
-                 P->skip);
+    OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 3, SL>), grid, dim3(threads), lds, P, post, fft, src,
+                 (long long)P->n[0] * P->n[1], spec_field_stride<SL>(P), spec_row_stride<SL>(P), F,
+                 reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
     return;
   }
-  OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 0>), grid, dim3(threads), lds, P, post, fft, src,
-                     (long long)P->n[0] * P->n[1], P->Nhp, P->nhp, F, reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
+  OFDFT_LAUNCH((k_rows_inv<T, FFT, Post, 0, SL>), grid, dim3(threads), lds, P, post, fft, src,
+               (long long)P->n[0] * P->n[1], spec_field_stride<SL>(P), spec_row_stride<SL>(P), F,
+               reinterpret_cast<const cplx<T>*>(P->tw[2]), part, P->skip);
 }
 
 // reduce: the post functor's sums end in P->red, and also in the mapped host
 // buffer when to_host (for a fetch_sums that follows).
-template <typename T, class Post>
+template <typename T, class Post, int SL = 0>
 int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src, bool reduce, bool to_host = true,
                     bool final_reduce = true) {
   const long long nrows = (long long)P->n[0] * P->n[1];
@@ -2397,13 +2653,14 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
   dim3 grid((unsigned)nblk);
   double* part = reduce ? P->partials : nullptr;
   {
-    ProfScope ps(P, Post::kProf, (long long)P->rs * (post.nio() * P->N + 2LL * F * P->Nh));
+    ProfScope ps(P, Post::kProf, (long long)P->rs * post.nio() * P->N + spec_bytes<SL>(P, F));
     switch (P->idx32 ? P->n[2] : 0) {
 #define OFDFT_CASE(NN) \
-      case NN: go_rows_inv<T>(P, post, RowsFFT<T, NN>{}, F, src, grid, rows_th<NN>(), lds, part); break;
+      case NN: go_rows_inv<T, SL>(P, post, RowsFFT<T, NN>{}, F, src, grid, rows_th<NN>(), lds, part); break;
       OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
       default: {
+        if (SL == 1) return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time row length");
         FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
         // radix 11 / 13 schedules: one instantiation, up to four slots per thread
         if (has_large(P->pl[2])) go_rows_inv<T>(P, post, FFTRuntime<4, true>{P->pl[2]}, F, src, grid, L.threads, lds, part);
@@ -2430,17 +2687,23 @@ void go_axis(ofdft_plan_s* P, const FFT& fft, cplx<T>* spec, int axis, dim3 grid
                      P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
 }
 
+template <typename T, int N1, int N2, int W, int SL>
+void go_axis_reg(ofdft_plan_s* P, dim3 grid, cplx<T>* spec, long long Nh, long long stride, long long ostride,
+                 int ncol, const cplx<T>* tw, bool inv, const PapArgs* pa) {
+  if (pa) {
+    OFDFT_LAUNCH((k_axis_reg_pap<T, N1, N2, W, SL>), grid, dim3(W * N2), 0, P, spec, Nh, stride, ostride, ncol, tw,
+                 inv ? 1 : 0, P->skip, *pa, (int)grid.y);
+    return;
+  }
+  OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W, SL>), grid, dim3(W * N2), 0, P, spec, Nh, stride, ostride, ncol, tw,
+               inv ? 1 : 0, P->skip, (int)grid.y);
+}
+
 template <typename T, int N1, int N2, int W>
 void axis_reg(ofdft_plan_s* P, cplx<T>* spec, int axis, int F, long long stride, long long ostride, unsigned outer,
               bool inv, const PapArgs* pa) {
-  const dim3 grid((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F);
-  if (pa) {
-    OFDFT_LAUNCH((k_axis_reg_pap<T, N1, N2, W>), grid, dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
-                 reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip, *pa);
-    return;
-  }
-  OFDFT_LAUNCH((k_axis_reg<T, N1, N2, W>), grid, dim3(W * N2), 0, P, spec, P->Nhp, stride, ostride, P->nh,
-               reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv ? 1 : 0, P->skip);
+  go_axis_reg<T, N1, N2, W, 0>(P, dim3((unsigned)((P->nhp + W - 1) / W), outer, (unsigned)F), spec, P->Nhp, stride,
+                               ostride, P->nh, reinterpret_cast<const cplx<T>*>(P->tw[axis]), inv, pa);
 }
 
 // pa: also complete p.Ap of the TN step (k_tn_pap), folded into the
@@ -2500,11 +2763,22 @@ void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t l
                      reinterpret_cast<const T*>(P->wtk), P->skip);
 }
 
+template <typename T, int N1, int N2, int W, int SL>
+void go_pencil_reg(ofdft_plan_s* P, dim3 grid, const GridGeom& g, const ConvKinds& ck, double* cgp, long long Nh) {
+  OFDFT_LAUNCH((k_pencil_reg<T, N1, N2, W, SL>), grid, dim3(W * N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), Nh,
+               g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck, reinterpret_cast<const T*>(P->wtk), P->skip, cgp);
+}
+
+// workgroups of the register pencils of tile width W (bf: the bf16 layout's
+// nb/W column tiles plus the Nyquist slot); one p.Ap partial sum each
+inline dim3 pencil_grid(const ofdft_plan_s* P, int W, int F, bool bf) {
+  const unsigned tx = bf ? (unsigned)(P->n[2] / 2 / W + 1) : (unsigned)((P->nhp + W - 1) / W);
+  return dim3(tx, (unsigned)P->n[1], (unsigned)F);
+}
+
 template <typename T, int N1, int N2, int W>
 void pencil_reg(ofdft_plan_s* P, int F, const GridGeom& g, const ConvKinds& ck, double* cgp) {
-  OFDFT_LAUNCH((k_pencil_reg<T, N1, N2, W>), dim3((unsigned)((P->nhp + W - 1) / W), (unsigned)P->n[1], (unsigned)F),
-               dim3(W * N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), P->Nhp, g,
-               reinterpret_cast<const cplx<T>*>(P->tw[0]), ck, reinterpret_cast<const T*>(P->wtk), P->skip, cgp);
+  go_pencil_reg<T, N1, N2, W, 0>(P, pencil_grid(P, W, F, false), g, ck, cgp, P->Nhp);
 }
 
 template <typename T>
@@ -2544,6 +2818,82 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
   }
   LAUNCHCHK();
   return 0;
+}
+
+// The axis-1 pass over bf16 spectra (SL = 1 of axis_reg_block): nb/W column
+// tiles plus one Nyquist tile slot per plane.
+template <typename T, int N1, int N2, int W>
+void axis_reg_bf(ofdft_plan_s* P, int F, bool inv, const PapArgs* pa) {
+  const int nb = P->n[2] / 2;
+  go_axis_reg<T, N1, N2, W, 1>(P, dim3((unsigned)(nb / W + 1), (unsigned)P->n[0], (unsigned)F),
+                               reinterpret_cast<cplx<T>*>(P->spec), spec_field_stride<1>(P), nb,
+                               (long long)P->n[1] * nb, nb, reinterpret_cast<const cplx<T>*>(P->tw[1]), inv, pa);
+}
+
+template <typename T>
+int launch_axis_bf(ofdft_plan_s* P, int F, bool inv, const PapArgs* pa = nullptr) {
+  if constexpr (sizeof(T) != 4) {
+    return fail(OFDFT_EINVAL, "bf16 spectra are for FP32 plans");
+  } else {
+    int W = 0;
+    if (!cg_bf16_tiles(P, &W, nullptr)) return fail(OFDFT_EINVAL, "bf16 spectra do not apply to this plan");
+    ProfScope ps(P, PC_AXIS, 2 * spec_bytes<1>(P, F));
+    switch (P->n[1]) {
+#define OFDFT_CASE(NN)                                                                                     \
+      case NN:                                                                                             \
+        if (W == 8) axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, inv, pa);                       \
+        else if (W == 16) axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 16>(P, F, inv, pa);                \
+        else axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 32>(P, F, inv, pa);                             \
+        break;
+      OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+      default:
+        return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time axis-1 length");
+    }
+    LAUNCHCHK();
+    return 0;
+  }
+}
+
+// The axis-0 pencils over bf16 spectra (SL = 1 of pencil_reg_block); cgp
+// receives one partial sum per workgroup, (nb/W + 1) * n1 * F of them.
+template <typename T, int N1, int N2, int W>
+void pencil_reg_bf(ofdft_plan_s* P, int F, const GridGeom& g, const ConvKinds& ck, double* cgp) {
+  go_pencil_reg<T, N1, N2, W, 1>(P, pencil_grid(P, W, F, true), g, ck, cgp, spec_field_stride<1>(P));
+}
+
+template <typename T>
+int launch_pencil_bf(ofdft_plan_s* P, int F, const int* kinds, double* cgp, const double* coef) {
+  if constexpr (sizeof(T) != 4) {
+    return fail(OFDFT_EINVAL, "bf16 spectra are for FP32 plans");
+  } else {
+    int W = 0;
+    if (!cg_bf16_tiles(P, nullptr, &W)) return fail(OFDFT_EINVAL, "bf16 spectra do not apply to this plan");
+    ConvKinds ck;
+    for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
+    ck.scale = 1.0 / (double)P->N;
+    for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
+    for (int f = 0; f < F; ++f)
+      if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
+    GridGeom g = geom(P);
+    long long nwt = 0;
+    for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
+    ProfScope ps(P, PC_PENCIL, 2 * spec_bytes<1>(P, F) + (long long)P->rs * nwt * P->Nh);
+    switch (P->n[0]) {
+#define OFDFT_CASE(NN)                                                                                 \
+      case NN:                                                                                         \
+        if (W == 8) pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, g, ck, cgp);              \
+        else if (W == 16) pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 16>(P, F, g, ck, cgp);       \
+        else pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 32>(P, F, g, ck, cgp);                    \
+        break;
+      OFDFT_FIXED_SIZES(OFDFT_CASE)
+#undef OFDFT_CASE
+      default:
+        return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time pencil length");
+    }
+    LAUNCHCHK();
+    return 0;
+  }
 }
 
 // the final reduction kernel also wrote its sums into the mapped host buffer
@@ -2683,11 +3033,23 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
   auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
   int rc;
   long long nblk_pen = 0;
+  // bf16 spectra between the passes (FP32 plans that select them): the
+  // same transforms and functors, half the spectrum bytes
+  int wbf_axis = 0, wbf = 0;  // wbf: the bf16 pencils' tile width, 0 = plan-precision spectra
+  if (sizeof(T) == 4 && !cg_bf16_tiles(P, &wbf_axis, &wbf)) wbf = 0;
   if (F > 0) {
     PreHvp<T, LAY> pre{phi, p, fp};
     PreHvpCG<T, LAY> pcg{pre, phi, p, r, st};
-    if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+    if constexpr (sizeof(T) == 4) {
+      if (wbf) {
+        if ((rc = launch_rows_fwd<T, PreHvpCG<T, LAY>, 1>(P, pcg, F))) return rc;
+        if ((rc = launch_axis_bf<T>(P, F, false))) return rc;
+      }
+    }
+    if (!wbf) {
+      if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
+      if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+    }
     int kinds[3] = {0, 0, 0};
     double coef[3] = {0.0, 0.0, 0.0};
     // p.Ap = sum W0 p^2 + sum_f coef_f g_f.(K_f * g_f) with g_f the field the
@@ -2705,11 +3067,16 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
       kinds[fp.fW] = OFDFT_CONV_WT;
       coef[fp.fW] = fp.alpha / fp.beta;
     }
-    if ((rc = launch_pencil<T>(P, F, kinds, P->partials, coef))) return rc;
-    nblk_pen = (long long)((P->nhp + P->tile_pencil - 1) / P->tile_pencil) * P->n[1] * F;
+    {
+      const dim3 pg = pencil_grid(P, wbf ? wbf : P->tile_pencil, F, wbf != 0);
+      nblk_pen = (long long)pg.x * pg.y * pg.z;
+    }
+    if (nblk_pen > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
+    if ((rc = wbf ? launch_pencil_bf<T>(P, F, kinds, P->partials, coef) : launch_pencil<T>(P, F, kinds, P->partials, coef)))
+      return rc;
     // p.Ap is completed by the inverse axis pass (k_tn_pap folded in)
     const PapArgs pa{st, (const double*)P->partials, nblk_pen, (double*)P->red};
-    if ((rc = launch_axis<T>(P, spec, 1, F, true, &pa))) return rc;
+    if ((rc = wbf ? launch_axis_bf<T>(P, F, true, &pa) : launch_axis<T>(P, spec, 1, F, true, &pa))) return rc;
   } else {
     hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
                        P->red);
@@ -2717,7 +3084,14 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
   }
   PostHvpFast<T, LAY> h{phi, p, W0, nullptr, fp};
   PostHvpCG<T, LAY> post{h, d, r, best, st, (const double*)P->red};
-  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
+  if constexpr (sizeof(T) == 4) {
+    if (wbf && F > 0) {
+      if ((rc = launch_rows_inv<T, PostHvpCG<T, LAY>, 1>(P, post, F, spec, true, false, false))) return rc;
+    }
+  }
+  if (!(wbf && F > 0)) {
+    if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
+  }
   const long long npairs = ((long long)P->n[0] * P->n[1] + 1) / 2;
   *nblk_rows = (npairs + kRowPB<T> - 1) / kRowPB<T>;
   return 0;

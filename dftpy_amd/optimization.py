@@ -27,6 +27,12 @@ Options beyond the reference's:
   tn_graph       True (default): the device TN loop replays one captured
                  hipGraph per CG iteration instead of launching its kernels
                  (same results; about 3 % faster at 192^3); False: launches
+  cg_spectra     'plan' (default): the exact-HVP inner CG keeps its spectra
+                 in the work precision; 'bf16': FP32 inner CG loops store
+                 the spectra between transform passes as bf16 (the
+                 transforms still compute in FP32), on plans whose lengths
+                 allow it (Plan.cg_spectra_tile); the directions then carry
+                 bf16 rounding, the energies and the line search do not
 """
 import math
 import time
@@ -174,10 +180,14 @@ class Optimization:
         defaults = {"maxcor": 6, "ftol": 1.0e-7, "xtol": 1.0e-12, "maxfun": 50, "maxiter": 200, "maxls": 30,
                     "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
                     "ncheck": 2, "h0": 1.0, "hvp": "exact", "precision": None, "mixed_switch": None,
-                    "tn_graph": True,
+                    "tn_graph": True, "cg_spectra": "plan",
                     "ls_first_step": "unit"}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
+        if self.optimization_options["cg_spectra"] not in ("plan", "bf16"):
+            # storage of the spectra in the exact-HVP inner CG: the plan
+            # precision, or bf16 on FP32 plans (every transform still in FP32)
+            raise ValueError("cg_spectra must be 'plan' or 'bf16', not %r" % (self.optimization_options["cg_spectra"],))
         if EnergyEvaluator is None:
             raise AttributeError("Must provide an energy evaluator")
         if not callable(EnergyEvaluator):
@@ -460,6 +470,7 @@ class Optimization:
             self._works = {k: w for k, w in self._works.items() if k[0] != prec}
             self._works[key] = _Work(ev)
         ev.P.set_graph(bool(self.optimization_options.get("tn_graph")))
+        ev.P.set_cg_spectra(self.optimization_options["cg_spectra"])
         return self._works[key]
 
     def _residual(self, W, mu):

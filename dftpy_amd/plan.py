@@ -332,6 +332,32 @@ class Plan:
         L.check(self.lib.ofdft_plan_set_tiles(self._h, int(tile_axis), int(tile_pencil)))
         self.tiles = (int(tile_axis), int(tile_pencil))
 
+    CG_SPECTRA = {"plan": 0, "bf16": 1}
+
+    def set_cg_spectra(self, mode="plan", tiles=None):
+        """Storage of the spectra inside the device TN loop's Hessian
+        products: "plan" (the plan precision) or "bf16" (FP32 plans of
+        compile-time lengths; every transform still computes in FP32).
+        tiles: (axis, pencil) columns per workgroup of the bf16 passes; None
+        takes the autotuned widths for these lengths, 0 the widest that
+        divides n2/2 (ofdft_plan_set_cg_spectra)."""
+        if mode not in self.CG_SPECTRA:
+            raise ValueError("cg spectra mode must be one of %s, not %r" % (sorted(self.CG_SPECTRA), mode))
+        if tiles is None:
+            from . import tuning
+            tiles = tuning.bf16_tiles_for(self.nr) if mode == "bf16" else (0, 0)
+        L.check(self.lib.ofdft_plan_set_cg_spectra(self._h, self.CG_SPECTRA[mode], int(tiles[0]), int(tiles[1])))
+
+    @property
+    def cg_spectra_tiles(self):
+        """(axis, pencil) tile widths of the bf16 spectra the next device TN
+        loop uses, (0, 0) when it keeps plan-precision spectra
+        (ofdft_plan_cg_spectra)."""
+        a, p = ctypes.c_int(), ctypes.c_int()
+        mode = self.lib.ofdft_plan_cg_spectra(self._h, ctypes.byref(a), ctypes.byref(p))
+        L.check(min(mode, 0))
+        return (a.value, p.value)
+
     def set_graph(self, on=True):
         """Run the device TN loop's CG iterations as replays of one captured
         hipGraph (ofdft_plan_set_graph)."""
