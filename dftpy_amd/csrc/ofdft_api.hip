@@ -281,8 +281,8 @@ int ofdft_plan_set_tiles(ofdft_plan P, int tile_axis, int tile_pencil) {
 
 int ofdft_plan_set_cg_spectra(ofdft_plan P, int mode, int tile_axis, int tile_pencil) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
-  if (mode != OFDFT_CG_SPECTRA_PLAN && mode != OFDFT_CG_SPECTRA_BF16)
-    return fail(OFDFT_EINVAL, "CG spectra mode must be OFDFT_CG_SPECTRA_PLAN or OFDFT_CG_SPECTRA_BF16");
+  if (mode != OFDFT_CG_SPECTRA_PLAN && mode != OFDFT_CG_SPECTRA_BF16 && mode != OFDFT_CG_SPECTRA_PADDED)
+    return fail(OFDFT_EINVAL, "CG spectra mode must be OFDFT_CG_SPECTRA_PLAN, _BF16 or _PADDED");
   auto ok = [](int w) { return w == 0 || w == 8 || w == 16 || w == 32; };
   if (!ok(tile_axis) || !ok(tile_pencil))
     return fail(OFDFT_EINVAL, "bf16 tile widths must be 0 (automatic), 8, 16 or 32");
@@ -300,10 +300,10 @@ int ofdft_plan_set_cg_spectra(ofdft_plan P, int mode, int tile_axis, int tile_pe
 int ofdft_plan_cg_spectra(ofdft_plan P, int* tile_axis, int* tile_pencil) {
   if (!valid(P)) return fail(OFDFT_EINVAL, "null plan");
   int a = 0, p = 0;
-  const bool bf = cg_bf16_tiles(P, &a, &p);
+  const int sl = cg_layout(P, &a, &p);
   if (tile_axis) *tile_axis = a;
   if (tile_pencil) *tile_pencil = p;
-  return bf ? OFDFT_CG_SPECTRA_BF16 : OFDFT_CG_SPECTRA_PLAN;
+  return sl == 1 ? OFDFT_CG_SPECTRA_BF16 : (sl == 2 ? OFDFT_CG_SPECTRA_PLAN : OFDFT_CG_SPECTRA_PADDED);
 }
 
 int ofdft_plan_set_graph(ofdft_plan P, int on) {

@@ -390,6 +390,21 @@ __device__ __forceinline__ cplx<float> unpack_bf16(unsigned u) {
   return cplx<float>{__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u)};
 }
 
+// A spectrum value as stored in layout SL: SL = 0 (padded rows) and SL = 2
+// (split rows) keep the plan precision, SL = 1 (split rows) a bf16 pair.
+template <typename T, int SL>
+struct SpecElem {
+  using E = cplx<T>;
+  static __device__ __forceinline__ E put(cplx<T> v) { return v; }
+  static __device__ __forceinline__ cplx<T> get(E e) { return e; }
+};
+template <>
+struct SpecElem<float, 1> {
+  using E = unsigned;
+  static __device__ __forceinline__ E put(cplx<float> v) { return pack_bf16(v); }
+  static __device__ __forceinline__ cplx<float> get(E e) { return unpack_bf16(e); }
+};
+
 // Select one of three per-point values without a dynamically indexed array.
 template <typename T>
 __device__ __forceinline__ T pick3(const T* c, int f) {
@@ -1258,10 +1273,11 @@ constexpr int fixed_len() {
 // becomes constant); FC = 0: F taken from Frt.
 // One workgroup's share of the forward row pass (row pairs bx*PB ..):
 // the body of k_rows_fwd, also run by the persistent batched TN kernel.
-// SL = 1 (FP32, compile-time even length): the spectrum is stored in bf16,
-// columns 0 .. n2/2-1 of field f and row r at word f*Nh + r*nhp + k (the
-// caller passes Nh = n0*n1*n2/2 and nhp = n2/2), the Nyquist column apart
-// at word 3*Nh + f*nrows + r.
+// SL = 1 or 2 (compile-time even length; split rows): columns 0 .. n2/2-1
+// of field f and row r at element f*Nh + r*nhp + k (the caller passes Nh =
+// n0*n1*n2/2 and nhp = n2/2), the Nyquist column apart at element 3*Nh +
+// f*nrows + r; SL = 1 stores bf16 pairs (FP32 plans), SL = 2 the plan
+// precision.
 template <typename T, class FFT, class Pre, int FC, int SL = 0>
 __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, cplx<T>* __restrict__ spec,
                                                long long nrows, long long Nh, int nhp, int Frt,
@@ -1327,15 +1343,16 @@ __device__ __forceinline__ void rows_fwd_block(const Pre& pre, const FFT& fft, c
     const cplx<T> zm = lds[tr * ld + (k == 0 ? 0 : n2 - k)];
     const cplx<T> A{T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
     const cplx<T> B{T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
-    if constexpr (SL == 1) {
-      unsigned* sb = reinterpret_cast<unsigned*>(spec);
+    if constexpr (SL >= 1) {
+      using SE = SpecElem<T, SL>;
+      auto* sb = reinterpret_cast<typename SE::E*>(spec);
       const unsigned r0 = 2u * ((unsigned)pair0 + lp), r1 = r0 + 1;
       const unsigned nb = (unsigned)nhp;
       const unsigned o = (unsigned)k < nb ? (unsigned)f * (unsigned)Nh + r0 * nb + k
                                           : 3u * (unsigned)Nh + (unsigned)f * (unsigned)nrows + r0;
       const unsigned o1 = (unsigned)k < nb ? o + nb : o + 1u;
-      if (r0 < (unsigned)nrows) at(sb, o) = pack_bf16(A);
-      if (r1 < (unsigned)nrows) at(sb, o1) = pack_bf16(B);
+      if (r0 < (unsigned)nrows) at(sb, o) = SE::put(A);
+      if (r1 < (unsigned)nrows) at(sb, o1) = SE::put(B);
     } else if constexpr (FFT::kFixed) {
       const unsigned r0 = 2u * ((unsigned)pair0 + lp), r1 = r0 + 1;
       const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
@@ -1369,7 +1386,7 @@ __global__ __launch_bounds__(FFT::TH) void k_rows_fwd(Pre pre, FFT fft, cplx<T>*
 // One workgroup's share of the inverse row pass: the body of k_rows_inv,
 // also run by the persistent batched TN kernel; the workgroup's sums go to
 // partials[bx * NS ..] (none when partials is null).
-// SL = 1: the bf16 spectrum layout of rows_fwd_block's SL = 1.
+// SL = 1, 2: the split-row spectrum layouts of rows_fwd_block.
 template <typename T, class FFT, class Post, int FC, int SL = 0>
 __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft, const cplx<T>* __restrict__ spec,
                                                long long nrows, long long Nh, int nhp, int Frt,
@@ -1404,15 +1421,16 @@ __device__ __forceinline__ void rows_inv_block(const Post& post, const FFT& fft,
     const int tr = t / nh, k = t - tr * nh;
     const int lp = tr / F, f = tr - lp * F;
     cplx<T> A{T(0), T(0)}, B{T(0), T(0)};
-    if constexpr (SL == 1) {
-      const unsigned* sb = reinterpret_cast<const unsigned*>(spec);
+    if constexpr (SL >= 1) {
+      using SE = SpecElem<T, SL>;
+      const auto* sb = reinterpret_cast<const typename SE::E*>(spec);
       const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
       const unsigned nb = (unsigned)nhp;
       const unsigned o = (unsigned)k < nb ? (unsigned)f * (unsigned)Nh + r0 * nb + k
                                           : 3u * (unsigned)Nh + (unsigned)f * nr + r0;
       const unsigned o1 = (unsigned)k < nb ? o + nb : o + 1u;
-      if (r0 < nr) A = unpack_bf16(at(sb, o));
-      if (r1 < nr) B = unpack_bf16(at(sb, o1));
+      if (r0 < nr) A = SE::get(at(sb, o));
+      if (r1 < nr) B = SE::get(at(sb, o1));
     } else if constexpr (FFT::kFixed) {
       const unsigned r0 = 2u * (p0 + lp), r1 = r0 + 1;
       const unsigned o = (unsigned)f * (unsigned)Nh + r0 * (unsigned)nhp + k;
@@ -1680,21 +1698,24 @@ __device__ __forceinline__ void axis_reg_block(cplx<T>* __restrict__ spec, long 
   }
 }
 
-// The axis-1 transform of axis_reg_block over bf16 spectra (SL = 1 of
-// rows_fwd_block): Nh = n0*n1*nb words per field, stride = nb, ostride =
-// n1*nb, ncol = nb = n2/2 (a multiple of W), nouter = n0.  Tiles bx < nb/W
-// take W columns of plane by; the last tile slot, bx = nb/W, takes the
-// Nyquist columns of the W planes by*W .. by*W+W-1, stored contiguously
-// along the axis at word 3*Nh + (bz*n0 + plane)*N (workgroups with by*W >=
-// n0 have none).  The packed words are loaded first and unpacked after, so
-// that every load of the thread is in flight at once.
-template <typename T, int N1, int N2, int W>
-__device__ __forceinline__ void axis_bf_block(cplx<T>* __restrict__ spec, long long Nh, long long stride,
+// The axis-1 transform of axis_reg_block over the split-row spectra (SL =
+// 1 bf16, SL = 2 plan precision; rows_fwd_block): Nh = n0*n1*nb elements per
+// field, stride = nb, ostride = n1*nb, ncol = nb = n2/2 (a multiple of W),
+// nouter = n0.  Tiles bx < nb/W take W columns of plane by; the last tile
+// slot, bx = nb/W, takes the Nyquist columns of the W planes by*W ..
+// by*W+W-1, stored contiguously along the axis at element 3*Nh + (bz*n0 +
+// plane)*N (workgroups with by*W >= n0 have none), so that no workgroup
+// transforms a mostly empty tile.  The stored elements are loaded first and
+// converted after, so that every load of the thread is in flight at once.
+template <typename T, int N1, int N2, int W, int SL>
+__device__ __forceinline__ void axis_split_block(cplx<T>* __restrict__ spec, long long Nh, long long stride,
                                               long long ostride, int ncol, const cplx<T>* __restrict__ tw, int inv,
                                               int bx, int by, int bz, cplx<T>* lds, int nouter) {
   constexpr int S = col_block_stride<T, N1, N2, W>();
   constexpr unsigned NA = (unsigned)(N1 * N2);
-  unsigned* sp = reinterpret_cast<unsigned*>(spec);
+  using SE = SpecElem<T, SL>;
+  using E = typename SE::E;
+  E* sp = reinterpret_cast<E*>(spec);
   const int t = threadIdx.x, w = t % W, q = t / W;
   const bool act = t < W * N2;
   const bool iv = inv != 0;
@@ -1706,12 +1727,12 @@ __device__ __forceinline__ void axis_bf_block(cplx<T>* __restrict__ spec, long l
   const unsigned off = nyq ? 3u * (unsigned)Nh + ((unsigned)bz * n0 + pl) * NA
                            : (unsigned)bz * (unsigned)Nh + (unsigned)by * (unsigned)ostride + (unsigned)(bx * W + w);
   const unsigned js = nyq ? 1u : (unsigned)stride;
-  unsigned raw[N1];
+  E raw[N1];
 #pragma unroll
-  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : 0u;
+  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : E{};
   cplx<T> v[N2];
 #pragma unroll
-  for (int n1 = 0; n1 < N1; ++n1) v[n1] = unpack_bf16(raw[n1]);
+  for (int n1 = 0; n1 < N1; ++n1) v[n1] = SE::get(raw[n1]);
   if (act) {
     dftreg<T, N1>(v, iv);
 #pragma unroll
@@ -1730,23 +1751,23 @@ __device__ __forceinline__ void axis_bf_block(cplx<T>* __restrict__ spec, long l
     dftreg<T, N2>(v, iv);
     if (ok) {
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) at(sp, off + (unsigned)(q + N1 * k2) * js) = pack_bf16(v[k2]);
+      for (int k2 = 0; k2 < N2; ++k2) at(sp, off + (unsigned)(q + N1 * k2) * js) = SE::put(v[k2]);
     }
   }
 }
 
 // Four-step register transform of W columns along a strided axis (one
 // workgroup, W*N2 threads): x -> X in place, natural order in and out.
-// SL = 1: bf16 spectra (axis_bf_block); nouter: the number of planes.
+// SL = 1, 2: split-row spectra (axis_split_block); nouter: the number of planes.
 template <typename T, int N1, int N2, int W, int SL = 0>
 __global__ __launch_bounds__(W * N2) void k_axis_reg(cplx<T>* __restrict__ spec, long long Nh, long long stride,
                                                      long long ostride, int ncol, const cplx<T>* __restrict__ tw,
                                                      int inv, const int* __restrict__ skip, int nouter) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  if constexpr (SL == 1)
-    axis_bf_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds,
-                                nouter);
+  if constexpr (SL >= 1)
+    axis_split_block<T, N1, N2, W, SL>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z,
+                                       lds, nouter);
   else
     axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
 }
@@ -1877,20 +1898,22 @@ __device__ __forceinline__ void pencil_reg_block(cplx<T>* __restrict__ spec, lon
   }
 }
 
-// The axis-0 pencils of pencil_reg_block over bf16 spectra (SL = 1 of
-// rows_fwd_block; nb = n2/2 a multiple of W, NB = n0*n1*nb words per
-// field).  Tiles bx < nb/W take columns bx*W + w at axis-1 index by; the
-// last tile slot, bx = nb/W, takes the Nyquist columns of axis-1 indices
-// by*W .. by*W+W-1 (word 3*NB + bz*n0*n1 + k0*n1 + i1; workgroups with
-// by*W >= n1 have none and leave a zero p.Ap partial sum).  The packed words
-// are loaded first and unpacked after.
-template <typename T, int N1, int N2, int W>
-__device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, const GridGeom& geo,
+// The axis-0 pencils of pencil_reg_block over the split-row spectra (SL =
+// 1 bf16, SL = 2 plan precision; rows_fwd_block; nb = n2/2 a multiple of W,
+// NB = n0*n1*nb elements per field).  Tiles bx < nb/W take columns bx*W + w
+// at axis-1 index by; the last tile slot, bx = nb/W, takes the Nyquist
+// columns of axis-1 indices by*W .. by*W+W-1 (element 3*NB + bz*n0*n1 +
+// k0*n1 + i1; workgroups with by*W >= n1 have none and leave a zero p.Ap
+// partial sum).  The stored elements are loaded first and converted after.
+template <typename T, int N1, int N2, int W, int SL>
+__device__ __forceinline__ void pencil_split_block(cplx<T>* __restrict__ spec, const GridGeom& geo,
                                                 const cplx<T>* __restrict__ tw, const ConvKinds& ck,
                                                 const T* __restrict__ wtk, double* __restrict__ cgp, long long pslot,
                                                 int bx, int by, int bz, cplx<T>* lds) {
   constexpr int S = col_block_stride<T, N1, N2, W>();
-  unsigned* sp = reinterpret_cast<unsigned*>(spec);
+  using SE = SpecElem<T, SL>;
+  using E = typename SE::E;
+  E* sp = reinterpret_cast<E*>(spec);
   const int t = threadIdx.x, w = t % W, q = t / W;
   const bool act = t < W * N2;
   const int f = bz;
@@ -1909,9 +1932,9 @@ __device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, cons
   const unsigned off = nyq ? 3u * NB + (unsigned)f * n01 + (unsigned)i1
                            : (unsigned)f * NB + (unsigned)i1 * (unsigned)nb + (unsigned)c;
   const unsigned js = nyq ? (unsigned)geo.n1 : (unsigned)geo.n1 * (unsigned)nb;  // k0 stride
-  unsigned raw[N1];
+  E raw[N1];
 #pragma unroll
-  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : 0u;
+  for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : E{};
   T wm[N2];
   const bool wt = kind == OFDFT_CONV_WT && q < N1 && ok;
 #pragma unroll
@@ -1919,7 +1942,7 @@ __device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, cons
     wm[k2] = wt ? at(wtk, ((unsigned)(q + N1 * k2) * (unsigned)geo.n1 + i1) * (unsigned)geo.nh + c) : T(0);
   cplx<T> v[N2];
 #pragma unroll
-  for (int n1 = 0; n1 < N1; ++n1) v[n1] = unpack_bf16(raw[n1]);
+  for (int n1 = 0; n1 < N1; ++n1) v[n1] = SE::get(raw[n1]);
   if (act) {
     dftreg<T, N1>(v, false);
 #pragma unroll
@@ -1952,7 +1975,7 @@ __device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, cons
     dftreg<T, N1>(v, true);
     if (ok) {
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) at(sp, off + (unsigned)(N2 * n1 + q) * js) = pack_bf16(v[n1]);
+      for (int n1 = 0; n1 < N1; ++n1) at(sp, off + (unsigned)(N2 * n1 + q) * js) = SE::put(v[n1]);
     }
   }
   if (cgp) {
@@ -1963,7 +1986,7 @@ __device__ __forceinline__ void pencil_bf_block(cplx<T>* __restrict__ spec, cons
   }
 }
 
-// SL = 1: bf16 spectra (pencil_bf_block).
+// SL = 1, 2: split-row spectra (pencil_split_block).
 template <typename T, int N1, int N2, int W, int SL = 0>
 __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spec, long long Nh, GridGeom geo,
                                                        const cplx<T>* __restrict__ tw, ConvKinds ck,
@@ -1971,10 +1994,10 @@ __global__ __launch_bounds__(W * N2) void k_pencil_reg(cplx<T>* __restrict__ spe
                                                        double* __restrict__ cgp) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  if constexpr (SL == 1) {
-    pencil_bf_block<T, N1, N2, W>(spec, geo, tw, ck, wtk, cgp,
-                                  ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x, blockIdx.x,
-                                  blockIdx.y, blockIdx.z, lds);
+  if constexpr (SL >= 1) {
+    pencil_split_block<T, N1, N2, W, SL>(spec, geo, tw, ck, wtk, cgp,
+                                         ((long long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x,
+                                         blockIdx.x, blockIdx.y, blockIdx.z, lds);
     return;
   }
   pencil_reg_block<T, N1, N2, W>(spec, Nh, geo, tw, ck, wtk, cgp,
@@ -2286,9 +2309,9 @@ __global__ __launch_bounds__(W * N2) void k_axis_reg_pap(cplx<T>* __restrict__ s
                                                          int inv, const int* __restrict__ skip, PapArgs pa, int nouter) {
   if (skip && *skip) return;  // a finished device-side CG loop skips the rest of its queue
   __shared__ cplx<T> lds[W * col_block_stride<T, N1, N2, W>()];
-  if constexpr (SL == 1)
-    axis_bf_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds,
-                                nouter);
+  if constexpr (SL >= 1)
+    axis_split_block<T, N1, N2, W, SL>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z,
+                                       lds, nouter);
   else
     axis_reg_block<T, N1, N2, W>(spec, Nh, stride, ostride, ncol, tw, inv, blockIdx.x, blockIdx.y, blockIdx.z, lds);
   if ((blockIdx.x | blockIdx.y | blockIdx.z) == 0 && !pa.st->done) tn_pap_block(pa.st, pa.part, pa.nblk, pa.red);
@@ -2525,25 +2548,46 @@ constexpr bool pencil_fixed(int n) {
 // step) runs on compile-time lengths of plans with 32-bit offsets
 inline bool pencil_reg_ok(const ofdft_plan_s* P) { return P->idx32 && pencil_fixed(P->n[0]); }
 
-// Columns per workgroup of the bf16 spectrum passes of the TN inner loop
-// (*wa: axis pass, *wp: pencils); false when the plan keeps plan-precision
-// spectra.  bf16 needs the selection, an FP32 plan with 32-bit offsets,
-// compile-time lengths on all three axes and an even n2 whose half n2/2 is a
-// multiple of both tile widths (automatic: the widest of 32, 16, 8 that
-// divides it).
-inline bool cg_bf16_tiles(const ofdft_plan_s* P, int* wa = nullptr, int* wp = nullptr) {
-  if (P->cg_spectra != 1 || P->prec != OFDFT_F32 || !P->idx32) return false;
-  if (!pencil_fixed(P->n[0]) || !pencil_fixed(P->n[1]) || !pencil_fixed(P->n[2]) || (P->n[2] & 1)) return false;
+// Split rows (n2/2 columns, the Nyquist column apart) are possible: not
+// deselected, 32-bit offsets, compile-time lengths on all three axes, n2
+// even.
+inline bool split_rows_ok(const ofdft_plan_s* P) {
+  return P->cg_spectra != OFDFT_CG_SPECTRA_PADDED && P->idx32 && pencil_fixed(P->n[0]) && pencil_fixed(P->n[1]) &&
+         pencil_fixed(P->n[2]) && !(P->n[2] & 1);
+}
+
+// The spectrum layout of the passes of an evaluation or a Hessian product
+// (ofdft_plan_set_cg_spectra) and its tile widths (*wa: axis pass, *wp:
+// pencils): 2 split rows in the plan precision (the default) where
+// split_rows_ok and n2/2 is a multiple of the plan's column tile widths
+// (ofdft_plan_set_tiles), else 0 padded rows.
+inline int pass_layout(const ofdft_plan_s* P, int* wa = nullptr, int* wp = nullptr) {
   const int nb = P->n[2] / 2;
-  int widest = 0;
-  for (int w = 32; w >= 8 && !widest; w /= 2)
-    if (nb % w == 0) widest = w;
-  const int a = P->tile_bf_axis ? P->tile_bf_axis : widest;
-  const int p = P->tile_bf_pencil ? P->tile_bf_pencil : widest;
-  if (!a || !p || nb % a || nb % p) return false;
-  if (wa) *wa = a;
-  if (wp) *wp = p;
-  return true;
+  const bool split = split_rows_ok(P) && nb % P->tile_axis == 0 && nb % P->tile_pencil == 0;
+  if (wa) *wa = split ? P->tile_axis : 0;
+  if (wp) *wp = split ? P->tile_pencil : 0;
+  return split ? 2 : 0;
+}
+
+// The layout of the TN inner loop's Hessian products: 1 (bf16 split rows,
+// selected on an FP32 plan where split_rows_ok and n2/2 is a multiple of
+// both bf16 tile widths; automatic: the widest of 32, 16, 8 that divides
+// it), else that of pass_layout.
+inline int cg_layout(const ofdft_plan_s* P, int* wa = nullptr, int* wp = nullptr) {
+  const int nb = P->n[2] / 2;
+  if (P->cg_spectra == OFDFT_CG_SPECTRA_BF16 && P->prec == OFDFT_F32 && split_rows_ok(P)) {
+    int widest = 0;
+    for (int w = 32; w >= 8 && !widest; w /= 2)
+      if (nb % w == 0) widest = w;
+    const int a = P->tile_bf_axis ? P->tile_bf_axis : widest;
+    const int p = P->tile_bf_pencil ? P->tile_bf_pencil : widest;
+    if (a && p && nb % a == 0 && nb % p == 0) {
+      if (wa) *wa = a;
+      if (wp) *wp = p;
+      return 1;
+    }
+  }
+  return pass_layout(P, wa, wp);
 }
 
 constexpr int ceil_div(int a, int b) { return (a + b - 1) / b; }
@@ -2565,16 +2609,17 @@ using RowsFFT = FFTFixed<N, kRowPB<T> * 3, rows_th<N>()>;
 template <int N>
 using ColsFFT = FFTFixed<N, kColTileC, kColsTH>;
 
-// Spectrum layout of a row pass: SL = 0 the plan-precision spectrum (field
-// stride Nhp, row stride nhp); SL = 1 bf16 (field stride n0*n1*n2/2 words,
-// row stride n2/2, the Nyquist column apart).
+// Spectrum layout of a pass: SL = 0 padded rows in the plan precision
+// (field stride Nhp, row stride nhp); SL = 1 (bf16) and SL = 2 (plan
+// precision) split rows (field stride n0*n1*n2/2 elements, row stride n2/2,
+// the Nyquist column apart).
 template <int SL>
 long long spec_field_stride(const ofdft_plan_s* P) {
-  return SL == 1 ? (long long)P->n[0] * P->n[1] * (P->n[2] / 2) : P->Nhp;
+  return SL >= 1 ? (long long)P->n[0] * P->n[1] * (P->n[2] / 2) : P->Nhp;
 }
 template <int SL>
 int spec_row_stride(const ofdft_plan_s* P) {
-  return SL == 1 ? P->n[2] / 2 : P->nhp;
+  return SL >= 1 ? P->n[2] / 2 : P->nhp;
 }
 // bytes of F spectra in a layout (read or written once)
 template <int SL>
@@ -2595,7 +2640,7 @@ void go_rows_fwd(ofdft_plan_s* P, const Pre& pre, const FFT& fft, int F, dim3 gr
                spec_row_stride<SL>(P), F, reinterpret_cast<const cplx<T>*>(P->tw[2]), P->skip);
 }
 
-// SL = 1 (bf16 spectra) runs on compile-time lengths only (cg_bf16_tiles)
+// SL = 1, 2 (split rows) run on compile-time lengths only (cg_layout)
 template <typename T, class Pre, int SL = 0>
 int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
   const long long nrows = (long long)P->n[0] * P->n[1];
@@ -2612,7 +2657,7 @@ int launch_rows_fwd(ofdft_plan_s* P, const Pre& pre, int F) {
     OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
     default: {
-      if (SL == 1) return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time row length");
+      if (SL >= 1) return fail(OFDFT_EINVAL, "split-row spectra need a compile-time row length");
       FFTLaunch L = fft_launch(P->pl[2], PB * F);
       // radix 11 / 13 schedules: one instantiation, up to four slots per thread
       if (has_large(P->pl[2])) go_rows_fwd<T>(P, pre, FFTRuntime<4, true>{P->pl[2]}, F, grid, L.threads, lds);
@@ -2660,7 +2705,7 @@ int launch_rows_inv(ofdft_plan_s* P, const Post& post, int F, const cplx<T>* src
       OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
       default: {
-        if (SL == 1) return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time row length");
+        if (SL >= 1) return fail(OFDFT_EINVAL, "split-row spectra need a compile-time row length");
         FFTLaunch L = fft_launch(P->pl[2], PB * F, 256);
         // radix 11 / 13 schedules: one instantiation, up to four slots per thread
         if (has_large(P->pl[2])) go_rows_inv<T>(P, post, FFTRuntime<4, true>{P->pl[2]}, F, src, grid, L.threads, lds, part);
@@ -2820,80 +2865,73 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
   return 0;
 }
 
-// The axis-1 pass over bf16 spectra (SL = 1 of axis_reg_block): nb/W column
-// tiles plus one Nyquist tile slot per plane.
-template <typename T, int N1, int N2, int W>
-void axis_reg_bf(ofdft_plan_s* P, int F, bool inv, const PapArgs* pa) {
+// The axis-1 pass over split-row spectra (SL = 1 bf16, SL = 2 plan
+// precision; axis_split_block): nb/W column tiles plus one Nyquist tile slot
+// per plane, W columns per workgroup.
+template <typename T, int N1, int N2, int W, int SL>
+void axis_reg_split(ofdft_plan_s* P, int F, bool inv, const PapArgs* pa) {
   const int nb = P->n[2] / 2;
-  go_axis_reg<T, N1, N2, W, 1>(P, dim3((unsigned)(nb / W + 1), (unsigned)P->n[0], (unsigned)F),
-                               reinterpret_cast<cplx<T>*>(P->spec), spec_field_stride<1>(P), nb,
-                               (long long)P->n[1] * nb, nb, reinterpret_cast<const cplx<T>*>(P->tw[1]), inv, pa);
+  go_axis_reg<T, N1, N2, W, SL>(P, dim3((unsigned)(nb / W + 1), (unsigned)P->n[0], (unsigned)F),
+                                reinterpret_cast<cplx<T>*>(P->spec), spec_field_stride<SL>(P), nb,
+                                (long long)P->n[1] * nb, nb, reinterpret_cast<const cplx<T>*>(P->tw[1]), inv, pa);
 }
 
-template <typename T>
-int launch_axis_bf(ofdft_plan_s* P, int F, bool inv, const PapArgs* pa = nullptr) {
-  if constexpr (sizeof(T) != 4) {
-    return fail(OFDFT_EINVAL, "bf16 spectra are for FP32 plans");
-  } else {
-    int W = 0;
-    if (!cg_bf16_tiles(P, &W, nullptr)) return fail(OFDFT_EINVAL, "bf16 spectra do not apply to this plan");
-    ProfScope ps(P, PC_AXIS, 2 * spec_bytes<1>(P, F));
-    switch (P->n[1]) {
-#define OFDFT_CASE(NN)                                                                                     \
-      case NN:                                                                                             \
-        if (W == 8) axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, inv, pa);                       \
-        else if (W == 16) axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 16>(P, F, inv, pa);                \
-        else axis_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 32>(P, F, inv, pa);                             \
-        break;
-      OFDFT_FIXED_SIZES(OFDFT_CASE)
+template <typename T, int SL>
+int launch_axis_split(ofdft_plan_s* P, int F, bool inv, int W, const PapArgs* pa = nullptr) {
+  static_assert(SL == 2 || (SL == 1 && sizeof(T) == 4), "bf16 spectra are for FP32 plans");
+  ProfScope ps(P, PC_AXIS, 2 * spec_bytes<SL>(P, F));
+  switch (P->n[1]) {
+#define OFDFT_CASE(NN)                                                                                       \
+    case NN:                                                                                                 \
+      if (W == 8) axis_reg_split<T, Split<NN>::N1, Split<NN>::N2, 8, SL>(P, F, inv, pa);                    \
+      else if (sizeof(T) == 4 && W == 32)                                                                    \
+        axis_reg_split<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16), SL>(P, F, inv, pa);      \
+      else axis_reg_split<T, Split<NN>::N1, Split<NN>::N2, 16, SL>(P, F, inv, pa);                          \
+      break;
+    OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
-      default:
-        return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time axis-1 length");
-    }
-    LAUNCHCHK();
-    return 0;
+    default:
+      return fail(OFDFT_EINVAL, "split-row spectra need a compile-time axis-1 length");
   }
+  LAUNCHCHK();
+  return 0;
 }
 
-// The axis-0 pencils over bf16 spectra (SL = 1 of pencil_reg_block); cgp
+// The axis-0 pencils over split-row spectra (pencil_split_block); cgp
 // receives one partial sum per workgroup, (nb/W + 1) * n1 * F of them.
-template <typename T, int N1, int N2, int W>
-void pencil_reg_bf(ofdft_plan_s* P, int F, const GridGeom& g, const ConvKinds& ck, double* cgp) {
-  go_pencil_reg<T, N1, N2, W, 1>(P, pencil_grid(P, W, F, true), g, ck, cgp, spec_field_stride<1>(P));
+template <typename T, int N1, int N2, int W, int SL>
+void pencil_reg_split(ofdft_plan_s* P, int F, const GridGeom& g, const ConvKinds& ck, double* cgp) {
+  go_pencil_reg<T, N1, N2, W, SL>(P, pencil_grid(P, W, F, true), g, ck, cgp, spec_field_stride<SL>(P));
 }
 
-template <typename T>
-int launch_pencil_bf(ofdft_plan_s* P, int F, const int* kinds, double* cgp, const double* coef) {
-  if constexpr (sizeof(T) != 4) {
-    return fail(OFDFT_EINVAL, "bf16 spectra are for FP32 plans");
-  } else {
-    int W = 0;
-    if (!cg_bf16_tiles(P, nullptr, &W)) return fail(OFDFT_EINVAL, "bf16 spectra do not apply to this plan");
-    ConvKinds ck;
-    for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
-    ck.scale = 1.0 / (double)P->N;
-    for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
-    for (int f = 0; f < F; ++f)
-      if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
-    GridGeom g = geom(P);
-    long long nwt = 0;
-    for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
-    ProfScope ps(P, PC_PENCIL, 2 * spec_bytes<1>(P, F) + (long long)P->rs * nwt * P->Nh);
-    switch (P->n[0]) {
-#define OFDFT_CASE(NN)                                                                                 \
-      case NN:                                                                                         \
-        if (W == 8) pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 8>(P, F, g, ck, cgp);              \
-        else if (W == 16) pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 16>(P, F, g, ck, cgp);       \
-        else pencil_reg_bf<T, Split<NN>::N1, Split<NN>::N2, 32>(P, F, g, ck, cgp);                    \
-        break;
-      OFDFT_FIXED_SIZES(OFDFT_CASE)
+template <typename T, int SL>
+int launch_pencil_split(ofdft_plan_s* P, int F, const int* kinds, double* cgp, const double* coef, int W) {
+  static_assert(SL == 2 || (SL == 1 && sizeof(T) == 4), "bf16 spectra are for FP32 plans");
+  ConvKinds ck;
+  for (int f = 0; f < 3; ++f) ck.kind[f] = f < F ? kinds[f] : 0;
+  ck.scale = 1.0 / (double)P->N;
+  for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
+  for (int f = 0; f < F; ++f)
+    if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
+  GridGeom g = geom(P);
+  long long nwt = 0;
+  for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
+  ProfScope ps(P, PC_PENCIL, 2 * spec_bytes<SL>(P, F) + (long long)P->rs * nwt * P->Nh);
+  switch (P->n[0]) {
+#define OFDFT_CASE(NN)                                                                                   \
+    case NN:                                                                                             \
+      if (W == 8) pencil_reg_split<T, Split<NN>::N1, Split<NN>::N2, 8, SL>(P, F, g, ck, cgp);           \
+      else if (sizeof(T) == 4 && W == 32)                                                                \
+        pencil_reg_split<T, Split<NN>::N1, Split<NN>::N2, (sizeof(T) == 4 ? 32 : 16), SL>(P, F, g, ck, cgp); \
+      else pencil_reg_split<T, Split<NN>::N1, Split<NN>::N2, 16, SL>(P, F, g, ck, cgp);                 \
+      break;
+    OFDFT_FIXED_SIZES(OFDFT_CASE)
 #undef OFDFT_CASE
-      default:
-        return fail(OFDFT_EINVAL, "bf16 spectra need a compile-time pencil length");
-    }
-    LAUNCHCHK();
-    return 0;
+    default:
+      return fail(OFDFT_EINVAL, "split-row spectra need a compile-time pencil length");
   }
+  LAUNCHCHK();
+  return 0;
 }
 
 // the final reduction kernel also wrote its sums into the mapped host buffer
@@ -2918,6 +2956,40 @@ void finalize_energies(const ofdft_plan_s* P, double* s) {
   s[OFDFT_S_DTHETA] *= 2.0 * dV;
 }
 
+// The five passes of an evaluation or a Hessian product: forward rows with
+// `pre` (F fields), axis-1, the pencils with the multipliers `kinds`,
+// inverse axis-1, inverse rows with `post` (reduce: its sums to P->red and,
+// with to_host, the mapped host buffer).  The spectra take the split rows of
+// pass_layout where the plan allows, else the padded rows.
+template <typename T, class Pre, class Post>
+int conv_passes(ofdft_plan_s* P, const Pre& pre, const Post& post, int F, const int* kinds, bool reduce,
+                bool to_host = true) {
+  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
+  int rc, wa = 0, wp = 0;
+  if (F > 0 && pass_layout(P, &wa, &wp) == 2) {
+    if ((rc = launch_rows_fwd<T, Pre, 2>(P, pre, F))) return rc;
+    if ((rc = launch_axis_split<T, 2>(P, F, false, wa))) return rc;
+    if ((rc = launch_pencil_split<T, 2>(P, F, kinds, nullptr, nullptr, wp))) return rc;
+    if ((rc = launch_axis_split<T, 2>(P, F, true, wa))) return rc;
+    return launch_rows_inv<T, Post, 2>(P, post, F, spec, reduce, to_host);
+  }
+  if (F > 0) {
+    if ((rc = launch_rows_fwd<T>(P, pre, F))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+    if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
+    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
+  }
+  return launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, reduce, to_host);
+}
+
+// multipliers of the convolution fields of a functional
+inline void conv_kinds(const FnParams& fp, int* kinds) {
+  kinds[0] = kinds[1] = kinds[2] = 0;
+  if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
+  if (fp.fV >= 0) kinds[fp.fV] = OFDFT_CONV_GG;
+  if (fp.fW >= 0) kinds[fp.fW] = OFDFT_CONV_WT;
+}
+
 template <typename T>
 int eval_impl(ofdft_plan_s* P, const T* rho, const T* phi, const T* p, double c, double s, T* V, T* aux,
               double* sums) {
@@ -2925,21 +2997,10 @@ int eval_impl(ofdft_plan_s* P, const T* rho, const T* phi, const T* p, double c,
   if (P->fn.pseudo && !P->have_vloc) return fail(OFDFT_ESTATE, "V_loc not set");
   FnParams fp = make_fn(P->fn);
   PreEval<T> pre{rho, phi, p, T(c), T(s), fp};
-  const int F = fp.F;
-  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
-  int rc;
-  if (F > 0) {
-    if ((rc = launch_rows_fwd<T>(P, pre, F))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
-    int kinds[3];
-    if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
-    if (fp.fV >= 0) kinds[fp.fV] = OFDFT_CONV_GG;
-    if (fp.fW >= 0) kinds[fp.fW] = OFDFT_CONV_WT;
-    if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
-  }
+  int rc, kinds[3];
+  conv_kinds(fp, kinds);
   PostEval<T> post{pre, reinterpret_cast<const T*>(P->vloc), V, aux, P->N};
-  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true))) return rc;
+  if ((rc = conv_passes<T>(P, pre, post, fp.F, kinds, true))) return rc;
   if (sums) {
     double tmp[OFDFT_NSUM];
     if ((rc = fetch_sums(P, OFDFT_NSUM, tmp))) return rc;
@@ -2954,22 +3015,11 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
              int raw = 0) {
   if (!P->have_fn) return fail(OFDFT_ESTATE, "functional not set");
   FnParams fp = make_fn(P->fn);
-  const int F = fp.F;
-  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
-  int rc;
-  if (F > 0) {
-    PreHvp<T> pre{phi, p, fp};
-    if ((rc = launch_rows_fwd<T>(P, pre, F))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
-    int kinds[3];
-    if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
-    if (fp.fV >= 0) kinds[fp.fV] = OFDFT_CONV_GG;
-    if (fp.fW >= 0) kinds[fp.fW] = OFDFT_CONV_WT;
-    if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
-  }
+  int rc, kinds[3];
+  conv_kinds(fp, kinds);
+  PreHvp<T> pre{phi, p, fp};
   PostHvp<T> post{phi, p, V, aux, Ap, T(mu), P->N, fp, raw};
-  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true))) return rc;
+  if ((rc = conv_passes<T>(P, pre, post, fp.F, kinds, true))) return rc;
   if (pAp) {
     double t;
     if ((rc = fetch_sums(P, 1, &t))) return rc;
@@ -2984,27 +3034,17 @@ int hvp_impl(ofdft_plan_s* P, const T* phi, const T* p, const T* V, const T* aux
 template <typename T, int LAY>
 int hvp_fast_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, const T* p, const T* W0, T* Ap, double* pAp,
                  const T* cg_r, const TnState* cg_st) {
-  const int F = fp.F;
-  auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
-  int rc;
-  if (F > 0) {
-    PreHvp<T, LAY> pre{phi, p, fp};
-    if (cg_st) {
-      PreHvpCG<T, LAY> pcg{pre, phi, const_cast<T*>(p), cg_r, cg_st};
-      if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
-    } else if ((rc = launch_rows_fwd<T>(P, pre, F))) {
-      return rc;
-    }
-    if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
-    int kinds[3];
-    if (fp.fH >= 0) kinds[fp.fH] = OFDFT_CONV_HARTREE;
-    if (fp.fV >= 0) kinds[fp.fV] = OFDFT_CONV_GG;
-    if (fp.fW >= 0) kinds[fp.fW] = OFDFT_CONV_WT;
-    if ((rc = launch_pencil<T>(P, F, kinds))) return rc;
-    if ((rc = launch_axis<T>(P, spec, 1, F, true))) return rc;
-  }
+  int rc, kinds[3];
+  conv_kinds(fp, kinds);
+  PreHvp<T, LAY> pre{phi, p, fp};
   PostHvpFast<T, LAY> post{phi, p, W0, Ap, fp};
-  if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, pAp != nullptr))) return rc;
+  if (cg_st) {
+    PreHvpCG<T, LAY> pcg{pre, phi, const_cast<T*>(p), cg_r, cg_st};
+    rc = conv_passes<T>(P, pcg, post, fp.F, kinds, true, pAp != nullptr);
+  } else {
+    rc = conv_passes<T>(P, pre, post, fp.F, kinds, true, pAp != nullptr);
+  }
+  if (rc) return rc;
   if (pAp) {
     double t;
     if ((rc = fetch_sums(P, 1, &t))) return rc;
@@ -3033,50 +3073,47 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
   auto* spec = reinterpret_cast<cplx<T>*>(P->spec);
   int rc;
   long long nblk_pen = 0;
-  // bf16 spectra between the passes (FP32 plans that select them): the
-  // same transforms and functors, half the spectrum bytes
-  int wbf_axis = 0, wbf = 0;  // wbf: the bf16 pencils' tile width, 0 = plan-precision spectra
-  if (sizeof(T) == 4 && !cg_bf16_tiles(P, &wbf_axis, &wbf)) wbf = 0;
+  // spectrum layout of the passes (cg_layout): split rows (the Nyquist
+  // column apart) in the plan precision or, selected on FP32 plans, bf16
+  int wa = 0, wp = 0;
+  const int sl = F > 0 ? cg_layout(P, &wa, &wp) : 0;
   if (F > 0) {
     PreHvp<T, LAY> pre{phi, p, fp};
     PreHvpCG<T, LAY> pcg{pre, phi, p, r, st};
-    if constexpr (sizeof(T) == 4) {
-      if (wbf) {
-        if ((rc = launch_rows_fwd<T, PreHvpCG<T, LAY>, 1>(P, pcg, F))) return rc;
-        if ((rc = launch_axis_bf<T>(P, F, false))) return rc;
-      }
-    }
-    if (!wbf) {
-      if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
-      if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
-    }
-    int kinds[3] = {0, 0, 0};
-    double coef[3] = {0.0, 0.0, 0.0};
+    int kinds[3];
+    conv_kinds(fp, kinds);
     // p.Ap = sum W0 p^2 + sum_f coef_f g_f.(K_f * g_f) with g_f the field the
     // forward pass transforms: Hartree g = 2 phi p, vW g = sign(phi) p, WT
     // g = 2 beta rho^(beta-1) phi p (the weights of PostHvpFast over these)
-    if (fp.fH >= 0) {
-      kinds[fp.fH] = OFDFT_CONV_HARTREE;
-      coef[fp.fH] = 0.5;
-    }
-    if (fp.fV >= 0) {
-      kinds[fp.fV] = OFDFT_CONV_GG;
-      coef[fp.fV] = 0.5 * fp.y;
-    }
-    if (fp.fW >= 0) {
-      kinds[fp.fW] = OFDFT_CONV_WT;
-      coef[fp.fW] = fp.alpha / fp.beta;
-    }
+    double coef[3] = {0.0, 0.0, 0.0};
+    if (fp.fH >= 0) coef[fp.fH] = 0.5;
+    if (fp.fV >= 0) coef[fp.fV] = 0.5 * fp.y;
+    if (fp.fW >= 0) coef[fp.fW] = fp.alpha / fp.beta;
     {
-      const dim3 pg = pencil_grid(P, wbf ? wbf : P->tile_pencil, F, wbf != 0);
+      const dim3 pg = pencil_grid(P, sl ? wp : P->tile_pencil, F, sl != 0);
       nblk_pen = (long long)pg.x * pg.y * pg.z;
     }
     if (nblk_pen > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
-    if ((rc = wbf ? launch_pencil_bf<T>(P, F, kinds, P->partials, coef) : launch_pencil<T>(P, F, kinds, P->partials, coef)))
-      return rc;
     // p.Ap is completed by the inverse axis pass (k_tn_pap folded in)
     const PapArgs pa{st, (const double*)P->partials, nblk_pen, (double*)P->red};
-    if ((rc = wbf ? launch_axis_bf<T>(P, F, true, &pa) : launch_axis<T>(P, spec, 1, F, true, &pa))) return rc;
+    if (sl == 2) {
+      if ((rc = launch_rows_fwd<T, PreHvpCG<T, LAY>, 2>(P, pcg, F))) return rc;
+      if ((rc = launch_axis_split<T, 2>(P, F, false, wa))) return rc;
+      if ((rc = launch_pencil_split<T, 2>(P, F, kinds, P->partials, coef, wp))) return rc;
+      if ((rc = launch_axis_split<T, 2>(P, F, true, wa, &pa))) return rc;
+    } else if (sl == 1) {
+      if constexpr (sizeof(T) == 4) {
+        if ((rc = launch_rows_fwd<T, PreHvpCG<T, LAY>, 1>(P, pcg, F))) return rc;
+        if ((rc = launch_axis_split<T, 1>(P, F, false, wa))) return rc;
+        if ((rc = launch_pencil_split<T, 1>(P, F, kinds, P->partials, coef, wp))) return rc;
+        if ((rc = launch_axis_split<T, 1>(P, F, true, wa, &pa))) return rc;
+      }
+    } else {
+      if ((rc = launch_rows_fwd<T>(P, pcg, F))) return rc;
+      if ((rc = launch_axis<T>(P, spec, 1, F, false))) return rc;
+      if ((rc = launch_pencil<T>(P, F, kinds, P->partials, coef))) return rc;
+      if ((rc = launch_axis<T>(P, spec, 1, F, true, &pa))) return rc;
+    }
   } else {
     hipLaunchKernelGGL(k_tn_pap, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials, nblk_pen,
                        P->red);
@@ -3084,13 +3121,14 @@ int hvp_cg_lay(ofdft_plan_s* P, const FnParams& fp, const T* phi, T* p, const T*
   }
   PostHvpFast<T, LAY> h{phi, p, W0, nullptr, fp};
   PostHvpCG<T, LAY> post{h, d, r, best, st, (const double*)P->red};
-  if constexpr (sizeof(T) == 4) {
-    if (wbf && F > 0) {
+  if (sl == 2) {
+    if ((rc = launch_rows_inv<T, PostHvpCG<T, LAY>, 2>(P, post, F, spec, true, false, false))) return rc;
+  } else if (sl == 1) {
+    if constexpr (sizeof(T) == 4) {
       if ((rc = launch_rows_inv<T, PostHvpCG<T, LAY>, 1>(P, post, F, spec, true, false, false))) return rc;
     }
-  }
-  if (!(wbf && F > 0)) {
-    if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) return rc;
+  } else if ((rc = launch_rows_inv<T>(P, post, F > 0 ? F : 1, spec, true, false, false))) {
+    return rc;
   }
   const long long npairs = ((long long)P->n[0] * P->n[1] + 1) / 2;
   *nblk_rows = (npairs + kRowPB<T> - 1) / kRowPB<T>;

@@ -28,11 +28,13 @@ Options beyond the reference's:
                  hipGraph per CG iteration instead of launching its kernels
                  (same results; about 3 % faster at 192^3); False: launches
   cg_spectra     'plan' (default): the exact-HVP inner CG keeps its spectra
-                 in the work precision; 'bf16': FP32 inner CG loops store
-                 the spectra between transform passes as bf16 (the
+                 in the work precision, in split rows (the Nyquist column
+                 apart) where the lengths allow; 'bf16': FP32 inner CG loops
+                 store the spectra between transform passes as bf16 (the
                  transforms still compute in FP32), on plans whose lengths
-                 allow it (Plan.cg_spectra_tile); the directions then carry
-                 bf16 rounding, the energies and the line search do not
+                 allow it (Plan.cg_spectra); the directions then carry bf16
+                 rounding, the energies and the line search do not;
+                 'padded': the padded rows of the evaluations
 """
 import math
 import time
@@ -184,10 +186,12 @@ class Optimization:
                     "ls_first_step": "unit"}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
-        if self.optimization_options["cg_spectra"] not in ("plan", "bf16"):
-            # storage of the spectra in the exact-HVP inner CG: the plan
-            # precision, or bf16 on FP32 plans (every transform still in FP32)
-            raise ValueError("cg_spectra must be 'plan' or 'bf16', not %r" % (self.optimization_options["cg_spectra"],))
+        if self.optimization_options["cg_spectra"] not in ("plan", "bf16", "padded"):
+            # layout and storage of the spectra in the exact-HVP inner CG: the
+            # plan precision in split rows (default), bf16 on FP32 plans
+            # (every transform still in FP32), or the padded rows
+            raise ValueError("cg_spectra must be 'plan', 'bf16' or 'padded', not %r"
+                             % (self.optimization_options["cg_spectra"],))
         if EnergyEvaluator is None:
             raise AttributeError("Must provide an energy evaluator")
         if not callable(EnergyEvaluator):

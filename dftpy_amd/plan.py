@@ -332,12 +332,14 @@ class Plan:
         L.check(self.lib.ofdft_plan_set_tiles(self._h, int(tile_axis), int(tile_pencil)))
         self.tiles = (int(tile_axis), int(tile_pencil))
 
-    CG_SPECTRA = {"plan": 0, "bf16": 1}
+    CG_SPECTRA = {"plan": 0, "bf16": 1, "padded": 2}
 
     def set_cg_spectra(self, mode="plan", tiles=None):
-        """Storage of the spectra inside the device TN loop's Hessian
-        products: "plan" (the plan precision) or "bf16" (FP32 plans of
-        compile-time lengths; every transform still computes in FP32).
+        """Layout and storage of the spectra inside the device TN loop's
+        Hessian products: "plan" (the plan precision in split rows, the
+        Nyquist column apart, where the lengths allow; the default), "bf16"
+        (split rows of bf16 pairs on FP32 plans; every transform still
+        computes in FP32) or "padded" (the padded rows of the other passes).
         tiles: (axis, pencil) columns per workgroup of the bf16 passes; None
         takes the autotuned widths for these lengths, 0 the widest that
         divides n2/2 (ofdft_plan_set_cg_spectra)."""
@@ -348,15 +350,22 @@ class Plan:
             tiles = tuning.bf16_tiles_for(self.nr) if mode == "bf16" else (0, 0)
         L.check(self.lib.ofdft_plan_set_cg_spectra(self._h, self.CG_SPECTRA[mode], int(tiles[0]), int(tiles[1])))
 
-    @property
-    def cg_spectra_tiles(self):
-        """(axis, pencil) tile widths of the bf16 spectra the next device TN
-        loop uses, (0, 0) when it keeps plan-precision spectra
+    def cg_spectra(self):
+        """(layout, (axis, pencil) tile widths) the next device TN loop uses,
+        after the fall-backs: layout "plan", "bf16" or "padded"
         (ofdft_plan_cg_spectra)."""
         a, p = ctypes.c_int(), ctypes.c_int()
         mode = self.lib.ofdft_plan_cg_spectra(self._h, ctypes.byref(a), ctypes.byref(p))
         L.check(min(mode, 0))
-        return (a.value, p.value)
+        name = {v: k for k, v in self.CG_SPECTRA.items()}[mode]
+        return name, (a.value, p.value)
+
+    @property
+    def cg_spectra_tiles(self):
+        """(axis, pencil) tile widths of the bf16 spectra the next device TN
+        loop uses, (0, 0) when it keeps plan-precision spectra."""
+        name, tiles = self.cg_spectra()
+        return tiles if name == "bf16" else (0, 0)
 
     def set_graph(self, on=True):
         """Run the device TN loop's CG iterations as replays of one captured

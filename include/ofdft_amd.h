@@ -107,29 +107,35 @@ int ofdft_plan_set_graph(ofdft_plan plan, int on);
  * applied by dftpy_amd at plan creation.  Results do not depend on them.
  * No counterpart in the reference (numpy.fft has no tiling).              */
 int ofdft_plan_set_tiles(ofdft_plan plan, int tile_axis, int tile_pencil);
-/* Storage of the spectra between the transform passes of the Hessian
- * products in ofdft_tn_direction's inner CG loop (the Hessian product of
- * get_direction_TN, optimization.py:92-148):
- * mode OFDFT_CG_SPECTRA_PLAN (default) keeps them in the plan precision;
- * OFDFT_CG_SPECTRA_BF16 stores them as bf16 pairs (each component rounded
- * to nearest even from FP32; every transform, multiplier and pointwise step
- * still computes in FP32, sums in FP64), halving their HBM traffic; the
- * rows then hold n2/2 columns and the Nyquist column sits in a plane of its
- * own.  bf16 applies to FP32 plans whose three lengths have compile-time
- * kernels and whose n2 is even with n2/2 a multiple of both tile widths;
- * other plans keep plan-precision spectra.  tile_axis, tile_pencil: columns
- * per workgroup of the bf16 axis pass and pencils, 8, 16 or 32, or 0 for
- * the widest dividing n2/2.  The directions differ from the FP32 ones by
- * the rounding (relative 2^-9 per stored value); evaluations, line searches
- * and every other call keep plan-precision spectra.  No counterpart in the
- * reference (its FFTs run in FP64).                                       */
+/* Layout and storage of the spectra between the transform passes of the
+ * Hessian products in ofdft_tn_direction's inner CG loop (the Hessian
+ * product of get_direction_TN, optimization.py:92-148):
+ *  OFDFT_CG_SPECTRA_PLAN (default): plan precision in split rows -- n2/2
+ *    columns per row, the Nyquist column in a plane of its own -- where the
+ *    plan allows them (compile-time kernels on all three lengths, n2 even,
+ *    n2/2 a multiple of the plan's column tile widths), else padded rows;
+ *  OFDFT_CG_SPECTRA_BF16: split rows of bf16 pairs (each component rounded
+ *    to nearest even from FP32; every transform, multiplier and pointwise
+ *    step still computes in FP32, sums in FP64), halving their HBM traffic,
+ *    on FP32 plans where n2/2 is a multiple of both bf16 tile widths; the
+ *    directions then differ from the FP32 ones by the rounding (relative
+ *    2^-9 per stored value); other plans fall back as for _PLAN;
+ *  OFDFT_CG_SPECTRA_PADDED: plan precision in the padded rows of the other
+ *    passes (n2/2+1 columns, row stride a multiple of 16).
+ * tile_axis, tile_pencil: columns per workgroup of the bf16 axis pass and
+ * pencils, 8, 16 or 32, or 0 for the widest dividing n2/2 (the other modes
+ * use the plan's tiles).  Evaluations, line searches and every other call
+ * keep padded plan-precision spectra.  Nothing is allocated: the layouts
+ * live in the plan's spectrum workspace.  No counterpart in the reference
+ * (its FFTs run in FP64 through numpy).                                    */
 #define OFDFT_CG_SPECTRA_PLAN 0
 #define OFDFT_CG_SPECTRA_BF16 1
+#define OFDFT_CG_SPECTRA_PADDED 2
 int ofdft_plan_set_cg_spectra(ofdft_plan plan, int mode, int tile_axis, int tile_pencil);
-/* The spectra the next inner CG loop of this plan stores:
- * OFDFT_CG_SPECTRA_BF16 (with its tile widths in *tile_axis and
- * *tile_pencil when those are not NULL) or OFDFT_CG_SPECTRA_PLAN (tile
- * widths 0); OFDFT_EINVAL for a null plan.                                */
+/* The layout the next inner CG loop of this plan uses (one of the
+ * OFDFT_CG_SPECTRA_* above, after the fall-backs), with its column tile
+ * widths in *tile_axis and *tile_pencil when those are not NULL (0 for
+ * padded rows); OFDFT_EINVAL for a null plan.                              */
 int ofdft_plan_cg_spectra(ofdft_plan plan, int* tile_axis, int* tile_pencil);
 
 /* Batched TN inner loop over many independent cells (the C5 batching form;

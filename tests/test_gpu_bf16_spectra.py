@@ -1,5 +1,7 @@
-"""bf16 spectra in the device TN loop's Hessian products
-(ofdft_plan_set_cg_spectra, Optimization option cg_spectra='bf16').
+"""Spectrum layouts of the device TN loop's Hessian products
+(ofdft_plan_set_cg_spectra, Optimization option cg_spectra): split rows in
+the plan precision (the default) against the padded rows, and bf16 split
+rows (cg_spectra='bf16').
 
 The spectra between the transform passes of the exact Hessian product are
 stored as bf16 pairs (n2/2 columns per row, the Nyquist column in a plane of
@@ -42,10 +44,10 @@ def _golden(name):
         return json.load(fh)[name]
 
 
-def _inputs(nr, cell, seed=0):
-    """An FP32 plan with TF + vW + WT + LDA + Hartree (no PP), a smooth
-    positive phi, and its residual and Hessian diagonal at mu = 0.3."""
-    P = Plan(nr, cell, precision="fp32")
+def _inputs(nr, cell, seed=0, prec="fp32"):
+    """A plan with TF + vW + WT + LDA + Hartree (no PP), a smooth positive
+    phi, and its residual and Hessian diagonal at mu = 0.3."""
+    P = Plan(nr, cell, precision=prec)
     rng = np.random.default_rng(seed)
     x = [np.arange(n) / n for n in nr]
     X = np.meshgrid(*x, indexing="ij")
@@ -111,22 +113,67 @@ def test_bf16_hessian_product_within_bf16_rounding(nr, tiles):
     assert torch.equal(d32b, d32)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("nr", [(32, 32, 32), (48, 64, 96), (96, 48, 64), (192, 96, 128)])
+def test_split_rows_give_the_padded_rows_result(prec, nr):
+    """The default layout of the loop's spectra (split rows: n2/2 columns,
+    the Nyquist column in a plane of its own) against the padded rows of the
+    other passes: the same transforms of the same values, so the same
+    product; only the grouping of p.Ap's FP64 partial sums differs."""
+    print("case split vs padded rows", prec, nr, flush=True)
+    cell = np.diag([0.35 * n for n in nr])
+    P, phi, res, W0 = _inputs(nr, cell, prec=prec)
+    assert P.cg_spectra()[0] == "plan"
+    ap1, pap1, d1 = _one_step(P, phi, res, W0)
+    P.set_cg_spectra("padded")
+    assert P.cg_spectra() == ("padded", (0, 0))
+    ap0, pap0, d0 = _one_step(P, phi, res, W0)
+    epap = abs(pap1 - pap0) / abs(pap0)
+    ed = float((d1.double() - d0.double()).norm() / d0.double().norm())
+    print("p.Ap rel. difference %.2e, direction rel. difference %.2e" % (epap, ed))
+    assert epap < 1e-12
+    assert ed < (1e-6 if prec == "fp32" else 1e-14)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("nr", [(32, 32, 32), (48, 64, 96), (192, 96, 128)])
+def test_split_rows_evaluation_is_bitwise_the_padded_one(prec, nr):
+    """Evaluations (energies, potential, the line-search derivative) take
+    the split rows too: the same arithmetic at other addresses, so the same
+    bits as the padded rows."""
+    print("case split vs padded rows, evaluation", prec, nr, flush=True)
+    cell = np.diag([0.35 * n for n in nr])
+    P, phi, res, W0 = _inputs(nr, cell, prec=prec)
+    p = torch.sin(phi * 7.0)
+    out = {}
+    for mode in ("plan", "padded"):
+        P.set_cg_spectra(mode)
+        V = P.empty()
+        aux = torch.empty(2 * P.N, dtype=P.dtype, device=P.device)
+        sums = P.eval_phi(phi, p, 0.8, 0.6, V, aux)
+        out[mode] = (np.array(sums), V.clone(), aux.clone())
+    assert P.cg_spectra()[0] == "padded"
+    assert np.array_equal(out["plan"][0], out["padded"][0])
+    assert torch.equal(out["plan"][1], out["padded"][1])
+    assert torch.equal(out["plan"][2], out["padded"][2])
+
+
 def test_bf16_tile_selection_and_fallback():
     # n2 = 90: n2/2 = 45 has no tile width; FP64 plans keep their spectra
     cell = np.eye(3) * 20.0
     P = Plan((32, 32, 90), cell, precision="fp32")
     P.set_cg_spectra("bf16")
-    assert P.cg_spectra_tiles == (0, 0)
+    assert P.cg_spectra() == ("padded", (0, 0))  # no split rows for an odd n2/2
     P = Plan((32, 32, 48), cell, precision="fp32")
     P.set_cg_spectra("bf16", (0, 0))
     assert P.cg_spectra_tiles == (8, 8)
     P.set_cg_spectra("bf16", (8, 16))  # 24 columns are no whole number of 16-wide tiles
-    assert P.cg_spectra_tiles == (0, 0)
+    assert P.cg_spectra_tiles == (0, 0) and P.cg_spectra()[0] != "bf16"
     with pytest.raises(Exception):
         P.set_cg_spectra("bf16", (12, 8))
     P = Plan((32, 32, 32), cell, precision="fp64")
     P.set_cg_spectra("bf16")
-    assert P.cg_spectra_tiles == (0, 0)
+    assert P.cg_spectra()[0] == "plan"  # FP64 plans keep their precision, in split rows
     with pytest.raises(ValueError):
         P.set_cg_spectra("fp16")
 
