@@ -183,6 +183,7 @@ int ofdft_plan_destroy(ofdft_plan P) {
     if (P->tw[i]) (void)hipFree(P->tw[i]);
   if (P->spec) (void)hipFree(P->spec);
   if (P->wtk) (void)hipFree(P->wtk);
+  if (P->wtks) (void)hipFree(P->wtks);
   if (P->vloc) (void)hipFree(P->vloc);
   if (P->partials) (void)hipFree(P->partials);
   if (P->d_knots) (void)hipFree(P->d_knots);
@@ -322,6 +323,7 @@ long long ofdft_plan_workspace_bytes(ofdft_plan P) {
   if (!valid(P)) return -1;
   long long b = 2 * (long long)P->rs * (3 * P->Nhp + P->n[0] + P->n[1] + P->n[2]);
   if (P->wtk) b += (long long)P->rs * P->Nh;
+  if (P->wtks) b += (long long)P->rs * P->Nh;
   if (P->vloc) b += (long long)P->rs * P->N;
   b += sizeof(double) * (P->npartials + kMaxSum);
   return b;
@@ -381,12 +383,23 @@ int ofdft_set_functional(ofdft_plan P, const ofdft_functional* f) {
     double factor = 5.0 / (9.0 * f->alpha * f->beta * std::pow(rho0, f->alpha + f->beta - 5.0 / 3.0));
     factor *= ctf;
     GridGeom g = geom(P);
-    if (P->prec == OFDFT_F64)
+    // the split-row copy for the pencils of split-row spectra (n2 even)
+    const bool split = P->n[2] % 2 == 0;
+    if (split && !P->wtks && hipMalloc(&P->wtks, P->rs * P->Nh) != hipSuccess)
+      return fail(OFDFT_ENOMEM, "WT table (split rows)");
+    if (P->prec == OFDFT_F64) {
       hipLaunchKernelGGL((k_wt_table<double>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream,
                          reinterpret_cast<double*>(P->wtk), g, tkf, factor, f->x);
-    else
+      if (split)
+        hipLaunchKernelGGL((k_wt_split<double>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream,
+                           reinterpret_cast<const double*>(P->wtk), reinterpret_cast<double*>(P->wtks), g);
+    } else {
       hipLaunchKernelGGL((k_wt_table<float>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream,
                          reinterpret_cast<float*>(P->wtk), g, tkf, factor, f->x);
+      if (split)
+        hipLaunchKernelGGL((k_wt_split<float>), dim3(vec_blocks(P->Nh)), dim3(256), 0, P->stream,
+                           reinterpret_cast<const float*>(P->wtk), reinterpret_cast<float*>(P->wtks), g);
+    }
     LAUNCHCHK();
     P->have_wtk = 1;
   }

@@ -146,6 +146,7 @@ struct ofdft_plan_s {
   void* tw[3];      // twiddle tables, plan precision complex
   void* spec;       // 3*Nh complex workspace
   void* wtk;        // WT kernel table, Nh reals (plan precision)
+  void* wtks;       // the same table in split rows (n2 even): n0*n1 rows of n2/2, then the Nyquist plane [k0][i1]
   void* vloc;       // N reals
   double* partials; // per-block partial sums
   long long npartials;
@@ -1935,11 +1936,14 @@ __device__ __forceinline__ void pencil_split_block(cplx<T>* __restrict__ spec, c
   E raw[N1];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1) raw[n1] = ok ? at(sp, off + (unsigned)(N2 * n1 + q) * js) : E{};
+  // WT multipliers from the table in the same split rows (wtk: the plan's
+  // wtks): W contiguous entries of a row, or of the Nyquist plane, per k0
   T wm[N2];
   const bool wt = kind == OFDFT_CONV_WT && q < N1 && ok;
+  const unsigned woff = nyq ? NB + (unsigned)i1 : (unsigned)i1 * (unsigned)nb + (unsigned)c;
+  const unsigned wjs = nyq ? (unsigned)geo.n1 : (unsigned)geo.n1 * (unsigned)nb;
 #pragma unroll
-  for (int k2 = 0; k2 < N2; ++k2)
-    wm[k2] = wt ? at(wtk, ((unsigned)(q + N1 * k2) * (unsigned)geo.n1 + i1) * (unsigned)geo.nh + c) : T(0);
+  for (int k2 = 0; k2 < N2; ++k2) wm[k2] = wt ? at(wtk, woff + (unsigned)(q + N1 * k2) * wjs) : T(0);
   cplx<T> v[N2];
 #pragma unroll
   for (int n1 = 0; n1 < N1; ++n1) v[n1] = SE::get(raw[n1]);
@@ -2019,6 +2023,21 @@ __global__ void k_wt_table(T* __restrict__ out, GridGeom geo, double tkf, double
     double k = lindhard_dev(sqrt(gsq(geo, i0, i1, c)) / tkf, lbda, 1.0);
     if (geo.full) k = 0.5 * (k + lindhard_dev(sqrt(gsq(geo, i0, i1, c, 1)) / tkf, lbda, 1.0));
     out[idx] = T(k * factor);
+  }
+}
+
+// The WT table in split rows (n2 even, nb = n2/2): entry (k0, k1, c < nb)
+// at (k0*n1 + k1)*nb + c, the Nyquist column c = nb at n0*n1*nb + k0*n1 + k1;
+// the layout of the split-row spectra, so that a pencil tile's multipliers
+// are whole, aligned segments of a row or of the Nyquist plane.
+template <typename T>
+__global__ void k_wt_split(const T* __restrict__ wtk, T* __restrict__ out, GridGeom geo) {
+  const long long n01 = (long long)geo.n0 * geo.n1, nb = geo.nh - 1;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n01 * geo.nh;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const long long r = idx / geo.nh;
+    const long long c = idx - r * geo.nh;
+    out[c < nb ? r * nb + c : n01 * nb + r] = wtk[idx];
   }
 }
 
@@ -2811,7 +2830,8 @@ void go_pencil(ofdft_plan_s* P, const FFT& fft, dim3 grid, int threads, size_t l
 template <typename T, int N1, int N2, int W, int SL>
 void go_pencil_reg(ofdft_plan_s* P, dim3 grid, const GridGeom& g, const ConvKinds& ck, double* cgp, long long Nh) {
   OFDFT_LAUNCH((k_pencil_reg<T, N1, N2, W, SL>), grid, dim3(W * N2), 0, P, reinterpret_cast<cplx<T>*>(P->spec), Nh,
-               g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck, reinterpret_cast<const T*>(P->wtk), P->skip, cgp);
+               g, reinterpret_cast<const cplx<T>*>(P->tw[0]), ck,
+               reinterpret_cast<const T*>(SL >= 1 ? P->wtks : P->wtk), P->skip, cgp);
 }
 
 // workgroups of the register pencils of tile width W (bf: the bf16 layout's
@@ -2837,7 +2857,7 @@ int launch_pencil(ofdft_plan_s* P, int F, const int* kinds, double* cgp = nullpt
   for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
   if (cgp && !pencil_reg_ok(P)) return fail(OFDFT_EINVAL, "p.Ap sums need a compile-time pencil length");
   for (int f = 0; f < F; ++f)
-    if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
+    if (kinds[f] == OFDFT_CONV_WT && !(P->have_wtk && P->wtks)) return fail(OFDFT_ESTATE, "WT kernel table not built");
   GridGeom g = geom(P);
   long long nwt = 0;
   for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;
@@ -2912,7 +2932,7 @@ int launch_pencil_split(ofdft_plan_s* P, int F, const int* kinds, double* cgp, c
   ck.scale = 1.0 / (double)P->N;
   for (int f = 0; f < 3; ++f) ck.coef[f] = coef ? coef[f] : 0.0;
   for (int f = 0; f < F; ++f)
-    if (kinds[f] == OFDFT_CONV_WT && !P->have_wtk) return fail(OFDFT_ESTATE, "WT kernel table not built");
+    if (kinds[f] == OFDFT_CONV_WT && !(P->have_wtk && P->wtks)) return fail(OFDFT_ESTATE, "WT kernel table not built");
   GridGeom g = geom(P);
   long long nwt = 0;
   for (int f = 0; f < F; ++f) nwt += kinds[f] == OFDFT_CONV_WT ? 1 : 0;

@@ -124,8 +124,11 @@ int ofdft_plan_set_tiles(ofdft_plan plan, int tile_axis, int tile_pencil);
  *    passes (n2/2+1 columns, row stride a multiple of 16).
  * tile_axis, tile_pencil: columns per workgroup of the bf16 axis pass and
  * pencils, 8, 16 or 32, or 0 for the widest dividing n2/2 (the other modes
- * use the plan's tiles).  Evaluations, line searches and every other call
- * keep padded plan-precision spectra.  Nothing is allocated: the layouts
+ * use the plan's tiles).  The mode also sets the layout of the five passes
+ * of every evaluation of this plan (ofdft_eval, ofdft_eval_phi, the
+ * line-search trials): split rows in the plan precision unless the mode is
+ * OFDFT_CG_SPECTRA_PADDED or the plan does not allow them; bf16 storage is
+ * used by the inner CG loop only.  Nothing is allocated: the layouts
  * live in the plan's spectrum workspace.  No counterpart in the reference
  * (its FFTs run in FP64 through numpy).                                    */
 #define OFDFT_CG_SPECTRA_PLAN 0
