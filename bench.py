@@ -51,9 +51,9 @@ PROFILE_STRIDE = 5
 # same counters read for a plain device copy (FETCH x 2.00, WRITE x 1.00);
 # the summary, the commands and the source revision they ran on are in
 # profiles/axis_c2c_fp32_al864_192.md.
-MEASURED_TRAFFIC = {("al864_192", "fp32", "axis_c2c"): 187.7e6}
+MEASURED_TRAFFIC = {("al864_192", "fp32", "axis_c2c"): 171.75e6}
 TRAFFIC_SOURCE = ("calibrated constant: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, copy-kernel scaled, "
-                  "measured at commit 9df5f32 (profiles/axis_c2c_fp32_al864_192.md)")
+                  "measured at commit 801a045 on the split-row axis pass (profiles/axis_c2c_fp32_al864_192.md)")
 
 
 def parse():
