@@ -171,8 +171,13 @@ int ofdft_plan_create(ofdft_plan* out, int n0, int n1, int n2, const double* lat
     return cleanup(OFDFT_EHIP, "pinned host buffer mapping");
   if (hipMalloc(&P->tnst, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_ENOMEM, "TN state");
   if (hipMemset(P->tnst, 0, sizeof(TnState)) != hipSuccess) return cleanup(OFDFT_EHIP, "TN state clear");
-  if (hipHostMalloc(&P->htnst, sizeof(TnState), hipHostMallocDefault) != hipSuccess)
+  if (hipHostMalloc(&P->htnst, sizeof(TnState), hipHostMallocCoherent) != hipSuccess)
     return cleanup(OFDFT_ENOMEM, "pinned TN state");
+  std::memset(P->htnst, 0, sizeof(TnState));
+  if (hipHostGetDevicePointer(&P->dhtnst, P->htnst, 0) != hipSuccess)
+    return cleanup(OFDFT_EHIP, "pinned TN state mapping");
+  if (hipEventCreateWithFlags(&P->tn_ev, hipEventDisableTiming) != hipSuccess)
+    return cleanup(OFDFT_EHIP, "TN loop event");
   *out = P;
   return 0;
 }
@@ -202,6 +207,7 @@ int ofdft_plan_destroy(ofdft_plan P) {
   if (P->tnst) (void)hipFree(P->tnst);
   if (P->htnst) (void)hipHostFree(P->htnst);
   if (P->tn_exec) (void)hipGraphExecDestroy(P->tn_exec);
+  if (P->tn_ev) (void)hipEventDestroy(P->tn_ev);
   if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;

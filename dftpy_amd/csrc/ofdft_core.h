@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -155,6 +156,7 @@ struct ofdft_plan_s {
   double* dhred;    // device address of hred (reduction kernels write final sums there)
   void* tnst;       // device TnState of ofdft_tn_direction
   void* htnst;      // pinned host mirror of it
+  void* dhtnst;     // device address of htnst: the control kernels publish the state there
   const int* skip;  // while set, FFT and reduction kernels return at once when *skip != 0
   ofdft_functional fn;
   int have_fn, have_vloc, have_wtk;
@@ -203,6 +205,7 @@ struct ofdft_plan_s {
   int tile_bf_axis = 0, tile_bf_pencil = 0;
   int use_graph = 0;
   hipGraphExec_t tn_exec = nullptr;
+  hipEvent_t tn_ev = nullptr;  // the TN loop's check point, one iteration behind the queue
   const void* tn_key[8] = {};
   hipStream_t cap_stream = nullptr;  // capture happens here (the null stream cannot capture)
 };
@@ -2202,9 +2205,43 @@ __device__ __forceinline__ void tn_start_state(TnState* st, const double* __rest
   st->done = maxit > 0 ? 0 : 1;
 }
 
-__global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit) {
+// Copy of the final loop state into mapped host memory (hst, may be null),
+// for a host that checks the loop at an event while later iterations still
+// run: written once, by the control step that ends the loop (k_tn_start
+// only clears `done`): every field but `done` first, then a system-scope
+// fence, then `done`, so that a host that reads done == 1 (and fences)
+// reads the fields of that same write.  Once done is set no later kernel of
+// the loop writes again.
+__device__ __forceinline__ void tn_publish(const TnState* st, TnState* hst) {
+  if (!hst || !st->done) return;
+  volatile TnState* h = hst;
+  h->r0 = st->r0;
+  h->rconv = st->rconv;
+  h->rmin = st->rmin;
+  h->r1 = st->r1;
+  h->pAp = st->pAp;
+  h->beta = st->beta;
+  h->scale = st->scale;
+  h->wpp = st->wpp;
+  h->it = st->it;
+  h->niter = st->niter;
+  h->status = st->status;
+  h->act = st->act;
+  h->maxit = st->maxit;
+  h->alias = st->alias;
+  h->fin = st->fin;
+  __threadfence_system();
+  h->done = st->done;
+  __threadfence_system();
+}
+
+__global__ void k_tn_start(TnState* st, const double* __restrict__ rr, int maxit, TnState* hst) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   tn_start_state(st, rr, maxit);
+  if (hst) {
+    reinterpret_cast<volatile TnState*>(hst)->done = st->done;
+    __threadfence_system();
+  }
 }
 
 // Best = direction when the previous iteration asked for a copy (direction
@@ -2288,11 +2325,14 @@ __device__ void tn_ctrl(TnState* st, double pAp, double r1) {
 
 // r1 = sum of the step kernel's partials, then the tests; pAp = red[0].
 __global__ void k_tn_reduce_ctrl(TnState* st, const double* __restrict__ partials, long long nblk,
-                                 const double* __restrict__ red, double* __restrict__ rr) {
+                                 const double* __restrict__ red, double* __restrict__ rr, TnState* hst) {
   if (st->done) return;
   double tmp[1] = {thread_partials<1>(partials, nblk, 0)};
   block_sums<1>(tmp, rr);
-  if (threadIdx.x == 0) tn_ctrl(st, red[0], rr[0]);
+  if (threadIdx.x == 0) {
+    tn_ctrl(st, red[0], rr[0]);
+    tn_publish(st, hst);
+  }
 }
 
 // pAp = the pencil's spectral sums + sum W0 p^2 (kept in the state) into
@@ -2371,9 +2411,10 @@ __device__ __forceinline__ void tn_ctrl4_block(TnState* st, const double* __rest
 }
 
 __global__ void k_tn_reduce_ctrl4(TnState* st, const double* __restrict__ partials, long long nblk,
-                                  const double* __restrict__ red, double* __restrict__ rr) {
+                                  const double* __restrict__ red, double* __restrict__ rr, TnState* hst) {
   if (st->done) return;
   tn_ctrl4_block(st, partials, nblk, red, rr);
+  if (threadIdx.x == 0) tn_publish(st, hst);
 }
 
 // p = r + beta p when the control asked for it: used only where no forward
@@ -4649,7 +4690,7 @@ int tn_graph_iteration(ofdft_plan_s* P, const T* phi, T* p, const T* W0, T* d, T
     int rc = hvp_cg_impl<T>(P, phi, p, W0, d, r, best, (const TnState*)st, &nbr);
     if (!rc) {
       hipLaunchKernelGGL(k_tn_reduce_ctrl4, dim3(1), dim3(kRedThreads), 0, P->stream, st, (const double*)P->partials,
-                         nbr, (const double*)P->red, rr);
+                         nbr, (const double*)P->red, rr, reinterpret_cast<TnState*>(P->dhtnst));
       if (hipGetLastError() != hipSuccess) rc = fail(OFDFT_EHIP, "control launch during graph capture");
     }
     hipGraph_t g = nullptr;
@@ -4693,17 +4734,27 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
   hipLaunchKernelGGL((k_reduce_partials<2>), dim3(2), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb, rr,
                      (double*)nullptr, (const int*)nullptr);
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_tn_start, dim3(1), dim3(1), 0, P->stream, st, (const double*)rr, maxfun);
+  auto* hst = reinterpret_cast<TnState*>(P->dhtnst);
+  hipLaunchKernelGGL(k_tn_start, dim3(1), dim3(1), 0, P->stream, st, (const double*)rr, maxfun, hst);
   LAUNCHCHK();
-  auto* h = reinterpret_cast<TnState*>(P->htnst);
+  auto* h = reinterpret_cast<volatile TnState*>(P->htnst);
   const size_t prof_mark = P->prof_pending.size();
   P->skip = &st->done;
   const bool fused = make_fn(P->fn).F > 0;  // a forward row pass carries the update of p
   // the CG step rides on the inverse row pass when the pencil can sum p.Ap
   const bool cg_fused = !fused || pencil_reg_ok(P);
+  // Queue k iterations, then 2 at a time.  The host checks the state that the
+  // control kernels publish to mapped memory at an event after the
+  // second-to-last queued iteration, so the device still runs the last one
+  // while the host decides: no idle gap when more are needed, and a
+  // finished loop leaves at most one iteration of launches that return at
+  // entry.  The call returns without synchronising the stream.
   int rc = 0, queued = 0, k = std::max(1, chunk);
+  bool done = false;
   while (queued < maxfun) {
-    for (int j = 0; j < k && queued < maxfun; ++j, ++queued) {
+    const int nq = std::min(k, maxfun - queued);
+    const int mark = (nq >= 2 && queued + nq < maxfun) ? nq - 2 : nq - 1;
+    for (int j = 0; j < nq; ++j, ++queued) {
       P->prof_tag = queued;
       if (!fused) {
         ProfScope ps(P, PC_VEC, (long long)P->rs * 3 * N);
@@ -4715,7 +4766,7 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
         long long nbr = 0;
         if ((rc = hvp_cg_impl<T>(P, phi, p, W0, d, r, best, (const TnState*)st, &nbr))) break;
         hipLaunchKernelGGL(k_tn_reduce_ctrl4, dim3(1), dim3(kRedThreads), 0, P->stream, st,
-                           (const double*)P->partials, nbr, (const double*)P->red, rr);
+                           (const double*)P->partials, nbr, (const double*)P->red, rr, hst);
       } else {
         if ((rc = hvp_fast_impl<T>(P, phi, p, W0, Ap, nullptr, (const T*)r, (const TnState*)st))) break;
         {
@@ -4724,26 +4775,36 @@ int tn_direction_impl(ofdft_plan_s* P, const T* phi, const T* res, const T* W0, 
                        (const T*)p, (const T*)Ap, d, r, best, N, P->partials);
         }
         hipLaunchKernelGGL(k_tn_reduce_ctrl, dim3(1), dim3(kRedThreads), 0, P->stream, st,
-                           (const double*)P->partials, (long long)nb, (const double*)P->red, rr);
+                           (const double*)P->partials, (long long)nb, (const double*)P->red, rr, hst);
       }
       if (hipGetLastError() != hipSuccess) {
         rc = fail(OFDFT_EHIP, "kernel launch failed in the TN loop");
         break;
       }
+      if (j == mark && hipEventRecord(P->tn_ev, P->stream) != hipSuccess) {
+        rc = fail(OFDFT_EHIP, "TN loop event");
+        break;
+      }
     }
     if (rc) break;
     P->prof_tag = -1;
-    if (hipMemcpyAsync(h, st, sizeof(TnState), hipMemcpyDeviceToHost, P->stream) != hipSuccess ||
-        hipStreamSynchronize(P->stream) != hipSuccess) {
-      rc = fail(OFDFT_EHIP, "TN state read-back");
+    if (hipEventSynchronize(P->tn_ev) != hipSuccess) {
+      rc = fail(OFDFT_EHIP, "TN loop event wait");
       break;
     }
-    if (h->done) break;
+    if (h->done) {
+      done = true;
+      break;
+    }
     k = 2;
   }
   P->prof_tag = -1;
   P->skip = nullptr;
   if (rc) return rc;
+  // every iteration was queued (maxfun) without the state showing done at a
+  // check point: the last iteration's control ends the loop
+  if (!done && hipStreamSynchronize(P->stream) != hipSuccess) return fail(OFDFT_EHIP, "TN state read-back");
+  std::atomic_thread_fence(std::memory_order_acquire);
   if (!h->done) return fail(OFDFT_ESTATE, "TN loop did not finish");
   if (h->fin) {
     ProfScope ps(P, PC_VEC, (long long)P->rs * 2 * N);
