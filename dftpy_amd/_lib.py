@@ -100,6 +100,9 @@ _SIGS = {
     "ofdft_cg_update": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _dp]),
     "ofdft_dot_vphip": (_i, [_vp, _vp, _vp, _vp, _dp]),
     "ofdft_orth_sums": (_i, [_vp, _vp, _vp, _vp, _dp]),
+    "ofdft_orth_normalize": (_i, [_vp, _vp, _vp, _vp]),
+    "ofdft_hvp_prepare_async": (_i, [_vp, _vp, _vp, _vp, _d, _vp, _vp]),
+    "ofdft_plan_deferred_sums": (_i, [_vp, _dp]),
     "ofdft_convert": (_i, [_vp, _vp, _i, _vp, _ll]),
     "ofdft_vloc_build": (_i, [_vp, _i, ctypes.POINTER(VLine), _i, _dp, ctypes.POINTER(ctypes.c_int), _i, _i, _vp]),
     "ofdft_local_forces": (_i, [_vp, _vp, _dp]),

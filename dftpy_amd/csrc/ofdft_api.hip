@@ -478,6 +478,58 @@ int ofdft_hvp_prepare(ofdft_plan P, const void* phi, const void* V, const void* 
   return reduce_blocks(P, nb, 1, res2);
 }
 
+// mapped host slots of the deferred sums: hred[kDeferRes] the sum res^2 of
+// ofdft_hvp_prepare_async, hred[kDeferOrth ..] the five sums of
+// ofdft_orth_normalize (the other reductions use hred[0 .. 9])
+constexpr int kDeferRes = 10;
+constexpr int kDeferOrth = 11;
+static_assert(kDeferOrth + 5 <= kMaxSum, "deferred sums exceed the mapped slots");
+
+int ofdft_hvp_prepare_async(ofdft_plan P, const void* phi, const void* V, const void* aux, double mu, void* res,
+                            void* W0) {
+  int rc = ofdft_hvp_prepare(P, phi, V, aux, mu, res, W0, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_reduce_partials<1>), dim3(1), dim3(kRedThreads), 0, P->stream, P->partials,
+                     (long long)vec_blocks(P->N), P->red + kDeferRes, P->dhred + kDeferRes, (const int*)nullptr);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ofdft_orth_normalize(ofdft_plan P, void* p, const void* phi, const void* V) {
+  if (!valid(P) || !p || !phi || !V) return fail(OFDFT_EINVAL, "null argument");
+  const int nb = vec_blocks(P->N);
+  if ((long long)nb * 5 > P->npartials) return fail(OFDFT_ESTATE, "partials buffer too small");
+  double* coef = P->red + 12;  // after the five sums in red[0..4]
+  if (P->prec == OFDFT_F64)
+    hipLaunchKernelGGL((k_orth_sums<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)p, (const double*)phi,
+                       (const double*)V, P->N, P->partials);
+  else
+    hipLaunchKernelGGL((k_orth_sums<float>), dim3(nb), dim3(256), 0, P->stream, (const float*)p, (const float*)phi,
+                       (const float*)V, P->N, P->partials);
+  LAUNCHCHK();
+  hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb,
+                     P->red, P->dhred + kDeferOrth, (const int*)nullptr);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_orth_coef, dim3(1), dim3(1), 0, P->stream, (const double*)P->red, P->dV, coef);
+  LAUNCHCHK();
+  if (P->prec == OFDFT_F64)
+    hipLaunchKernelGGL((k_lincomb_coef<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)coef,
+                       (const double*)p, (const double*)phi, (double*)p, P->N);
+  else
+    hipLaunchKernelGGL((k_lincomb_coef<float>), dim3(nb), dim3(256), 0, P->stream, (const double*)coef,
+                       (const float*)p, (const float*)phi, (float*)p, P->N);
+  LAUNCHCHK();
+  return 0;
+}
+
+int ofdft_plan_deferred_sums(ofdft_plan P, double* out6) {
+  if (!valid(P) || !out6) return fail(OFDFT_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(P->stream));
+  out6[0] = P->hred[kDeferRes];
+  for (int i = 0; i < 5; ++i) out6[1 + i] = P->hred[kDeferOrth + i];
+  return 0;
+}
+
 int ofdft_hvp_fast(ofdft_plan P, const void* phi, const void* p, const void* W0, void* Ap, double* pAp) {
   if (!valid(P) || !phi || !p || !W0 || !Ap) return fail(OFDFT_EINVAL, "null argument");
   if (P->prec == OFDFT_F64)

@@ -2531,6 +2531,28 @@ __global__ void k_orth_sums(const T* __restrict__ p, const T* __restrict__ phi, 
   block_sums<5>(acc, partials + (long long)blockIdx.x * 5);
 }
 
+// OrthogonalNormalization's factor f = <p,phi>/<phi,phi> and scale
+// s = sqrt(Ne / |p - f phi|^2) (optimization.py:205-226) from the five sums
+// in red[0..4], with the host's operation order: coef = {s, -s f}.
+__global__ void k_orth_coef(const double* __restrict__ red, double dV, double* __restrict__ coef) {
+  if ((threadIdx.x | blockIdx.x) != 0) return;
+  const double pphi = red[0], phiphi = red[1], pp = red[2];
+  const double ne = phiphi * dV;
+  const double factor = pphi / phiphi;
+  const double pn = (pp - factor * pphi) * dV;
+  const double sc = sqrt(ne / pn);
+  coef[0] = sc;
+  coef[1] = -sc * factor;
+}
+
+// z = coef[0] x + coef[1] y with the coefficients read on the device
+template <typename T>
+__global__ void k_lincomb_coef(const double* __restrict__ coef, const T* x, const T* y, T* z, long long N) {
+  const T alpha = T(coef[0]), beta = T(coef[1]);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
+    z[i] = alpha * x[i] + beta * y[i];
+}
+
 template <typename T>
 __global__ void k_dot_vphip(const T* __restrict__ V, const T* __restrict__ phi, const T* __restrict__ p, long long N,
                             double* __restrict__ partials) {

@@ -417,30 +417,62 @@ class Optimization:
         return 1
 
     # --- orthogonalisation and line search -----------------------------
-    def _orth_norm(self, W):
+    @staticmethod
+    def _orth_from_sums(dV, sums):
         """OrthogonalNormalization (optimization.py:205-226) from one pass of
         sums: p' = s (p - f phi) with f = <p,phi>/<phi,phi>, |p - f phi|^2 =
-        <p,p> - f <p,phi>, s = sqrt(Ne/|p - f phi|^2).  Also returns
-        sum V phi p' (the derivative at theta = 0 over 2 dV, :370-380)."""
-        P = W.P
-        dV = P.dV
-        pphi, phiphi, pp, vp, vphi = P.orth_sums(W.p, W.phi, W.V)
+        <p,p> - f <p,phi>, s = sqrt(Ne/|p - f phi|^2).  Returns theta0, s, f
+        and sum V phi p' (the derivative at theta = 0 over 2 dV, :370-380)."""
+        pphi, phiphi, pp, vp, vphi = sums
         Ne = phiphi * dV
         factor = pphi / phiphi
         pn = (pp - factor * pphi) * dV
         theta = math.sqrt(pn / Ne)
         s = math.sqrt(Ne / pn)
-        P.lincomb(s, W.p, -s * factor, W.phi, W.p)
-        return theta, s * (vp - factor * vphi)
+        return theta, s, factor, s * (vp - factor * vphi)
 
-    def _line_search(self, W, E0, g0, theta, maxiter=100):
+    def _orth_norm(self, W):
+        P = W.P
+        theta, s, factor, vphip = self._orth_from_sums(P.dV, P.orth_sums(W.p, W.phi, W.V))
+        P.lincomb(s, W.p, -s * factor, W.phi, W.p)
+        return theta, vphip
+
+    def _deferred(self):
+        """The exact-HVP TN loop on device terms with the unit first trial:
+        the orthonormalisation, the first line-search trial and the residual
+        norm are queued without waiting for their sums."""
+        o = self.optimization_options
+        return (self.optimization_method == "TN" and o["hvp"] == "exact" and not self.host_terms
+                and o["ls_first_step"] != "theta")
+
+    def _flush_res(self, st, raw=None):
+        """Complete the log line of the step whose residual norm was left on
+        the device (hvp_prepare_async): raw = its sum res^2 when the caller
+        already has it from the same plan's deferred_sums()."""
+        pend = st.res_pending
+        if pend is None:
+            return
+        st.res_pending = None
+        P, entry = pend
+        if raw is None:
+            raw = P.deferred_sums()[0]
+        st.resN = _finite(raw * P.dV, "residual norm")
+        entry["dP"] = st.resN
+        self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
+            entry["step"], entry["energy"], entry["dE"], st.resN, entry["Nd"], entry["Nls"], entry["time"]))
+
+    def _line_search(self, W, E0, g0, theta, maxiter=100, first=None):
+        """first = (t, sums): the first trial, already evaluated."""
         o = self.optimization_options
         cache = {}
         count = [0]
 
         def phi_(t):
             count[0] += 1
-            sums = self._eval(W, W.phi, W.p, t)
+            if first is not None and count[0] == 1 and t == first[0]:
+                sums = first[1]
+            else:
+                sums = self._eval(W, W.phi, W.p, t)
             cache["t"], cache["sums"] = t, sums
             return self._energy(W, sums)
 
@@ -450,6 +482,8 @@ class Optimization:
             return float(cache["sums"][L.S_DTHETA])
 
         a1 = theta if o["ls_first_step"] == "theta" else 1.0
+        if first is not None and first[0] != a1:
+            first = None
         if not 0 < o["c1"] < o["c2"] < 1:
             # scipy's scalar_search_wolfe1 check (the reference's call path)
             raise ValueError("'c1' and 'c2' do not satisfy '0 < c1 < c2 < 1'.")
@@ -477,8 +511,15 @@ class Optimization:
         ev.P.set_cg_spectra(self.optimization_options["cg_spectra"])
         return self._works[key]
 
-    def _residual(self, W, mu):
-        """(V - mu) phi into W.res; with the exact TN also the HVP diagonal."""
+    def _residual(self, W, mu, defer=False):
+        """(V - mu) phi into W.res; with the exact TN also the HVP diagonal.
+        defer: the norm is left on the device (None returned; the plan's
+        deferred_sums() gives it)."""
+        if defer:
+            if W.W0 is None:
+                W.W0 = W.e()
+            W.P.hvp_prepare_async(W.phi, W.V, W.aux, mu, W.res, W.W0)
+            return None
         if self.optimization_method == "TN" and self.optimization_options["hvp"] == "exact":
             if W.W0 is None:
                 W.W0 = W.e()
@@ -505,6 +546,7 @@ class Optimization:
             if self._advance(st, it, nd):
                 break
         else:
+            self._flush_res(st)
             self._print("!WARN: Not converged, but reached max steps")
         return self._end()
 
@@ -539,16 +581,20 @@ class Optimization:
             w.P.bind_stream(True)
         st.t0 = time.perf_counter()
         with torch._C.DisableTorchFunctionSubclass():
+            # device scalars throughout: no host synchronisation before the
+            # first evaluation
             if guess_phi is None:
                 rho = self.rho.to(torch.float64)
                 phi = torch.where(rho > 0, torch.sqrt(torch.where(rho > 0, rho, torch.ones_like(rho))),
                                   torch.full_like(rho, 1e-300))
-                phi = phi * math.sqrt(float(rho.sum()) / float((phi * phi).sum()))
+                phi = phi * torch.sqrt(rho.sum() / (phi * phi).sum())
             else:
                 phi = guess_phi.to(torch.float64)
             W.phi.copy_(phi.to(W.P.dtype))
-            # electron count fixed in FP64 from the initial phi (optimization.py:305-315)
-            self._nelec = float((phi * phi).sum()) * W.P.dV
+            # electron count fixed in FP64 from the initial phi (optimization.py:305-315);
+            # read on the host only where the FP64 constraint needs it
+            self._nelec_sum = (phi * phi).sum()
+            self._nelec = None
         st.E, st.mu, st.resN, st.sums = self._start(W)
         _finite(st.E, "energy")
         _finite(st.resN, "residual norm")
@@ -561,6 +607,7 @@ class Optimization:
                                                                                    self.log[-1]["time"]))
         st.theta = 0.1
         st.converged = 2
+        st.res_pending = None
         self._lbfgs = dict(s=[], y=[], rho=[])
         self._dir_prev = None
         st.first = True
@@ -589,18 +636,34 @@ class Optimization:
         o = self.optimization_options
         m = self.optimization_method
         W = st.W
-        theta0, vphip = self._orth_norm(W)
+        defer = self._deferred()
+        first = None
+        if defer:
+            # orthonormalise on the device and evaluate the first trial
+            # (theta = 1) behind it; one synchronisation returns the trial's
+            # sums, the orthonormalisation sums and the previous residual
+            W.P.orth_normalize(W.p, W.phi, W.V)
+            first = (1.0, self._eval(W, W.phi, W.p, 1.0))
+            d6 = W.P.deferred_sums()
+            if st.res_pending is not None and st.res_pending[0] is W.P:
+                self._flush_res(st, d6[0])
+            theta0, _, _, vphip = self._orth_from_sums(W.P.dV, d6[1:])
+        else:
+            theta0, vphip = self._orth_norm(W)
         if vphip * 2.0 > 0:
             self._print("!WARN: Change to steepest decent")
             W.P.lincomb(-1.0, W.res, 0.0, W.res, W.p)
             theta0, vphip = self._orth_norm(W)
+            first = None
         st.theta = min(theta0, st.theta)
         g0 = vphip * 2.0 * W.P.dV
         # an FP32 line search that cannot converge is cut short: the outer
         # loop then continues in FP64
-        stp, nls, cache = self._line_search(W, st.E, g0, st.theta, 8 if (st.phase1 and st.cur == "fp32") else 100)
+        stp, nls, cache = self._line_search(W, st.E, g0, st.theta, 8 if (st.phase1 and st.cur == "fp32") else 100,
+                                            first)
         failed = stp is None
         if failed and not (st.phase1 and st.cur == "fp32"):
+            self._flush_res(st)
             st.converged = 1
             self._print("!!!ERROR : Line-Search Failed!!!")
             self._print("!!!ERROR : Density Optimization NOT Converged  !!!")
@@ -618,7 +681,10 @@ class Optimization:
             Enew = _finite(self._energy(W, sums), "energy")
             st.mu = sums[L.S_VRHO] / sums[L.S_RHO]
             W.res_prev.copy_(W.res)
-            st.resN = _finite(self._residual(W, st.mu), "residual norm")
+            self._flush_res(st)
+            st.resN = self._residual(W, st.mu, defer=defer)
+            if st.resN is not None:
+                _finite(st.resN, "residual norm")
             if m == "LBFGS":
                 self._lbfgs_update(W, old_phi)
             if m.startswith("CG"):
@@ -631,11 +697,16 @@ class Optimization:
             hist = st.hist
             self.log.append(dict(step=it, energy=st.E, dE=hist[-1] - hist[-2], dP=st.resN, Nd=nd, Nls=nls,
                                  time=time.perf_counter() - st.t0, prec=st.cur))
-            self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
-                it, st.E, hist[-1] - hist[-2], st.resN, nd, nls, self.log[-1]["time"]))
+            if st.resN is None:
+                # printed once the residual norm is read (_flush_res)
+                st.res_pending = (W.P, self.log[-1])
+            else:
+                self._print("{:<8d}{:<24.12E}{:<16.6E}{:<16.6E}{:<8d}{:<8d}{:<16.6E}".format(
+                    it, st.E, hist[-1] - hist[-2], st.resN, nd, nls, self.log[-1]["time"]))
         # mixed precision: leave FP32 near its noise floor
         if st.phase1 and st.cur == "fp32" and (failed or (len(st.hist) > 1 and
                                                           abs(st.hist[-1] - st.hist[-2]) < st.switch)):
+            self._flush_res(st)
             W32 = W
             st.cur = "fp64"
             st.W = W = self._work("fp64")
@@ -644,6 +715,8 @@ class Optimization:
             W.P.convert_from(W32.phi, W.phi)
             # FP64 electron-count constraint: FP32 rotations drift the norm
             nphi = W.P.dot3(W.phi, W.phi)[1] * W.P.dV
+            if self._nelec is None:
+                self._nelec = float(self._nelec_sum) * W.P.dV
             W.P.lincomb(math.sqrt(self._nelec / nphi), W.phi, 0.0, W.phi, W.phi)
             E32 = st.hist[-1]
             st.E, st.mu, st.resN, st.sums = self._start(W)
@@ -659,6 +732,7 @@ class Optimization:
             self._print("#### FP32 -> FP64 at step %d, E = %.12E" % (it, st.E))
             return False
         if self._check_converge(st.hist):
+            self._flush_res(st)
             st.converged = 0
             self._print("#### Density Optimization Converged ####")
             return True
@@ -666,6 +740,7 @@ class Optimization:
 
     def _end(self):
         st = self._st
+        self._flush_res(st)
         for w in self._works.values():
             w.P.bind_stream(False)
         self.mu = st.mu

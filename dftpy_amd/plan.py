@@ -286,6 +286,28 @@ class Plan:
         L.check(self.lib.ofdft_orth_sums(self._h, self._real(p, "p"), self._real(phi, "phi"), self._real(V, "V"), out))
         return tuple(out[:5])
 
+    def orth_normalize(self, p, phi, V):
+        """p = s (p - f phi) on the device (ofdft_orth_normalize); the five
+        sums come back with the next deferred_sums().  No synchronisation."""
+        self._stream()
+        L.check(self.lib.ofdft_orth_normalize(self._h, self._real(p, "p"), self._real(phi, "phi"),
+                                              self._real(V, "V")))
+
+    def hvp_prepare_async(self, phi, V, aux, mu, res, W0):
+        """hvp_prepare whose sum res^2 comes back with the next deferred_sums()."""
+        self._stream()
+        auxp = self._realn(aux, "aux", 2 * self.N) if aux is not None else None
+        L.check(self.lib.ofdft_hvp_prepare_async(self._h, self._real(phi, "phi"), self._real(V, "V"), auxp,
+                                                 float(mu), self._real(res, "res"), self._real(W0, "W0")))
+
+    def deferred_sums(self):
+        """(sum res^2 of the last hvp_prepare_async, the five sums of the last
+        orth_normalize); synchronises the stream."""
+        out = L.darr(6)
+        self._stream()
+        L.check(self.lib.ofdft_plan_deferred_sums(self._h, out))
+        return tuple(out[:6])
+
     def convert_from(self, src, dst=None):
         """dst (this plan's precision) = src (any float precision), same size."""
         if dst is None:

@@ -474,6 +474,22 @@ int ofdft_dot_vphip(ofdft_plan plan, const void* V, const void* phi, const void*
  * descent test / derivative at theta = 0 (:370-380, :270-277) follow from
  * them for p' = s (p - f phi) without another pass over the fields.      */
 int ofdft_orth_sums(ofdft_plan plan, const void* p, const void* phi, const void* V, double* out5);
+/* OrthogonalNormalization (optimization.py:205-226) done on the device:
+ * p = s (p - f phi) with f = sum p*phi / sum phi^2 and
+ * s = sqrt(sum phi^2 dV / ((sum p^2 - f sum p*phi) dV)), from the five sums
+ * of ofdft_orth_sums, which are kept for ofdft_plan_deferred_sums.  Does
+ * not synchronise: the first line-search trial can be queued behind it.   */
+int ofdft_orth_normalize(ofdft_plan plan, void* p, const void* phi, const void* V);
+/* ofdft_hvp_prepare without waiting for the sum: res and W0 as there, sum
+ * res^2 kept for ofdft_plan_deferred_sums; does not synchronise.          */
+int ofdft_hvp_prepare_async(ofdft_plan plan, const void* phi, const void* V, const void* aux, double mu, void* res,
+                            void* W0);
+/* Synchronises the plan stream and returns the sums the two calls above
+ * kept: out6[0] = sum res^2 of the last ofdft_hvp_prepare_async,
+ * out6[1..5] = the five sums of the last ofdft_orth_normalize (as
+ * ofdft_orth_sums orders them).  The sums of other calls do not overwrite
+ * them.  No counterpart in the reference (its sums are numpy values).     */
+int ofdft_plan_deferred_sums(ofdft_plan plan, double* out6);
 /* precision conversion between plans: dst (dst precision) = src (src precision), N values */
 int ofdft_convert(ofdft_plan dst_plan, void* dst, int src_precision, const void* src, long long n);
 

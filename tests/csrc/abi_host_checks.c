@@ -104,6 +104,9 @@ int main(void) {
   expect_code(ofdft_cg_update(P0, 1.0, NULL, NULL, NULL, NULL, d), OFDFT_EINVAL, "cg_update");
   expect_code(ofdft_dot_vphip(P0, NULL, NULL, NULL, d), OFDFT_EINVAL, "dot_vphip");
   expect_code(ofdft_orth_sums(P0, NULL, NULL, NULL, d), OFDFT_EINVAL, "orth_sums");
+  expect_code(ofdft_orth_normalize(P0, NULL, NULL, NULL), OFDFT_EINVAL, "orth_normalize");
+  expect_code(ofdft_hvp_prepare_async(P0, NULL, NULL, NULL, 0.0, NULL, NULL), OFDFT_EINVAL, "hvp_prepare_async");
+  expect_code(ofdft_plan_deferred_sums(P0, NULL), OFDFT_EINVAL, "plan_deferred_sums");
   expect_code(ofdft_convert(P0, NULL, OFDFT_F64, NULL, 8), OFDFT_EINVAL, "convert");
 
   /* batches */
