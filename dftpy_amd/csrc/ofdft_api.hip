@@ -510,7 +510,7 @@ int ofdft_orth_normalize(ofdft_plan P, void* p, const void* phi, const void* V) 
   hipLaunchKernelGGL((k_reduce_partials<5>), dim3(5), dim3(kRedThreads), 0, P->stream, P->partials, (long long)nb,
                      P->red, P->dhred + kDeferOrth, (const int*)nullptr);
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_orth_coef, dim3(1), dim3(1), 0, P->stream, (const double*)P->red, P->dV, coef);
+  hipLaunchKernelGGL(k_orth_coef, dim3(1), dim3(1), 0, P->stream, (const double*)P->red, coef);
   LAUNCHCHK();
   if (P->prec == OFDFT_F64)
     hipLaunchKernelGGL((k_lincomb_coef<double>), dim3(nb), dim3(256), 0, P->stream, (const double*)coef,

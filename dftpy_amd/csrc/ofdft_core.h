@@ -2086,10 +2086,17 @@ __global__ void k_dot3(const T* __restrict__ a, const T* __restrict__ b, long lo
   block_sums<3>(acc, partials + (long long)blockIdx.x * 3);
 }
 
+// alpha x + beta y with one fixed rounding order (beta y rounded, then one
+// fused multiply-add), whichever kernel forms it
+template <typename T>
+__device__ __forceinline__ T lincomb_point(T alpha, T x, T beta, T y) {
+  return fma(alpha, x, beta * y);
+}
+
 template <typename T>
 __global__ void k_lincomb(T alpha, const T* x, T beta, const T* y, T* z, long long N) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-    z[i] = alpha * x[i] + beta * y[i];
+    z[i] = lincomb_point(alpha, x[i], beta, y[i]);
 }
 
 template <typename T>
@@ -2533,16 +2540,17 @@ __global__ void k_orth_sums(const T* __restrict__ p, const T* __restrict__ phi, 
 
 // OrthogonalNormalization's factor f = <p,phi>/<phi,phi> and scale
 // s = sqrt(Ne / |p - f phi|^2) (optimization.py:205-226) from the five sums
-// in red[0..4], with the host's operation order: coef = {s, -s f}.
-__global__ void k_orth_coef(const double* __restrict__ red, double dV, double* __restrict__ coef) {
+// in red[0..4]: coef = {s, -s f}.  The volume element cancels in Ne / |.|^2
+// and is left out, and every operation is rounded on its own (no
+// contraction), so the host's formulas give the same bits.
+__global__ void k_orth_coef(const double* __restrict__ red, double* __restrict__ coef) {
   if ((threadIdx.x | blockIdx.x) != 0) return;
   const double pphi = red[0], phiphi = red[1], pp = red[2];
-  const double ne = phiphi * dV;
-  const double factor = pphi / phiphi;
-  const double pn = (pp - factor * pphi) * dV;
-  const double sc = sqrt(ne / pn);
+  const double factor = __ddiv_rn(pphi, phiphi);
+  const double pn = __dsub_rn(pp, __dmul_rn(factor, pphi));
+  const double sc = __dsqrt_rn(__ddiv_rn(phiphi, pn));
   coef[0] = sc;
-  coef[1] = -sc * factor;
+  coef[1] = __dmul_rn(-sc, factor);
 }
 
 // z = coef[0] x + coef[1] y with the coefficients read on the device
@@ -2550,7 +2558,7 @@ template <typename T>
 __global__ void k_lincomb_coef(const double* __restrict__ coef, const T* x, const T* y, T* z, long long N) {
   const T alpha = T(coef[0]), beta = T(coef[1]);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long long)gridDim.x * blockDim.x)
-    z[i] = alpha * x[i] + beta * y[i];
+    z[i] = lincomb_point(alpha, x[i], beta, y[i]);
 }
 
 template <typename T>

@@ -26,7 +26,14 @@ Options beyond the reference's:
                  'theta' (first trial = the running theta)
   tn_graph       True (default): the device TN loop replays one captured
                  hipGraph per CG iteration instead of launching its kernels
-                 (same results; about 3 % faster at 192^3); False: launches
+                 (same results; launch mode is as fast at 192^3); False:
+                 launches
+  defer_sums     True (default; exact-HVP TN on device terms with the unit
+                 first step): the orthonormalisation runs on the device with
+                 the first line-search trial queued behind it, and the
+                 residual norm is read with the next orthonormalisation sums
+                 (each step's log line is completed then); same results as
+                 False, which waits for each sum
   cg_spectra     'plan' (default): the exact-HVP inner CG keeps its spectra
                  in the work precision, in split rows (the Nyquist column
                  apart) where the lengths allow; 'bf16': FP32 inner CG loops
@@ -183,7 +190,7 @@ class Optimization:
                     "econv": 1.0e-5, "c1": 1e-4, "c2": 0.2, "algorithm": "EMM", "vector": "Orthogonalization",
                     "ncheck": 2, "h0": 1.0, "hvp": "exact", "precision": None, "mixed_switch": None,
                     "tn_graph": True, "cg_spectra": "plan",
-                    "ls_first_step": "unit"}
+                    "ls_first_step": "unit", "defer_sums": True}
         for k, v in defaults.items():
             self.optimization_options.setdefault(k, v)
         if self.optimization_options["cg_spectra"] not in ("plan", "bf16", "padded"):
@@ -418,22 +425,23 @@ class Optimization:
 
     # --- orthogonalisation and line search -----------------------------
     @staticmethod
-    def _orth_from_sums(dV, sums):
+    def _orth_from_sums(sums):
         """OrthogonalNormalization (optimization.py:205-226) from one pass of
         sums: p' = s (p - f phi) with f = <p,phi>/<phi,phi>, |p - f phi|^2 =
-        <p,p> - f <p,phi>, s = sqrt(Ne/|p - f phi|^2).  Returns theta0, s, f
-        and sum V phi p' (the derivative at theta = 0 over 2 dV, :370-380)."""
+        <p,p> - f <p,phi>, s = sqrt(Ne/|p - f phi|^2) (the volume element of
+        Ne and of the norm cancels and is left out: ofdft_orth_normalize
+        forms the same bits on the device).  Returns theta0, s, f and
+        sum V phi p' (the derivative at theta = 0 over 2 dV, :370-380)."""
         pphi, phiphi, pp, vp, vphi = sums
-        Ne = phiphi * dV
         factor = pphi / phiphi
-        pn = (pp - factor * pphi) * dV
-        theta = math.sqrt(pn / Ne)
-        s = math.sqrt(Ne / pn)
+        pn = pp - factor * pphi
+        theta = math.sqrt(pn / phiphi)
+        s = math.sqrt(phiphi / pn)
         return theta, s, factor, s * (vp - factor * vphi)
 
     def _orth_norm(self, W):
         P = W.P
-        theta, s, factor, vphip = self._orth_from_sums(P.dV, P.orth_sums(W.p, W.phi, W.V))
+        theta, s, factor, vphip = self._orth_from_sums(P.orth_sums(W.p, W.phi, W.V))
         P.lincomb(s, W.p, -s * factor, W.phi, W.p)
         return theta, vphip
 
@@ -442,8 +450,8 @@ class Optimization:
         the orthonormalisation, the first line-search trial and the residual
         norm are queued without waiting for their sums."""
         o = self.optimization_options
-        return (self.optimization_method == "TN" and o["hvp"] == "exact" and not self.host_terms
-                and o["ls_first_step"] != "theta")
+        return (bool(o["defer_sums"]) and self.optimization_method == "TN" and o["hvp"] == "exact"
+                and not self.host_terms and o["ls_first_step"] != "theta")
 
     def _flush_res(self, st, raw=None):
         """Complete the log line of the step whose residual norm was left on
@@ -647,7 +655,7 @@ class Optimization:
             d6 = W.P.deferred_sums()
             if st.res_pending is not None and st.res_pending[0] is W.P:
                 self._flush_res(st, d6[0])
-            theta0, _, _, vphip = self._orth_from_sums(W.P.dV, d6[1:])
+            theta0, _, _, vphip = self._orth_from_sums(d6[1:])
         else:
             theta0, vphip = self._orth_norm(W)
         if vphip * 2.0 > 0:

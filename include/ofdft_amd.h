@@ -476,7 +476,8 @@ int ofdft_dot_vphip(ofdft_plan plan, const void* V, const void* phi, const void*
 int ofdft_orth_sums(ofdft_plan plan, const void* p, const void* phi, const void* V, double* out5);
 /* OrthogonalNormalization (optimization.py:205-226) done on the device:
  * p = s (p - f phi) with f = sum p*phi / sum phi^2 and
- * s = sqrt(sum phi^2 dV / ((sum p^2 - f sum p*phi) dV)), from the five sums
+ * s = sqrt(sum phi^2 / (sum p^2 - f sum p*phi)) (the reference's dV factors
+ * cancel and are left out; each operation rounded once), from the five sums
  * of ofdft_orth_sums, which are kept for ofdft_plan_deferred_sums.  Does
  * not synchronise: the first line-search trial can be queued behind it.   */
 int ofdft_orth_normalize(ofdft_plan plan, void* p, const void* phi, const void* V);

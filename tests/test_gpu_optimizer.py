@@ -179,3 +179,35 @@ def test_tn_graph_replays_give_the_same_solve(prec):
     assert res[True].energy == res[False].energy
     assert [l["Nd"] for l in res[True].log] == [l["Nd"] for l in res[False].log]
     assert [l["energy"] for l in res[True].log] == [l["energy"] for l in res[False].log]
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_deferred_sums_give_the_same_solve(prec, capsys):
+    """The orthonormalisation on the device with the first line-search trial
+    queued behind it, and the residual norm read one step later
+    (defer_sums=True), reproduce the solve that waits for each sum: the same
+    energies, residual norms, inner and line-search counts at every step, and
+    the same printed log lines."""
+    print("case deferred sums al108_96", prec, flush=True)
+    wl = W.build("al108_96")
+    grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
+    ions = D.Ions(wl["symbols"], wl["positions"], wl["cell"])
+    pp = D.LocalPseudo(grid=grid, ions=ions, PP_list={"Al": D.empty_core(3.0, 1.12)})
+    ev = D.TotalFunctional(KEDF=D.KEDF("WT"), XC=D.XC("LDA"), HARTREE=D.Hartree(), PSEUDO=pp)
+    res, out = {}, {}
+    for defer in (False, True):
+        opt = D.Optimization(EnergyEvaluator=ev, optimization_method="TN",
+                             optimization_options={"econv": 1e-9, "precision": prec, "defer_sums": defer,
+                                                   "mixed_phase1": True})
+        capsys.readouterr()
+        opt.optimize_rho(guess_rho=D.DirectField(grid=grid, data=wl["rho_ini"]))
+        # the time column aside, the printed lines must agree
+        out[defer] = [ln.split()[:6] for ln in capsys.readouterr().out.splitlines() if ln[:1].isdigit()]
+        res[defer] = opt
+        assert opt.converged == 0
+    a, b = res[False], res[True]
+    assert b.energy == a.energy
+    for key in ("energy", "dE", "dP", "Nd", "Nls", "prec"):
+        assert [l[key] for l in b.log] == [l[key] for l in a.log], key
+    assert out[True] == out[False] and len(out[True]) == len(a.log) - (1 if prec == "mixed" else 0)
+    assert b.nfeval == a.nfeval
