@@ -1,5 +1,5 @@
-"""The reference-side ctypes binding of the integration notes (INTEGRATION.md,
-section 4), run as written against the built library.  Its
+"""The reference-side ctypes binding of the integration notes (the first
+code block of their section 4), run as written against the built library.  Its
 GPUTotalFunctional (WT + TF + vW + LDA + Hartree + local PP through
 ofdft_plan_create / ofdft_set_functional / ofdft_set_vloc / ofdft_eval)
 must give the oracle's energy and potential on the same density.
@@ -39,7 +39,7 @@ def _stub_source():
 
 
 def test_integration_stub_matches_oracle(monkeypatch):
-    print("case INTEGRATION.md stub, al4_32", flush=True)
+    print("case integration-notes stub, al4_32", flush=True)
     names = {"dftpy": {}, "dftpy.functional": {},
              "dftpy.functional.abstract_functional": {"AbstractFunctional": D.AbstractFunctional},
              "dftpy.functional.functional_output": {"FunctionalOutput": D.FunctionalOutput},
@@ -53,7 +53,7 @@ def test_integration_stub_matches_oracle(monkeypatch):
     monkeypatch.setattr(ctypes, "CDLL",
                         lambda name, *a, **k: real_cdll(L.LIB_PATH if name == "libofdft_amd.so" else name, *a, **k))
     ns = {"__name__": "gpu_total"}
-    exec(compile(_stub_source(), "INTEGRATION.md (section 4)", "exec"), ns)
+    exec(compile(_stub_source(), "integration-notes-stub", "exec"), ns)
 
     wl = W.build("al4_32")
     grid = D.DirectGrid(wl["cell"], nr=wl["nr"], full=False)
